@@ -1,0 +1,259 @@
+#!/usr/bin/env python3
+"""Benchmark of the short-range MD step on one MI355X.
+
+Workload: Lennard-Jones fluid, N = 4,194,304 (128^3 jittered simple-cubic
+lattice), density 0.8442, eps = sigma = 1, rc = 2.5 (shift auto), skin 0.3,
+dt = 0.005.  Warm-up: W Langevin steps at T = 0.72, gamma = 1 (untimed), then
+K timed NVE steps.  A step is one velocity-Verlet step: integrate1, skin check,
+rebuild when triggered, pair forces, integrate2.
+
+Prints one JSON line: particle-steps/s of the mixed-precision path (value),
+the all-FP64 GPU path on the same state, force accuracy of the mixed path
+against the FP64 path, NVE energy drift, the pair-force kernel's HBM roofline
+fraction (HIP events on the library's stream) and a CPU baseline (the FP64
+oracle, OpenMP over the host cores, on a bounded sample of the same system).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def lattice(n, rho, seed, jitter=0.05):
+    L = (n / rho) ** (1.0 / 3.0)
+    a = int(round(n ** (1.0 / 3.0)))
+    if a ** 3 < n:
+        a += 1
+    sp = L / a
+    g = np.arange(a, dtype=np.float64)
+    ii, jj, kk = np.meshgrid(g, g, g, indexing="ij")
+    sites = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1)[:n]
+    rng = np.random.default_rng(seed)
+    return (sites + 0.5) * sp + rng.uniform(-jitter, jitter, size=(n, 3)), (L, L, L)
+
+
+def velocities(n, T, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(0.0, math.sqrt(T), size=(n, 3))
+    v -= v.mean(axis=0)
+    v *= math.sqrt(T / (np.sum(v * v) / (3 * n)))
+    return v
+
+
+def make_system(espressopp, box, cells, rc, skin, precision, pos, vel):
+    system = espressopp.System(precision=precision)
+    system.rng = espressopp.esutil.RNG(2024)
+    system.bc = espressopp.bc.OrthorhombicBC(system.rng, box)
+    system.skin = skin
+    system.storage = espressopp.storage.DomainDecomposition(system, (1, 1, 1), cells)
+    n = len(pos)
+    system.storage.addParticlesArray(np.arange(n, dtype=np.int32), pos, None, vel)
+    system.storage.decompose()
+    vl = espressopp.VerletList(system, cutoff=rc)
+    lj = espressopp.interaction.VerletListLennardJones(vl)
+    lj.setPotential(0, 0, espressopp.interaction.LennardJones(1.0, 1.0, rc, "auto"))
+    system.addInteraction(lj)
+    integ = espressopp.integrator.VelocityVerlet(system)
+    integ.dt = 0.005
+    return system, vl, lj, integ
+
+
+def total_energy(system, lj, n):
+    T = system.storage._h.out_dbl("espp_kinetic")
+    return lj.computeEnergy() + 0.5 * T
+
+
+def timed_run(espressopp, system, integ, steps):
+    lib = espressopp._lib.load()
+    espressopp._lib.check(lib.espp_device_synchronize())
+    t0 = time.perf_counter()
+    integ.run(steps)
+    espressopp._lib.check(lib.espp_device_synchronize())
+    return time.perf_counter() - t0
+
+
+def kernel_stats(h, name):
+    import ctypes
+    from espressopp_amd import _lib
+    cnt, ms = ctypes.c_int64(0), ctypes.c_double(0.0)
+    _lib.check(h.lib.espp_prof_stats(h.h, _lib.KERNEL[name], ctypes.byref(cnt), ctypes.byref(ms)))
+    return cnt.value, ms.value
+
+
+def vl_entries(h, vl):
+    import ctypes
+    from espressopp_amd import _lib
+    e, cap, mx = ctypes.c_int64(0), ctypes.c_int32(0), ctypes.c_int32(0)
+    _lib.check(h.lib.espp_vl_stats(h.h, vl._id, ctypes.byref(e), ctypes.byref(cap), ctypes.byref(mx)))
+    return e.value, cap.value, mx.value
+
+
+def cpu_baseline(pos, vel, box, cells, rc, skin, steps, threads):
+    from oracle.oracle import OracleSim
+    n = len(pos)
+    s = OracleSim(box, cells, skin, threads=threads)
+    s.add_particles(np.arange(n, dtype=np.int32), pos, vel=vel)
+    s.decompose()
+    ia = s.lj_create(s.vl_create(rc))
+    s.lj_set(ia, 0, 0, 1.0, 1.0, rc, "auto")
+    s.run(0, dt=0.005)                      # list build and first forces, untimed
+    t0 = time.perf_counter()
+    s.run(steps)
+    dt = time.perf_counter() - t0
+    return n * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--n", type=int, default=128 ** 3)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the short-range MD path here runs on a single MI355X "
+              "(--gpus %d requested). Multi-GPU domain decomposition is not built." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import espressopp_amd as espressopp
+
+    if espressopp._lib.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    n, rho, rc, skin = args.n, 0.8442, 2.5, 0.3
+    pos, box = lattice(n, rho, seed=2024)
+    vel = velocities(n, 0.72, seed=2025)
+    cells = [int(b / (rc + skin)) for b in box]
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+
+    # mixed path: Langevin equilibration (untimed), then timed NVE
+    t_setup = time.perf_counter()
+    system, vl, lj, integ = make_system(espressopp, box, cells, rc, skin, "mixed", pos, vel)
+    log("setup mixed: %.2fs" % (time.perf_counter() - t_setup))
+    lang = espressopp.integrator.LangevinThermostat(system)
+    lang.gamma = 1.0
+    lang.temperature = 0.72
+    integ.addExtension(lang)
+    t = time.perf_counter()
+    integ.run(args.warmup)
+    log("warmup %d Langevin steps: %.2fs" % (args.warmup, time.perf_counter() - t))
+    lang.disconnect()
+    h = system.storage._h
+    state = system.storage.getParticleArrays()
+
+    # force accuracy of the mixed path against the FP64 path on the same state
+    err = None
+    fp64 = None
+    if not args.no_fp64:
+        sys64, vl64, lj64, integ64 = make_system(espressopp, box, cells, rc, skin, "fp64",
+                                                 state["pos"] + state["image"] * np.asarray(box), state["v"])
+        h.call("espp_compute_forces")
+        sys64.storage._h.call("espp_compute_forces")
+        f32 = system.storage.getParticleArrays()["f"]
+        f64 = sys64.storage.getParticleArrays()["f"]
+        nref = np.linalg.norm(f64, axis=1)
+        floor = 0.05 * math.sqrt(np.mean(nref ** 2))
+        rel = np.linalg.norm(f32 - f64, axis=1) / np.maximum(nref, floor)
+        e32, e64 = lj.computeEnergy(), lj64.computeEnergy()
+        w32, w64 = lj.computeVirial(), lj64.computeVirial()
+        err = dict(force_rel_max=float(rel.max()), force_rel_p999=float(np.quantile(rel, 0.999)),
+                   force_rel_median=float(np.median(rel)), force_floor=float(floor),
+                   energy_rel=abs(e32 - e64) / abs(e64), virial_rel=abs(w32 - w64) / abs(w64))
+        log("accuracy:", err)
+        del f32, f64
+
+    e0 = total_energy(system, lj, n)
+    h.call("espp_prof_reset")
+    h.call("espp_prof_enable", 1)
+    dt_run = timed_run(espressopp, system, integ, args.steps)
+    h.call("espp_prof_enable", 0)
+    e1 = total_energy(system, lj, n)
+    value = n * args.steps / dt_run
+    launches, pair_ms = kernel_stats(h, "pair")
+    entries, cap, maxcount = vl_entries(h, vl)
+    nbar = entries / n
+    b_force = 16 + 4 * nbar + 24
+    avg_s = pair_ms / launches / 1e3
+    achieved = b_force * n / avg_s / 1e9
+    stats = {k: kernel_stats(h, k) for k in ("nbr_build", "integrate1", "integrate2", "binning")}
+    resorts = integ.getNumResorts()
+    log("mixed: %.3f ms/step, %.4g particle-steps/s, pair kernel %.3f ms avg (%d launches), nbar %.2f, "
+        "cap %d, resorts %d" % (dt_run / args.steps * 1e3, value, avg_s * 1e3, launches, nbar, cap, resorts))
+    for k, (c, ms) in stats.items():
+        log("  %-11s %5d launches, %.3f ms avg" % (k, c, ms / max(c, 1)))
+
+    if not args.no_fp64:
+        k64 = args.fp64_steps or args.steps
+        sys64.storage._h.call("espp_prof_reset")
+        sys64.storage._h.call("espp_prof_enable", 1)
+        dt64 = timed_run(espressopp, sys64, integ64, k64)
+        sys64.storage._h.call("espp_prof_enable", 0)
+        l64, ms64 = kernel_stats(sys64.storage._h, "pair")
+        fp64 = dict(value=n * k64 / dt64, ms_per_step=dt64 / k64 * 1e3, pair_kernel_ms=ms64 / max(l64, 1),
+                    steps=k64)
+        log("fp64: %.3f ms/step, %.4g particle-steps/s, pair kernel %.3f ms" %
+            (fp64["ms_per_step"], fp64["value"], fp64["pair_kernel_ms"]))
+        del sys64, vl64, lj64, integ64
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        cval, ctime = cpu_baseline(state["pos"] + state["image"] * np.asarray(box), state["v"], box, cells, rc,
+                                   skin, args.cpu_steps, threads)
+        cpu = {"value": cval, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": "same 4,194,304-particle LJ state after warm-up, %d NVE steps of the FP64 "
+                         "oracle (half Verlet list, Newton-3, OpenMP) = %.1f s" % (args.cpu_steps, ctime)}
+        log("cpu:", cpu)
+
+    drift = (e1 - e0) / n / args.steps * 1e5
+    out = {
+        "metric": "particle-steps/s, LJ 4M: mixed vs FP64 GPU path at force rel err <=1e-5",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_run / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-pair/fp64-integrate",
+        "data": "synthetic: jittered simple-cubic lattice, Maxwell-Boltzmann velocities, Langevin warm-up",
+        "config": {"workload": "LJ fluid N=%d rho=0.8442 rc=2.5 skin=0.3 dt=0.005 NVE after %d-step "
+                               "Langevin warm-up at T=0.72" % (n, args.warmup),
+                   "n": n, "box": box[0], "cells": cells, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_lj_mixed", "bytes_per_particle": b_force, "nbar": nbar,
+                     "avg_launch_ms": avg_s * 1e3, "launches": launches},
+        "fp64_path": fp64,
+        "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
+        "accuracy": err,
+        "nve_drift_per_particle_per_1e5_steps": drift,
+        "resorts": resorts,
+        "cpu_baseline": cpu,
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "lj_mixed_4M_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            out["roofline"]["traffic"] = json.load(f).get("bytes_per_launch")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

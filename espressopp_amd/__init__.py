@@ -1,0 +1,20 @@
+"""espressopp_amd -- MI355X-native short-range MD step of ESPResSo++.
+
+A drop-in for the reference's Python API on this path: ``import
+espressopp_amd as espressopp`` and the classes System, storage.DomainDecomposition,
+VerletList, FixedPairList, interaction.LennardJones / VerletListLennardJones /
+FENE / FixedPairListFENE, integrator.VelocityVerlet / LangevinThermostat and
+analysis.Temperature / Pressure run the step on one GPU through the C ABI of
+libespp_amd.so (include/espp_amd.h).
+"""
+from . import _lib
+from ._vec3 import Int3D, Real3D, toInt3DFromVector, toReal3DFromVector
+from .system import MPI, FixedPairList, System, VerletList
+from . import analysis, bc, esutil, integrator, interaction, storage, tools
+
+infinity = float("inf")
+nan = float("nan")
+
+__all__ = ["System", "VerletList", "FixedPairList", "Real3D", "Int3D", "MPI", "analysis", "bc",
+           "esutil", "integrator", "interaction", "storage", "tools", "infinity",
+           "toReal3DFromVector", "toInt3DFromVector"]

@@ -1,0 +1,47 @@
+// Shared host/device definitions of the MI355X short-range MD library.
+//
+// Particle storage on the device is cell-sorted ("slot" order): after every
+// decompose the particles of inner cell 0 come first, then cell 1, ... and
+// inside a cell ascending particle id.  Cells use the reference's inner-cell
+// numbering x + nx*(y + ny*z) (src/esutil/Grid.hpp:68-71).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace espp {
+
+constexpr int kWave = 64;          // CDNA wavefront
+constexpr int kMaxTypes = 16;      // particle types with a device parameter table
+
+// Box and cell-grid geometry, passed by value to kernels.
+struct Geom {
+  double L[3], invL[3], halfL[3];
+  int n[3];                        // inner cells per dimension
+  double ics[3];                   // 1 / cellSize, cellSize = L / n (src/storage/CellGrid.cpp:55-56)
+  double qinv[3];                  // 2^32 / L: FP64 position -> 32-bit fixed point
+  float qs[3];                     // L / 2^32: fixed-point difference -> FP32 length
+};
+
+// Lennard-Jones coefficients of one type pair (src/interaction/LennardJones.hpp:79-87).
+struct LJParam {
+  float ff1, ff2, rc2, shift;      // FP32 copies for the mixed kernel
+  double dff1, dff2, drc2, dshift; // FP64 copies
+  double sig2, eps4;               // energy: 4 eps ((sig2/r2)^6 - (sig2/r2)^3)
+  int set;
+  int pad;
+};
+
+struct FENEParam {
+  double K, r0, rMax, rMaxSqr, cutoffSqr, shift;
+};
+
+// Per-call control block kept on the device.
+struct Ctl {
+  unsigned long long maxsq_bits;   // max |dx|^2 of the last integrate1 (non-negative double bits)
+  int overflow;                    // neighbour list capacity exceeded
+  int maxcount;                    // largest neighbour count of the last build
+  int nanpos;                      // a NaN position was met while folding
+  int pad;
+};
+
+}  // namespace espp

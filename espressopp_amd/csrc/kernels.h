@@ -1,0 +1,46 @@
+// Launch wrappers of kernels.hip, used by the host runtime (sim.hip).
+#pragma once
+#include "espp_dev.h"
+
+namespace espp {
+
+void launch_fold_cell(hipStream_t s, int n, double4* pos, int4* img, int* cell_of, int* rank,
+                      int* cell_count, const Geom& g, Ctl* ctl, int do_fold);
+void launch_scan_cells(hipStream_t s, int ncell, const int* count, int* start);
+void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start,
+                    int* order);
+void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const int* id);
+void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, const double4* pos,
+                    const double4* vel, const double* fx, const double* fy, const double* fz,
+                    const int* id, const int4* img, double4* pos2, double4* vel2, double* fx2,
+                    double* fy2, double* fz2, int* id2, int4* img2, int* cell_of_slot,
+                    int* slot_of_id, uint4* posq, const Geom& g);
+void launch_fill_int(hipStream_t s, int n, int* a, int v);
+void launch_reset_ctl(hipStream_t s, Ctl* ctl);
+void launch_nbr_build(hipStream_t s, int n, const double4* pos, const int* id,
+                      const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
+                      const int* ex_start, const int* ex_ids, int* nbr, int* cnt, int cap,
+                      Ctl* ctl);
+int lj_blocks(int n);
+void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n,
+               const uint4* posq, const double4* pos, const int* nbr, const int* cnt, int cap,
+               const Geom& g, const LJParam* prm, int ntypes, double* fx, double* fy, double* fz,
+               double* partial);
+void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, const double4* pos,
+                 const uint4* posq, const int* id, const int* slot_of_id, int nid,
+                 const int* bstart, const int* bpartner, const Geom& g, const FENEParam& fp,
+                 double* fx, double* fy, double* fz, double* partial);
+void launch_zero3(hipStream_t s, int n, double* a, double* b, double* c);
+void launch_thermo_integrate2(hipStream_t s, int n, const int* id, double4* vel, double* fx,
+                              double* fy, double* fz, int lang, double pref1, double pref2,
+                              unsigned long long seed, unsigned long long call,
+                              const unsigned char* excl, int nid, int do_int2, double half_dt);
+void launch_integrate1(hipStream_t s, int n, double4* pos, double4* vel, const double* fx,
+                       const double* fy, const double* fz, double dt, uint4* posq, const Geom& g,
+                       Ctl* ctl);
+int kinetic_blocks(int n);
+void launch_kinetic(hipStream_t s, int n, const double4* vel, const double4* pos,
+                    unsigned long long type_mask, double* partial);
+void launch_reduce_partials(hipStream_t s, int nblk, int nv, const double* partial, double* out);
+
+}  // namespace espp

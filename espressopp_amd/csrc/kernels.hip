@@ -1,0 +1,839 @@
+// HIP kernels of the MI355X short-range MD step (gfx950, wave64).
+//
+//   binning          k_fold_cell, k_scan_cells, k_scatter, k_sort_cells, k_permute
+//   neighbour build  k_nbr_build   (full list, wavefront-padded ragged SoA)
+//   pair forces      k_lj_mixed    (32-bit fixed-point positions, FP32 pair math,
+//                                   FP32 chunks carried into FP64 accumulators)
+//                    k_lj_fp64     (all-FP64 comparison path)
+//   bonds            k_fene        (per-particle bond adjacency, no atomics)
+//   integrator       k_integrate1, k_thermo_integrate2
+//   reductions       k_kinetic, k_reduce_partials
+//
+// Launch wrappers at the bottom are the only entry points used by the host
+// runtime (sim.hip).
+#include "espp_dev.h"
+#include "kernels.h"
+
+namespace espp {
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ uint32_t fixq(double x, double qinv) {
+  // position -> 32-bit fixed point with L <-> 2^32; the int64 detour makes the
+  // mapping modular, so unfolded positions wrap onto the same periodic image.
+  return (uint32_t)(unsigned long long)(long long)__builtin_rint(x * qinv);
+}
+
+__device__ __forceinline__ float rcp_nr(float x) {
+  // v_rcp_f32 followed by one Newton step: ~0.5 ulp, three VALU ops
+  float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// ------------------------------------------------------------------ binning
+
+// Fold every coordinate (OrthorhombicBC::foldCoordinate, src/bc/OrthorhombicBC.cpp:144-161)
+// and map the folded position to its inner cell (CellGrid::mapPositionToCellChecked
+// for one node, src/storage/CellGrid.cpp:98-136).  Cell ranks come from atomics;
+// k_sort_cells makes the order inside a cell deterministic afterwards.
+__global__ void k_fold_cell(int n, double4* __restrict__ pos, int4* __restrict__ img,
+                            int* __restrict__ cell_of, int* __restrict__ rank,
+                            int* __restrict__ cell_count, Geom g, Ctl* ctl, int do_fold) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 p = pos[i];
+  int4 im = img[i];
+  double x[3] = {p.x, p.y, p.z};
+  int iv[3] = {im.x, im.y, im.z};
+  int c[3];
+  bool bad = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!(x[d] == x[d])) { bad = true; c[d] = 0; continue; }
+    if (do_fold) {
+      int tmp = (int)floor(x[d] * g.invL[d]);
+      iv[d] += tmp;
+      x[d] -= (double)tmp * g.L[d];
+    }
+    int cc = (int)((x[d] - 0.0) * g.ics[d]) + 1;
+    if (cc < 1) cc = 1;
+    else if (cc >= g.n[d] + 1) cc = g.n[d];
+    c[d] = cc - 1;
+  }
+  if (bad) ctl->nanpos = 1;
+  p.x = x[0]; p.y = x[1]; p.z = x[2];
+  pos[i] = p;
+  img[i] = make_int4(iv[0], iv[1], iv[2], 0);
+  int cell = c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+  cell_of[i] = cell;
+  rank[i] = atomicAdd(&cell_count[cell], 1);
+}
+
+// Exclusive scan of per-cell counts in one workgroup of 1024 threads: tiles of
+// 4096 cells, int4 loads per thread, wavefront shuffles and an LDS pass over the
+// 16 wave totals, carried from tile to tile.
+__global__ __launch_bounds__(1024) void k_scan_cells(int ncell, const int* __restrict__ count,
+                                                     int* __restrict__ start) {
+  __shared__ int wsum[16];
+  __shared__ int carry_s;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid == 0) carry_s = 0;
+  __syncthreads();
+  for (int base = 0; base < ncell; base += 4096) {
+    int idx = base + tid * 4;
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (idx + k < ncell) ? count[idx + k] : 0;
+    int local = v[0] + v[1] + v[2] + v[3];
+    int incl = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    if (tid < 64) {
+      int w = (tid < 16) ? wsum[tid] : 0;
+      int wi = w;
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        int t = __shfl_up(wi, o, 64);
+        if (tid >= o) wi += t;
+      }
+      if (tid < 16) wsum[tid] = wi - w;   // exclusive wave offsets
+    }
+    __syncthreads();
+    int run = carry_s + wsum[wid] + incl - local;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (idx + k < ncell) start[idx + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (tid == 1023) carry_s = run;
+    __syncthreads();
+  }
+  if (tid == 0) start[ncell] = carry_s;
+}
+
+__global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
+                          const int* __restrict__ start, int* __restrict__ order) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  order[start[cell_of[i]] + rank[i]] = i;
+}
+
+// Inside each cell, order members by particle id (insertion sort; cells hold
+// a few tens of particles at liquid densities).
+__global__ void k_sort_cells(int ncell, const int* __restrict__ start, int* __restrict__ order,
+                             const int* __restrict__ id) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int b = start[c], e = start[c + 1];
+  for (int k = b + 1; k < e; ++k) {
+    int o = order[k];
+    int key = id[o];
+    int m = k - 1;
+    while (m >= b && id[order[m]] > key) {
+      order[m + 1] = order[m];
+      --m;
+    }
+    order[m + 1] = o;
+  }
+}
+
+__global__ void k_permute(int n, const int* __restrict__ order, const int* __restrict__ cell_of,
+                          const double4* __restrict__ pos, const double4* __restrict__ vel,
+                          const double* __restrict__ fx, const double* __restrict__ fy,
+                          const double* __restrict__ fz, const int* __restrict__ id,
+                          const int4* __restrict__ img, double4* __restrict__ pos2,
+                          double4* __restrict__ vel2, double* __restrict__ fx2,
+                          double* __restrict__ fy2, double* __restrict__ fz2,
+                          int* __restrict__ id2, int4* __restrict__ img2,
+                          int* __restrict__ cell_of_slot, int* __restrict__ slot_of_id,
+                          uint4* __restrict__ posq, Geom g) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int o = order[s];
+  double4 p = pos[o];
+  pos2[s] = p;
+  vel2[s] = vel[o];
+  fx2[s] = fx[o];
+  fy2[s] = fy[o];
+  fz2[s] = fz[o];
+  int pid = id[o];
+  id2[s] = pid;
+  img2[s] = img[o];
+  cell_of_slot[s] = cell_of[o];
+  slot_of_id[pid] = s;
+  if (posq)
+    posq[s] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]),
+                         (uint32_t)(int)p.w);
+}
+
+__global__ void k_fill_int(int n, int* __restrict__ a, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_reset_ctl(Ctl* ctl) {
+  if (threadIdx.x == 0) {
+    ctl->maxsq_bits = 0ull;
+    ctl->overflow = 0;
+    ctl->maxcount = 0;
+    ctl->nanpos = 0;
+  }
+}
+
+// ------------------------------------------------------------------ neighbour list
+
+__device__ bool is_excluded(int a, int b, const int* __restrict__ ex_start,
+                            const int* __restrict__ ex_ids) {
+  int lo = ex_start[a], hi = ex_start[a + 1] - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int v = ex_ids[mid];
+    if (v == b) return true;
+    if (v < b) lo = mid + 1; else hi = mid - 1;
+  }
+  return false;
+}
+
+// Full (both-direction) Verlet list.  Membership reproduces the reference's
+// half-list test exactly (src/VerletList.cpp:247-276): a pair is kept iff
+// d.sqr() <= (rc+skin)^2 in FP64 with no contraction, where d is evaluated in
+// the orientation the reference uses.  The reference evaluates a pair from the
+// cell whose framed index is smaller (half shell, src/storage/DomainDecomposition.cpp:354-357),
+// with the second particle's ghost copy at p + shift (src/Particle.hpp:93-98).
+// So for cell offset o of j relative to i: if o is lexicographically positive
+// (z most significant) d = xi - (xj + s L), else d = xj - (xi - s L).
+//
+// Layout: particle i's k-th neighbour is nbr[((i/64)*cap + k)*64 + i%64].
+template <bool HAS_EXCL>
+__global__ __launch_bounds__(256) void k_nbr_build(int n, const double4* __restrict__ pos,
+                                                   const int* __restrict__ id,
+                                                   const int* __restrict__ cell_of_slot,
+                                                   const int* __restrict__ cstart, Geom g,
+                                                   double cutsq, const int* __restrict__ ex_start,
+                                                   const int* __restrict__ ex_ids,
+                                                   int* __restrict__ nbr, int* __restrict__ cnt,
+                                                   int cap, Ctl* ctl) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = 0;
+  if (i < n) {
+    const double4 pi = pos[i];
+    const int c = cell_of_slot[i];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    const int myid = HAS_EXCL ? id[i] : 0;
+    int* out = nbr + (size_t)(i >> 6) * (size_t)cap * 64 + (i & 63);
+    for (int oz = -1; oz <= 1; ++oz) {
+      int qz = cz + oz, sz = 0;
+      if (qz < 0) { qz += nz; sz = -1; } else if (qz >= nz) { qz -= nz; sz = 1; }
+      for (int oy = -1; oy <= 1; ++oy) {
+        int qy = cy + oy, sy = 0;
+        if (qy < 0) { qy += ny; sy = -1; } else if (qy >= ny) { qy -= ny; sy = 1; }
+        for (int ox = -1; ox <= 1; ++ox) {
+          int qx = cx + ox, sx = 0;
+          if (qx < 0) { qx += nx; sx = -1; } else if (qx >= nx) { qx -= nx; sx = 1; }
+          const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
+          const bool unshifted = (sx | sy | sz) == 0;
+          const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
+          const int c2 = qx + nx * (qy + ny * qz);
+          const int jb = cstart[c2], je = cstart[c2 + 1];
+          for (int j = jb; j < je; ++j) {
+            if (j == i && unshifted) continue;
+            const double4 pj = pos[j];
+            double dx, dy, dz;
+            if (fwd) {
+              dx = pi.x - (pj.x + shx);
+              dy = pi.y - (pj.y + shy);
+              dz = pi.z - (pj.z + shz);
+            } else {
+              dx = pj.x - (pi.x - shx);
+              dy = pj.y - (pi.y - shy);
+              dz = pj.z - (pi.z - shz);
+            }
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 <= cutsq) {
+              if (HAS_EXCL && is_excluded(myid, id[j], ex_start, ex_ids)) continue;
+              if (k < cap) out[(size_t)k * 64] = j;
+              ++k;
+            }
+          }
+        }
+      }
+    }
+    cnt[i] = k;
+    if (k > cap) ctl->overflow = 1;
+  }
+  int m = k;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&ctl->maxcount, m);
+}
+
+// ------------------------------------------------------------------ LJ forces
+
+// Energy/virial partials per block: [e, w, wxx, wyy, wzz, wxy, wxz, wyz]
+constexpr int kNPart = 8;
+
+template <int NV>
+__device__ void block_reduce_store(double (&v)[NV], double* __restrict__ partial) {
+  __shared__ double red[NV][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = wave_sum(v[q]);
+    if (lane == 0) red[q][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    int q = threadIdx.x;
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[q][w];
+    partial[(size_t)blockIdx.x * NV + q] = s;
+  }
+}
+
+// Mixed-precision LJ over the full list.  Pair distances come from exact
+// integer differences of the 32-bit fixed-point mirror (minimum image for free,
+// one rounding when converted to FP32), pair math is FP32, and each particle's
+// force is summed in FP32 chunks of kChunk neighbours carried into FP64.  Pairs
+// whose FP32 r^2 lies within 1e-5 relative of rc^2 re-decide the cutoff test in
+// FP64 so the set of interacting pairs equals the FP64 path's.
+// Potential: src/interaction/LennardJones.hpp:129-136 (force), :109-114 (energy).
+template <bool ONETYPE, bool ENERGY, bool ACCUM, int kChunk>
+__global__ __launch_bounds__(256) void k_lj_mixed(
+    int n, const uint4* __restrict__ posq, const double4* __restrict__ pos,
+    const int* __restrict__ nbr, const int* __restrict__ cnt, int cap, Geom g,
+    const LJParam* __restrict__ prm, int ntypes, double* __restrict__ fx,
+    double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ partial) {
+  __shared__ float4 tab[kMaxTypes * kMaxTypes];
+  __shared__ float2 etab[kMaxTypes * kMaxTypes];
+  if (!ONETYPE) {
+    for (int t = threadIdx.x; t < ntypes * ntypes; t += blockDim.x) {
+      const LJParam p = prm[t];
+      tab[t] = make_float4(p.ff1, p.ff2, p.set ? p.rc2 : -1.0f, p.shift);
+      etab[t] = make_float2((float)p.sig2, (float)p.eps4);
+    }
+    __syncthreads();
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kNPart];
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
+  if (i < n) {
+    const uint4 qi = posq[i];
+    const int* np = nbr + (size_t)(i >> 6) * (size_t)cap * 64 + (i & 63);
+    const int c = min(cnt[i], cap);
+    float ff1 = prm[0].ff1, ff2 = prm[0].ff2, rc2 = prm[0].rc2, shift = prm[0].shift;
+    float sig2 = (float)prm[0].sig2, eps4 = (float)prm[0].eps4;
+    double drc2 = prm[0].drc2;
+    const int rowi = (int)qi.w * ntypes;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    double Ax = 0.0, Ay = 0.0, Az = 0.0;
+    float e32 = 0.f, w32 = 0.f;
+    float t32[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < c; ++k) {
+      const int j = np[(size_t)k * 64];
+      const uint4 qj = posq[j];
+      const float dx = (float)(int)(qi.x - qj.x) * g.qs[0];
+      const float dy = (float)(int)(qi.y - qj.y) * g.qs[1];
+      const float dz = (float)(int)(qi.z - qj.z) * g.qs[2];
+      if (!ONETYPE) {
+        const float4 t = tab[rowi + (int)qj.w];
+        ff1 = t.x; ff2 = t.y; rc2 = t.z; shift = t.w;
+        if (ENERGY) { const float2 et = etab[rowi + (int)qj.w]; sig2 = et.x; eps4 = et.y; }
+      }
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      bool in = r2 <= rc2;
+      if (__builtin_fabsf(r2 - rc2) <= 1e-5f * rc2 && rc2 > 0.f) {
+        // rare: settle the cutoff test in FP64 from the FP64 positions
+        if (!ONETYPE) drc2 = prm[rowi + (int)qj.w].drc2;
+        const double4 pi = pos[i], pj = pos[j];
+        double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+        ex = __builtin_fma(-g.L[0], __builtin_rint(ex * g.invL[0]), ex);
+        ey = __builtin_fma(-g.L[1], __builtin_rint(ey * g.invL[1]), ey);
+        ez = __builtin_fma(-g.L[2], __builtin_rint(ez * g.invL[2]), ez);
+        in = (ex * ex + ey * ey + ez * ez) <= drc2;
+      }
+      const float inv = rcp_nr(r2);
+      const float r6 = inv * inv * inv;
+      float ff = r6 * (ff1 * r6 - ff2) * inv;
+      ff = in ? ff : 0.0f;
+      ax = __builtin_fmaf(dx, ff, ax);
+      ay = __builtin_fmaf(dy, ff, ay);
+      az = __builtin_fmaf(dz, ff, az);
+      if (ENERGY) {
+        const float f2 = sig2 * inv;
+        const float f6 = f2 * f2 * f2;
+        const float e = eps4 * (f6 * f6 - f6) - shift;
+        e32 += in ? e : 0.0f;
+        w32 += r2 * ff;
+        t32[0] += dx * dx * ff; t32[1] += dy * dy * ff; t32[2] += dz * dz * ff;
+        t32[3] += dx * dy * ff; t32[4] += dx * dz * ff; t32[5] += dy * dz * ff;
+      }
+      if ((k % kChunk) == kChunk - 1) {
+        Ax += (double)ax; Ay += (double)ay; Az += (double)az;
+        ax = ay = az = 0.f;
+        if (ENERGY) {
+          acc[0] += (double)e32; acc[1] += (double)w32;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) { acc[2 + q] += (double)t32[q]; t32[q] = 0.f; }
+          e32 = w32 = 0.f;
+        }
+      }
+    }
+    Ax += (double)ax; Ay += (double)ay; Az += (double)az;
+    if (ENERGY) {
+      acc[0] += (double)e32; acc[1] += (double)w32;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc[2 + q] += (double)t32[q];
+    }
+    if (!ENERGY) {
+      if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
+      else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
+    }
+  }
+  if (ENERGY) {
+#pragma unroll
+    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;   // each pair is seen from both ends
+    block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
+// All-FP64 LJ over the same full list (the comparison path).
+template <bool ONETYPE, bool ENERGY, bool ACCUM>
+__global__ __launch_bounds__(256) void k_lj_fp64(
+    int n, const double4* __restrict__ pos, const int* __restrict__ nbr,
+    const int* __restrict__ cnt, int cap, Geom g, const LJParam* __restrict__ prm, int ntypes,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    double* __restrict__ partial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kNPart];
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
+  if (i < n) {
+    const double4 pi = pos[i];
+    const int* np = nbr + (size_t)(i >> 6) * (size_t)cap * 64 + (i & 63);
+    const int c = min(cnt[i], cap);
+    const int rowi = (int)pi.w * ntypes;
+    double ff1 = prm[0].dff1, ff2 = prm[0].dff2, rc2 = prm[0].drc2, shift = prm[0].dshift;
+    double sig2 = prm[0].sig2, eps4 = prm[0].eps4;
+    int set = prm[0].set;
+    double Ax = 0.0, Ay = 0.0, Az = 0.0;
+    for (int k = 0; k < c; ++k) {
+      const int j = np[(size_t)k * 64];
+      const double4 pj = pos[j];
+      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      dx = __builtin_fma(-g.L[0], __builtin_rint(dx * g.invL[0]), dx);
+      dy = __builtin_fma(-g.L[1], __builtin_rint(dy * g.invL[1]), dy);
+      dz = __builtin_fma(-g.L[2], __builtin_rint(dz * g.invL[2]), dz);
+      if (!ONETYPE) {
+        const LJParam& p = prm[rowi + (int)pj.w];
+        ff1 = p.dff1; ff2 = p.dff2; rc2 = p.drc2; shift = p.dshift; sig2 = p.sig2; eps4 = p.eps4;
+        set = p.set;
+      }
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const bool in = set && r2 <= rc2;
+      const double inv = rcp_nr(r2);
+      const double r6 = inv * inv * inv;
+      double ff = r6 * (ff1 * r6 - ff2) * inv;
+      ff = in ? ff : 0.0;
+      Ax = __builtin_fma(dx, ff, Ax);
+      Ay = __builtin_fma(dy, ff, Ay);
+      Az = __builtin_fma(dz, ff, Az);
+      if (ENERGY) {
+        const double f2 = sig2 * inv;
+        const double f6 = f2 * f2 * f2;
+        acc[0] += in ? eps4 * (f6 * f6 - f6) - shift : 0.0;
+        acc[1] += r2 * ff;
+        acc[2] += dx * dx * ff; acc[3] += dy * dy * ff; acc[4] += dz * dz * ff;
+        acc[5] += dx * dy * ff; acc[6] += dx * dz * ff; acc[7] += dy * dz * ff;
+      }
+    }
+    if (!ENERGY) {
+      if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
+      else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
+    }
+  }
+  if (ENERGY) {
+#pragma unroll
+    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
+    block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
+// ------------------------------------------------------------------ FENE bonds
+
+// Bonds are stored per particle id in both directions (bstart/bpartner), so
+// each particle sums F(x_i - x_j) over its partners and no atomics are needed;
+// this equals the reference's p1 += F, p2 -= F (src/interaction/FixedPairListInteractionTemplate.hpp:112-200).
+// Distance: minimum image as OrthorhombicBC::getMinimumImageVectorBox
+// (src/bc/OrthorhombicBC.cpp:86-105) in FP64; in mixed mode from the fixed-point
+// mirror, which is the same minimum image for |d| < L/2.
+template <bool MIXED, bool ENERGY, bool ACCUM>
+__global__ __launch_bounds__(256) void k_fene(
+    int n, const double4* __restrict__ pos, const uint4* __restrict__ posq,
+    const int* __restrict__ id, const int* __restrict__ slot_of_id, int nid,
+    const int* __restrict__ bstart, const int* __restrict__ bpartner, Geom g, FENEParam fp,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kNPart];
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
+  if (i < n) {
+    const int pid = id[i];
+    double Ax = 0.0, Ay = 0.0, Az = 0.0;
+    const int b0 = bstart[pid], b1 = bstart[pid + 1];
+    for (int b = b0; b < b1; ++b) {
+      const int q = bpartner[b];
+      const int j = (q >= 0 && q < nid) ? slot_of_id[q] : -1;
+      if (j < 0) continue;
+      double dx, dy, dz;
+      if (MIXED) {
+        const uint4 qi = posq[i], qj = posq[j];
+        dx = (double)((float)(int)(qi.x - qj.x) * g.qs[0]);
+        dy = (double)((float)(int)(qi.y - qj.y) * g.qs[1]);
+        dz = (double)((float)(int)(qi.z - qj.z) * g.qs[2]);
+      } else {
+        const double4 pi = pos[i], pj = pos[j];
+        dx = pi.x - pj.x; dy = pi.y - pj.y; dz = pi.z - pj.z;
+        if (dx < -g.halfL[0]) dx += g.L[0]; else if (dx > g.halfL[0]) dx -= g.L[0];
+        if (dy < -g.halfL[1]) dy += g.L[1]; else if (dy > g.halfL[1]) dy -= g.L[1];
+        if (dz < -g.halfL[2]) dz += g.L[2]; else if (dz > g.halfL[2]) dz -= g.L[2];
+      }
+      double ff, e = 0.0;
+      double r2;
+      if (MIXED) {
+        const float fdx = (float)dx, fdy = (float)dy, fdz = (float)dz;
+        const float fr2 = fdx * fdx + fdy * fdy + fdz * fdz;
+        r2 = fr2;
+        float f32;
+        if (fp.r0 > 2.220446049250313e-16) {
+          const float r = __builtin_sqrtf(fr2), rr = r - (float)fp.r0;
+          f32 = -(float)fp.K * rr / (r * (1.0f - rr * rr / (float)fp.rMaxSqr));
+        } else {
+          f32 = -(float)fp.K / (1.0f - fr2 / (float)fp.rMaxSqr);
+        }
+        ff = f32;
+      } else {
+        r2 = dx * dx + dy * dy + dz * dz;
+        if (fp.r0 > 2.220446049250313e-16) {
+          const double r = sqrt(r2);
+          ff = -fp.K * (r - fp.r0) / (r * (1 - ((r - fp.r0) * (r - fp.r0) / fp.rMaxSqr)));
+        } else {
+          ff = -fp.K / (1.0 - r2 / fp.rMaxSqr);
+        }
+      }
+      if (r2 > fp.cutoffSqr) ff = 0.0;
+      Ax += dx * ff; Ay += dy * ff; Az += dz * ff;
+      if (ENERGY) {
+        if (r2 <= fp.cutoffSqr) {
+          const double t = (sqrt(r2) - fp.r0) / fp.rMax;
+          e = -0.5 * fp.rMaxSqr * fp.K * log(1 - t * t) - fp.shift;
+        }
+        acc[0] += e;
+        acc[1] += r2 * ff;
+        acc[2] += dx * dx * ff; acc[3] += dy * dy * ff; acc[4] += dz * dz * ff;
+        acc[5] += dx * dy * ff; acc[6] += dx * dz * ff; acc[7] += dy * dz * ff;
+      }
+    }
+    if (!ENERGY) {
+      if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
+      else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
+    }
+  }
+  if (ENERGY) {
+#pragma unroll
+    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
+    block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
+__global__ void k_zero3(int n, double* __restrict__ a, double* __restrict__ b, double* __restrict__ c) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) { a[i] = 0.0; b[i] = 0.0; c[i] = 0.0; }
+}
+
+// ------------------------------------------------------------------ integrator
+
+// Philox4x32-10 (Salmon et al., SC'11); same stream as the oracle.
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Langevin friction + uniform noise (LangevinThermostat::frictionThermo,
+// src/integrator/LangevinThermostat.cpp:141-177) followed, when do_int2 is set,
+// by VelocityVerlet::integrate2 (src/integrator/VelocityVerlet.cpp:289-305).
+__global__ __launch_bounds__(256) void k_thermo_integrate2(
+    int n, const int* __restrict__ id, double4* __restrict__ vel, double* __restrict__ fx,
+    double* __restrict__ fy, double* __restrict__ fz, int lang, double pref1, double pref2,
+    unsigned long long seed, unsigned long long call, const unsigned char* __restrict__ excl,
+    int nid, int do_int2, double half_dt) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double4 v = vel[i];
+  double f0 = fx[i], f1 = fy[i], f2 = fz[i];
+  if (lang) {
+    const int pid = id[i];
+    const bool skip = excl && pid < nid && excl[pid];
+    if (!skip) {
+      uint32_t o[4];
+      philox((uint32_t)pid, (uint32_t)call, (uint32_t)(call >> 32), 0x4C414E47u,
+             (uint32_t)seed, (uint32_t)(seed >> 32), o);
+      const double u0 = ((double)o[0] + 0.5) * 2.3283064365386963e-10;
+      const double u1 = ((double)o[1] + 0.5) * 2.3283064365386963e-10;
+      const double u2 = ((double)o[2] + 0.5) * 2.3283064365386963e-10;
+      const double m = v.w, massf = sqrt(m);
+      f0 += (pref1 * v.x) * m + (pref2 * (u0 - 0.5)) * massf;
+      f1 += (pref1 * v.y) * m + (pref2 * (u1 - 0.5)) * massf;
+      f2 += (pref1 * v.z) * m + (pref2 * (u2 - 0.5)) * massf;
+      fx[i] = f0; fy[i] = f1; fz[i] = f2;
+    }
+  }
+  if (do_int2) {
+    const double dtfm = half_dt / v.w;
+    v.x += dtfm * f0;
+    v.y += dtfm * f1;
+    v.z += dtfm * f2;
+    vel[i] = v;
+  }
+}
+
+// VelocityVerlet::integrate1 (src/integrator/VelocityVerlet.cpp:236-287): half
+// kick, drift, global max |dx|^2 (atomicMax on the bits of a non-negative
+// double), and the fixed-point mirror for the mixed kernels.
+__global__ __launch_bounds__(256) void k_integrate1(
+    int n, double4* __restrict__ pos, double4* __restrict__ vel, const double* __restrict__ fx,
+    const double* __restrict__ fy, const double* __restrict__ fz, double dt,
+    uint4* __restrict__ posq, Geom g, Ctl* ctl) {
+#pragma clang fp contract(off)
+  __shared__ double red[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double sq = 0.0;
+  if (i < n) {
+    double4 v = vel[i];
+    double4 p = pos[i];
+    const double dtfm = 0.5 * dt / v.w;
+    v.x += dtfm * fx[i];
+    v.y += dtfm * fy[i];
+    v.z += dtfm * fz[i];
+    const double d0 = v.x * dt, d1 = v.y * dt, d2 = v.z * dt;
+    p.x += d0; p.y += d1; p.z += d2;
+    sq = d0 * d0 + d1 * d1 + d2 * d2;
+    vel[i] = v;
+    pos[i] = p;
+    if (posq)
+      posq[i] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]),
+                           (uint32_t)(int)p.w);
+  }
+  sq = wave_max(sq);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+    atomicMax(&ctl->maxsq_bits, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// Kinetic sums per block: [sum m v^2, count, m vx vx, m vy vy, m vz vz, m vx vy,
+// m vx vz, m vy vz] over particles whose type bit is set in type_mask
+// (src/analysis/Temperature.hpp:100-122, src/analysis/PressureTensor.hpp:51-82).
+__global__ __launch_bounds__(256) void k_kinetic(int n, const double4* __restrict__ vel,
+                                                 const double4* __restrict__ pos,
+                                                 unsigned long long type_mask,
+                                                 double* __restrict__ partial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    const double4 v = vel[i];
+    const int t = (int)pos[i].w;
+    if (t < 64 && ((type_mask >> t) & 1ull)) {
+      acc[0] = v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
+      acc[1] = 1.0;
+      acc[2] = v.w * v.x * v.x; acc[3] = v.w * v.y * v.y; acc[4] = v.w * v.z * v.z;
+      acc[5] = v.w * v.x * v.y; acc[6] = v.w * v.x * v.z; acc[7] = v.w * v.y * v.z;
+    }
+  }
+  block_reduce_store<8>(acc, partial);
+}
+
+// Sum nblk rows of nv partials in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_reduce_partials(int nblk, int nv,
+                                                         const double* __restrict__ partial,
+                                                         double* __restrict__ out) {
+  __shared__ double red[4];
+  for (int q = 0; q < nv; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[(size_t)b * nv + q];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+      out[q] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+
+static inline int nblocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+void launch_fold_cell(hipStream_t s, int n, double4* pos, int4* img, int* cell_of, int* rank,
+                      int* cell_count, const Geom& g, Ctl* ctl, int do_fold) {
+  if (n > 0) hipLaunchKernelGGL(k_fold_cell, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, img, cell_of, rank, cell_count, g, ctl, do_fold);
+}
+
+void launch_scan_cells(hipStream_t s, int ncell, const int* count, int* start) {
+  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncell, count, start);
+}
+
+void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start, int* order) {
+  if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, s, n, cell_of, rank, start, order);
+}
+
+void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const int* id) {
+  hipLaunchKernelGGL(k_sort_cells, dim3(nblocks(ncell, 256)), dim3(256), 0, s, ncell, start, order, id);
+}
+
+void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, const double4* pos,
+                    const double4* vel, const double* fx, const double* fy, const double* fz,
+                    const int* id, const int4* img, double4* pos2, double4* vel2, double* fx2,
+                    double* fy2, double* fz2, int* id2, int4* img2, int* cell_of_slot,
+                    int* slot_of_id, uint4* posq, const Geom& g) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_permute, dim3(nblocks(n, 256)), dim3(256), 0, s, n, order, cell_of, pos, vel, fx, fy, fz, id, img,
+                       pos2, vel2, fx2, fy2, fz2, id2, img2, cell_of_slot, slot_of_id, posq, g);
+}
+
+void launch_fill_int(hipStream_t s, int n, int* a, int v) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblocks(n, 256)), dim3(256), 0, s, n, a, v);
+}
+
+void launch_reset_ctl(hipStream_t s, Ctl* ctl) {
+  hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, s, ctl);
+}
+
+void launch_nbr_build(hipStream_t s, int n, const double4* pos, const int* id, const int* cell_of_slot,
+                      const int* cstart, const Geom& g, double cutsq, const int* ex_start,
+                      const int* ex_ids, int* nbr, int* cnt, int cap, Ctl* ctl) {
+  if (n <= 0) return;
+  if (ex_start)
+    hipLaunchKernelGGL(k_nbr_build<true>, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, id, cell_of_slot, cstart, g,
+                       cutsq, ex_start, ex_ids, nbr, cnt, cap, ctl);
+  else
+    hipLaunchKernelGGL(k_nbr_build<false>, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, id, cell_of_slot, cstart, g,
+                       cutsq, ex_start, ex_ids, nbr, cnt, cap, ctl);
+}
+
+int lj_blocks(int n) { return nblocks(n, 256); }
+
+void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n,
+               const uint4* posq, const double4* pos, const int* nbr, const int* cnt, int cap,
+               const Geom& g, const LJParam* prm, int ntypes, double* fx, double* fy, double* fz,
+               double* partial) {
+  if (n <= 0) return;
+  dim3 grid(nblocks(n, 256)), block(256);
+#define LJM(OT, EN, AC) hipLaunchKernelGGL((k_lj_mixed<OT, EN, AC, 8>), grid, block, 0, s, n, posq, pos, nbr, cnt, cap, g, prm, ntypes, fx, fy, fz, partial)
+#define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_fp64<OT, EN, AC>), grid, block, 0, s, n, pos, nbr, cnt, cap, g, prm, ntypes, fx, fy, fz, partial)
+  if (mixed) {
+    if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }
+    else if (onetype) { if (accum) LJM(true, false, true); else LJM(true, false, false); }
+    else { if (accum) LJM(false, false, true); else LJM(false, false, false); }
+  } else {
+    if (energy) { if (onetype) LJD(true, true, false); else LJD(false, true, false); }
+    else if (onetype) { if (accum) LJD(true, false, true); else LJD(true, false, false); }
+    else { if (accum) LJD(false, false, true); else LJD(false, false, false); }
+  }
+#undef LJM
+#undef LJD
+}
+
+void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, const double4* pos,
+                 const uint4* posq, const int* id, const int* slot_of_id, int nid, const int* bstart,
+                 const int* bpartner, const Geom& g, const FENEParam& fp, double* fx, double* fy,
+                 double* fz, double* partial) {
+  if (n <= 0) return;
+  dim3 grid(nblocks(n, 256)), block(256);
+#define FN(MX, EN, AC) hipLaunchKernelGGL((k_fene<MX, EN, AC>), grid, block, 0, s, n, pos, posq, id, slot_of_id, nid, bstart, bpartner, g, fp, fx, fy, fz, partial)
+  if (mixed) {
+    if (energy) FN(true, true, false); else if (accum) FN(true, false, true); else FN(true, false, false);
+  } else {
+    if (energy) FN(false, true, false); else if (accum) FN(false, false, true); else FN(false, false, false);
+  }
+#undef FN
+}
+
+void launch_zero3(hipStream_t s, int n, double* a, double* b, double* c) {
+  if (n > 0) hipLaunchKernelGGL(k_zero3, dim3(nblocks(n, 256)), dim3(256), 0, s, n, a, b, c);
+}
+
+void launch_thermo_integrate2(hipStream_t s, int n, const int* id, double4* vel, double* fx,
+                              double* fy, double* fz, int lang, double pref1, double pref2,
+                              unsigned long long seed, unsigned long long call,
+                              const unsigned char* excl, int nid, int do_int2, double half_dt) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_thermo_integrate2, dim3(nblocks(n, 256)), dim3(256), 0, s, n, id, vel, fx, fy, fz, lang, pref1,
+                       pref2, seed, call, excl, nid, do_int2, half_dt);
+}
+
+void launch_integrate1(hipStream_t s, int n, double4* pos, double4* vel, const double* fx,
+                       const double* fy, const double* fz, double dt, uint4* posq, const Geom& g,
+                       Ctl* ctl) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_integrate1, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, vel, fx, fy, fz, dt, posq, g, ctl);
+}
+
+int kinetic_blocks(int n) { return nblocks(n, 256); }
+
+void launch_kinetic(hipStream_t s, int n, const double4* vel, const double4* pos,
+                    unsigned long long type_mask, double* partial) {
+  if (n > 0) hipLaunchKernelGGL(k_kinetic, dim3(nblocks(n, 256)), dim3(256), 0, s, n, vel, pos, type_mask, partial);
+}
+
+void launch_reduce_partials(hipStream_t s, int nblk, int nv, const double* partial, double* out) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, nblk, nv, partial, out);
+}
+
+}  // namespace espp

@@ -1,0 +1,1187 @@
+// Host runtime of the MI355X short-range MD step and its C ABI (include/espp_amd.h).
+//
+// One espp_sim owns the device state of one periodic system and drives the
+// kernels of kernels.hip on one HIP stream.  The control flow restates
+// VelocityVerlet::run (src/integrator/VelocityVerlet.cpp:63-169): a full force
+// evaluation with the Langevin heat-up at run() start, then per step integrate1,
+// the skin/2 rebuild trigger on the accumulated maximum displacement, the force
+// evaluation in interaction order, the thermostat and integrate2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/espp_amd.h"
+#include "espp_dev.h"
+#include "kernels.h"
+
+using namespace espp;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(ESPP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device buffer; resize() discards contents, grow_keep() preserves them.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void resize(size_t m) {
+    if (m <= n && p) return;
+    release();
+    if (m == 0) m = 1;
+    if (hipMalloc(&p, m * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      fail(ESPP_ERR_OOM, "device allocation of " + std::to_string(m * sizeof(T)) + " bytes failed");
+    }
+    n = m;
+  }
+  void grow_keep(size_t m, size_t keep, hipStream_t s) {
+    if (m <= n && p) return;
+    T* q = nullptr;
+    if (m == 0) m = 1;
+    if (hipMalloc(&q, m * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      fail(ESPP_ERR_OOM, "device allocation of " + std::to_string(m * sizeof(T)) + " bytes failed");
+    }
+    if (p && keep) hip_check(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s), "grow copy");
+    hip_check(hipStreamSynchronize(s), "grow sync");
+    release();
+    p = q;
+    n = m;
+  }
+  void swap(DBuf& o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
+};
+
+struct VerletListState {
+  double cut = 0.0;
+  int cap = 0;
+  DBuf<int> nbr, cnt;
+  int64_t builds = 0;
+  int maxcount = 0;
+  bool stale = true;   // slot order changed since the last build
+  std::vector<std::pair<int32_t, int32_t>> excl;
+  DBuf<int> ex_start, ex_ids;
+  int ex_nid = -1;
+  bool has_excl = false;
+};
+
+struct FixedPairListState {
+  std::vector<int32_t> a, b;
+  DBuf<int> bstart, bpartner;
+  int built_nid = -1;
+  bool dirty = true;
+};
+
+struct InteractionState {
+  int kind = 0;  // 0: VerletList LJ, 1: FixedPairList FENE
+  int list = 0;
+  std::vector<LJParam> tab = std::vector<LJParam>(kMaxTypes * kMaxTypes);
+  int maxset = -1;       // largest type with a potential
+  DBuf<LJParam> dtab;
+  int dtab_nt = 0;
+  bool dirty = true;
+  FENEParam fp{};
+};
+
+}  // namespace
+
+struct espp_sim {
+  Geom g{};
+  double skin = 0.0;
+  uint64_t seed = 0;
+  int precision = ESPP_FP64;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  int64_t n = 0;
+  int maxtype = 0;
+  DBuf<double4> pos, vel, pos2, vel2;
+  DBuf<double> fx, fy, fz, fx2, fy2, fz2;
+  DBuf<int> id, id2;
+  DBuf<int4> img, img2;
+  DBuf<uint4> posq;
+  DBuf<int> cell_of, rank, order, cell_of_slot, cell_count, cstart;
+  DBuf<int> slot_of_id;
+  int nid = 0;
+  std::vector<int32_t> host_ids;  // ids present (unsorted)
+  bool sorted = false;
+
+  DBuf<Ctl> ctl;
+  Ctl* hctl = nullptr;
+  DBuf<double> partial, red;
+  double* hred = nullptr;
+
+  std::vector<VerletListState*> vls;
+  std::vector<FixedPairListState*> fpls;
+  std::vector<InteractionState*> ias;
+  std::vector<int> active;   // System::shortRangeInteractions, in addInteraction order
+
+  int64_t step = 0;
+  double maxDist = 0.0;
+  bool resortFlag = true;
+  int64_t nResorts = 0;
+
+  bool lang = false;
+  double gamma = 0.0, temperature = 0.0;
+  std::vector<int32_t> lang_excl;
+  DBuf<unsigned char> excl_flag;
+  int excl_nid = -1;
+  bool excl_dirty = true;
+  uint64_t therm_calls = 0;
+
+  double timers[10] = {0};
+
+  // optional per-launch timing with HIP events on this handle's stream
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_open;
+  int64_t prof_count[8] = {0};
+  double prof_ms[8] = {0};
+
+  hipEvent_t take_event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  hipEvent_t prof_begin() {
+    if (!prof) return nullptr;
+    hipEvent_t a = take_event();
+    hip_check(hipEventRecord(a, stream), "hipEventRecord");
+    return a;
+  }
+  void prof_end(int kernel, hipEvent_t a) {
+    if (!prof || !a) return;
+    hipEvent_t b = take_event();
+    hip_check(hipEventRecord(b, stream), "hipEventRecord");
+    ev_open.push_back({kernel, {a, b}});
+  }
+  void prof_collect() {
+    if (ev_open.empty()) return;
+    sync();
+    for (auto& r : ev_open) {
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, r.second.first, r.second.second), "hipEventElapsedTime");
+      prof_count[r.first]++;
+      prof_ms[r.first] += ms;
+      ev_pool.push_back(r.second.first);
+      ev_pool.push_back(r.second.second);
+    }
+    ev_open.clear();
+  }
+
+  ~espp_sim() {
+    for (auto& r : ev_open) { (void)hipEventDestroy(r.second.first); (void)hipEventDestroy(r.second.second); }
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    for (auto* v : vls) delete v;
+    for (auto* f : fpls) delete f;
+    for (auto* a : ias) delete a;
+    if (hctl) (void)hipHostFree(hctl);
+    if (hred) (void)hipHostFree(hred);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  bool mixed() const { return precision == ESPP_MIXED; }
+
+  void sync() { hip_check(hipStreamSynchronize(stream), "stream synchronize"); }
+
+  void read_ctl() {
+    hip_check(hipMemcpyAsync(hctl, ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, stream), "ctl readback");
+    sync();
+  }
+
+  // ------------------------------------------------------------ particles
+  void reserve_particles(int64_t need) {
+    if ((size_t)need <= pos.n && pos.p) return;
+    size_t m = std::max<size_t>((size_t)need, pos.n ? pos.n * 2 : 1024);
+    size_t keep = (size_t)n;
+    pos.grow_keep(m, keep, stream); vel.grow_keep(m, keep, stream);
+    fx.grow_keep(m, keep, stream); fy.grow_keep(m, keep, stream); fz.grow_keep(m, keep, stream);
+    id.grow_keep(m, keep, stream); img.grow_keep(m, keep, stream);
+    pos2.resize(m); vel2.resize(m); fx2.resize(m); fy2.resize(m); fz2.resize(m);
+    id2.resize(m); img2.resize(m);
+    cell_of.resize(m); rank.resize(m); order.resize(m); cell_of_slot.grow_keep(m, keep, stream);
+    if (mixed()) posq.grow_keep(m, keep, stream);
+  }
+
+  int64_t ncell() const { return (int64_t)g.n[0] * g.n[1] * g.n[2]; }
+
+  void ensure_id_map() {
+    int need = nid > 0 ? nid : 1;
+    if (slot_of_id.n < (size_t)need) {
+      slot_of_id.resize((size_t)need);
+    }
+  }
+
+  // Fold (optionally), bin into cells, sort each cell by id and permute every
+  // per-particle array into cell order.
+  void bin(bool do_fold) {
+    int ni = (int)n;
+    hipEvent_t pe = prof_begin();
+    launch_reset_ctl(stream, ctl.p);
+    launch_fill_int(stream, (int)ncell(), cell_count.p, 0);
+    ensure_id_map();
+    launch_fill_int(stream, nid, slot_of_id.p, -1);
+    if (ni > 0) {
+      Geom gg = g;
+      launch_fold_cell(stream, ni, pos.p, img.p, cell_of.p, rank.p, cell_count.p, gg, ctl.p, do_fold ? 1 : 0);
+    }
+    launch_scan_cells(stream, (int)ncell(), cell_count.p, cstart.p);
+    if (ni > 0) {
+      launch_scatter(stream, ni, cell_of.p, rank.p, cstart.p, order.p);
+      launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, id.p);
+      launch_permute(stream, ni, order.p, cell_of.p, pos.p, vel.p, fx.p, fy.p, fz.p, id.p, img.p, pos2.p,
+                     vel2.p, fx2.p, fy2.p, fz2.p, id2.p, img2.p, cell_of_slot.p, slot_of_id.p,
+                     mixed() ? posq.p : nullptr, g);
+      pos.swap(pos2); vel.swap(vel2); fx.swap(fx2); fy.swap(fy2); fz.swap(fz2);
+      id.swap(id2); img.swap(img2);
+    }
+    prof_end(4, pe);
+    for (auto* v : vls) v->stale = true;
+    if (do_fold) {
+      read_ctl();
+      if (hctl->nanpos) fail(ESPP_ERR_NAN, "particle has moved to outer space (one or more coordinates are nan)");
+    }
+    sorted = true;
+  }
+
+  void ensure_sorted() {
+    if (!sorted) bin(false);
+  }
+
+  // neighbour lists hold slot indices: rebuild every list the last binning invalidated
+  void ensure_lists() {
+    ensure_sorted();
+    for (auto* v : vls)
+      if (v->stale) build_vl(*v);
+  }
+
+  // ------------------------------------------------------------ lists
+  void build_vl(VerletListState& v) {
+    const double cutVerlet = v.cut + skin;
+    const double cutsq = cutVerlet * cutVerlet;
+    for (int d = 0; d < 3; ++d) {
+      double cs = g.L[d] / g.n[d];
+      if (cs < cutVerlet)
+        fail(ESPP_ERR_ARG, "cell size " + std::to_string(cs) + " in direction " + std::to_string(d) +
+                               " is smaller than cutoff+skin " + std::to_string(cutVerlet));
+    }
+    if (v.has_excl && v.ex_nid != nid) upload_exclusions(v);
+    const int ngroups = (int)((n + 63) / 64);
+    if (v.cap == 0) {
+      double vol = g.L[0] * g.L[1] * g.L[2];
+      double nbar = 4.0 / 3.0 * M_PI * cutVerlet * cutVerlet * cutVerlet * (double)n / vol;
+      int cap = (int)std::ceil(1.4 * nbar + 24.0);
+      cap = std::min<int64_t>(cap, std::max<int64_t>(n, 8));
+      v.cap = (cap + 7) & ~7;
+    }
+    v.cnt.resize((size_t)std::max<int64_t>(n, 1));
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
+      launch_reset_ctl(stream, ctl.p);
+      hipEvent_t pe = prof_begin();
+      launch_nbr_build(stream, (int)n, pos.p, id.p, cell_of_slot.p, cstart.p, g, cutsq,
+                       v.has_excl ? v.ex_start.p : nullptr, v.has_excl ? v.ex_ids.p : nullptr, v.nbr.p,
+                       v.cnt.p, v.cap, ctl.p);
+      prof_end(1, pe);
+      read_ctl();
+      v.maxcount = hctl->maxcount;
+      if (!hctl->overflow) break;
+      v.cap = ((int)std::ceil(v.maxcount * 1.2) + 8 + 7) & ~7;
+    }
+    v.stale = false;
+    v.builds++;
+  }
+
+  void upload_exclusions(VerletListState& v) {
+    sync();
+    std::vector<int> start((size_t)nid + 1, 0), ids;
+    std::vector<std::vector<int>> adj((size_t)nid);
+    for (auto& e : v.excl) {
+      if (e.first < nid && e.second < nid && e.first >= 0 && e.second >= 0) {
+        adj[(size_t)e.first].push_back(e.second);
+        adj[(size_t)e.second].push_back(e.first);
+      }
+    }
+    for (int i = 0; i < nid; ++i) {
+      std::sort(adj[(size_t)i].begin(), adj[(size_t)i].end());
+      adj[(size_t)i].erase(std::unique(adj[(size_t)i].begin(), adj[(size_t)i].end()), adj[(size_t)i].end());
+      start[(size_t)i + 1] = start[(size_t)i] + (int)adj[(size_t)i].size();
+    }
+    ids.reserve((size_t)start[(size_t)nid]);
+    for (int i = 0; i < nid; ++i) ids.insert(ids.end(), adj[(size_t)i].begin(), adj[(size_t)i].end());
+    v.ex_start.resize(start.size());
+    v.ex_ids.resize(std::max<size_t>(ids.size(), 1));
+    hip_check(hipMemcpy(v.ex_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice), "excl upload");
+    if (!ids.empty())
+      hip_check(hipMemcpy(v.ex_ids.p, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice), "excl upload");
+    v.ex_nid = nid;
+  }
+
+  void upload_bonds(FixedPairListState& f) {
+    sync();
+    std::vector<int> deg((size_t)nid + 1, 0);
+    for (size_t k = 0; k < f.a.size(); ++k) {
+      if (f.a[k] < nid && f.b[k] < nid) { deg[(size_t)f.a[k] + 1]++; deg[(size_t)f.b[k] + 1]++; }
+    }
+    std::partial_sum(deg.begin(), deg.end(), deg.begin());
+    std::vector<int> fill(deg.begin(), deg.end() - 1), part((size_t)std::max(deg[(size_t)nid], 1));
+    for (size_t k = 0; k < f.a.size(); ++k) {
+      if (f.a[k] < nid && f.b[k] < nid) {
+        part[(size_t)fill[(size_t)f.a[k]]++] = f.b[k];
+        part[(size_t)fill[(size_t)f.b[k]]++] = f.a[k];
+      }
+    }
+    f.bstart.resize(deg.size());
+    f.bpartner.resize(part.size());
+    hip_check(hipMemcpy(f.bstart.p, deg.data(), deg.size() * sizeof(int), hipMemcpyHostToDevice), "bond upload");
+    hip_check(hipMemcpy(f.bpartner.p, part.data(), part.size() * sizeof(int), hipMemcpyHostToDevice), "bond upload");
+    f.built_nid = nid;
+    f.dirty = false;
+  }
+
+  void decompose() {
+    bin(true);
+    for (auto* v : vls) build_vl(*v);
+  }
+
+  // ------------------------------------------------------------ interactions
+  int lj_ntypes(const InteractionState& a) const { return std::max(a.maxset, maxtype) + 1; }
+
+  void upload_lj(InteractionState& a) {
+    int nt = lj_ntypes(a);
+    if (nt > kMaxTypes) fail(ESPP_ERR_ARG, "at most " + std::to_string(kMaxTypes) + " particle types are supported");
+    if (!a.dirty && a.dtab_nt == nt) return;
+    sync();
+    std::vector<LJParam> t((size_t)nt * nt);
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j) t[(size_t)i * nt + j] = a.tab[(size_t)i * kMaxTypes + j];
+    a.dtab.resize(t.size());
+    hip_check(hipMemcpy(a.dtab.p, t.data(), t.size() * sizeof(LJParam), hipMemcpyHostToDevice), "lj table upload");
+    a.dtab_nt = nt;
+    a.dirty = false;
+  }
+
+  void run_interaction(InteractionState& a, bool energy, bool accum, double* partial_out) {
+    if (a.kind == 0) {
+      VerletListState& v = *vls[(size_t)a.list];
+      upload_lj(a);
+      bool onetype = a.dtab_nt == 1;
+      hipEvent_t pe = energy ? nullptr : prof_begin();
+      launch_lj(stream, mixed(), onetype, energy, accum, (int)n, posq.p, pos.p, v.nbr.p, v.cnt.p, v.cap, g,
+                a.dtab.p, a.dtab_nt, fx.p, fy.p, fz.p, partial_out);
+      prof_end(0, pe);
+    } else {
+      FixedPairListState& f = *fpls[(size_t)a.list];
+      if (f.dirty || f.built_nid != nid) upload_bonds(f);
+      hipEvent_t pe = energy ? nullptr : prof_begin();
+      launch_fene(stream, mixed(), energy, accum, (int)n, pos.p, posq.p, id.p, slot_of_id.p, nid, f.bstart.p,
+                  f.bpartner.p, g, a.fp, fx.p, fy.p, fz.p, partial_out);
+      prof_end(5, pe);
+    }
+  }
+
+  void calc_forces() {
+    ensure_lists();
+    bool first = true;
+    for (int k : active) {
+      run_interaction(*ias[(size_t)k], false, !first, nullptr);
+      first = false;
+    }
+    if (first) launch_zero3(stream, (int)n, fx.p, fy.p, fz.p);
+  }
+
+  void upload_langevin_excl() {
+    if (!excl_dirty && excl_nid == nid) return;
+    sync();
+    std::vector<unsigned char> f((size_t)std::max(nid, 1), 0);
+    for (int32_t e : lang_excl) if (e >= 0 && e < nid) f[(size_t)e] = 1;
+    excl_flag.resize(f.size());
+    hip_check(hipMemcpy(excl_flag.p, f.data(), f.size(), hipMemcpyHostToDevice), "langevin exclusion upload");
+    excl_nid = nid;
+    excl_dirty = false;
+  }
+
+  // ------------------------------------------------------------ reductions
+  void reduce(int nblk, int nv, double* out) {
+    launch_reduce_partials(stream, nblk, nv, partial.p, red.p);
+    hip_check(hipMemcpyAsync(hred, red.p, (size_t)nv * sizeof(double), hipMemcpyDeviceToHost, stream), "reduce readback");
+    sync();
+    std::memcpy(out, hred, (size_t)nv * sizeof(double));
+  }
+
+  void interaction_sums(InteractionState& a, double out8[8]) {
+    ensure_lists();
+    int nblk = std::max(lj_blocks((int)n), 1);
+    partial.resize((size_t)nblk * 8);
+    if (n == 0) { for (int q = 0; q < 8; ++q) out8[q] = 0.0; return; }
+    run_interaction(a, true, false, partial.p);
+    reduce(nblk, 8, out8);
+  }
+
+  // ------------------------------------------------------------ integrator
+  void run(int nsteps, double dt) {
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    double t_int1 = 0, t_resort = 0, t_force = 0, t_int2 = 0;
+    nResorts = 0;
+    const double skinHalf = 0.5 * skin;
+    const double pref1 = -gamma;
+    const double pref2 = std::sqrt(24.0 * temperature * gamma / dt);
+    if (lang) upload_langevin_excl();
+    if (resortFlag) {
+      decompose();
+      maxDist = 0.0;
+      resortFlag = false;
+    } else {
+      ensure_lists();
+    }
+    // recalc1 (heatUp) -> updateForces -> recalc2 (coolDown)
+    calc_forces();
+    if (lang) {
+      launch_thermo_integrate2(stream, (int)n, id.p, vel.p, fx.p, fy.p, fz.p, 1, pref1, pref2 * std::sqrt(3.0), seed,
+                               therm_calls++, excl_flag.p, excl_nid, 0, 0.5 * dt);
+    }
+    for (int i = 0; i < nsteps; ++i) {
+      auto ta = clk::now();
+      launch_reset_ctl(stream, ctl.p);
+      hipEvent_t pe1 = prof_begin();
+      launch_integrate1(stream, (int)n, pos.p, vel.p, fx.p, fy.p, fz.p, dt, mixed() ? posq.p : nullptr, g, ctl.p);
+      prof_end(2, pe1);
+      read_ctl();
+      double maxSq;
+      std::memcpy(&maxSq, &hctl->maxsq_bits, sizeof(double));
+      maxDist += std::sqrt(maxSq);
+      auto tb = clk::now();
+      t_int1 += std::chrono::duration<double>(tb - ta).count();
+      if (maxDist > skinHalf) resortFlag = true;
+      if (resortFlag) {
+        decompose();
+        maxDist = 0.0;
+        resortFlag = false;
+        nResorts++;
+      }
+      auto tc = clk::now();
+      t_resort += std::chrono::duration<double>(tc - tb).count();
+      calc_forces();
+      auto td = clk::now();
+      t_force += std::chrono::duration<double>(td - tc).count();
+      hipEvent_t pe2 = prof_begin();
+      launch_thermo_integrate2(stream, (int)n, id.p, vel.p, fx.p, fy.p, fz.p, lang ? 1 : 0, pref1, pref2, seed,
+                               lang ? therm_calls++ : 0ull, excl_flag.p, excl_nid, 1, 0.5 * dt);
+      prof_end(3, pe2);
+      step++;
+      t_int2 += std::chrono::duration<double>(clk::now() - td).count();
+    }
+    sync();
+    double t_run = std::chrono::duration<double>(clk::now() - t0).count();
+    prof_collect();
+    timers[0] = t_run;
+    timers[1] = t_force;
+    timers[2] = timers[3] = 0.0;
+    timers[4] = timers[5] = 0.0;
+    timers[6] = t_int1;
+    timers[7] = t_int2;
+    timers[8] = t_resort;
+    timers[9] = t_run - (t_force + t_int1 + t_int2 + t_resort);
+  }
+
+  // ------------------------------------------------------------ host access
+  std::vector<int64_t> slots_by_id(const std::vector<int>& ids_dev) const {
+    std::vector<int64_t> s((size_t)n);
+    std::iota(s.begin(), s.end(), 0);
+    std::sort(s.begin(), s.end(), [&](int64_t a, int64_t b) { return ids_dev[(size_t)a] < ids_dev[(size_t)b]; });
+    return s;
+  }
+};
+
+// ------------------------------------------------------------------ C ABI
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return ESPP_OK;
+  } catch (const Error& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host out of memory";
+    return ESPP_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return ESPP_ERR_STATE;
+  }
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) fail(ESPP_ERR_ARG, msg);
+}
+
+void set_geom(Geom& g, const double box[3], const int32_t cells[3]) {
+  for (int d = 0; d < 3; ++d) {
+    g.L[d] = box[d];
+    g.invL[d] = 1.0 / box[d];
+    g.halfL[d] = 0.5 * box[d];
+    g.n[d] = cells[d];
+    double cs = (box[d] - 0.0) / (double)cells[d];
+    g.ics[d] = 1.0 / cs;
+    g.qinv[d] = 4294967296.0 / box[d];
+    g.qs[d] = (float)(box[d] / 4294967296.0);
+  }
+}
+
+template <class T>
+std::vector<T> download(const DBuf<T>& b, size_t count) {
+  std::vector<T> h(count);
+  if (count) hip_check(hipMemcpy(h.data(), b.p, count * sizeof(T), hipMemcpyDeviceToHost), "download");
+  return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* espp_last_error(void) { return g_err.c_str(); }
+
+const char* espp_version(void) { return "espp_amd 0.1 (gfx950)"; }
+
+int espp_device_count(int32_t* count) {
+  return guarded([&] {
+    need(count != nullptr, "count is NULL");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+      (void)hipGetLastError();
+      c = 0;
+    }
+    *count = c;
+  });
+}
+
+int espp_create(const double box[3], const int32_t cell_grid[3], double skin, uint64_t seed, int32_t precision,
+                espp_sim** out) {
+  return guarded([&] {
+    need(out != nullptr && box != nullptr && cell_grid != nullptr, "NULL argument");
+    *out = nullptr;
+    for (int d = 0; d < 3; ++d) {
+      need(box[d] > 0.0 && std::isfinite(box[d]), "box lengths must be positive and finite");
+      need(cell_grid[d] > 0, "cell grid must be positive");
+    }
+    need(skin >= 0.0, "skin must be non-negative");
+    need(precision == ESPP_FP64 || precision == ESPP_MIXED, "precision must be ESPP_FP64 or ESPP_MIXED");
+    int dc = 0;
+    if (hipGetDeviceCount(&dc) != hipSuccess || dc == 0) {
+      (void)hipGetLastError();
+      fail(ESPP_ERR_DEVICE, "no HIP device available: the MI355X path needs a GPU");
+    }
+    auto* s = new espp_sim();
+    try {
+      set_geom(s->g, box, cell_grid);
+      s->skin = skin;
+      s->seed = seed;
+      s->precision = precision;
+      hip_check(hipGetDevice(&s->device), "hipGetDevice");
+      hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+      s->ctl.resize(1);
+      hip_check(hipHostMalloc((void**)&s->hctl, sizeof(Ctl), hipHostMallocDefault), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&s->hred, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+      s->red.resize(16);
+      int64_t nc = s->ncell();
+      need(nc < (int64_t)1 << 30, "cell grid too large");
+      s->cell_count.resize((size_t)nc);
+      s->cstart.resize((size_t)nc + 1);
+      s->reserve_particles(1024);
+      s->ensure_id_map();
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+int espp_destroy(espp_sim* sim) {
+  return guarded([&] {
+    if (!sim) return;
+    (void)hipStreamSynchronize(sim->stream);
+    delete sim;
+  });
+}
+
+int espp_set_skin(espp_sim* sim, double skin) {
+  return guarded([&] {
+    need(sim && skin >= 0.0, "invalid skin");
+    sim->skin = skin;
+  });
+}
+
+int espp_add_particles(espp_sim* s, int64_t n, const int32_t* id, const int32_t* type, const double* pos,
+                       const double* vel, const double* mass) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    need(n >= 0, "negative count");
+    if (n == 0) return;
+    need(id != nullptr && pos != nullptr, "id and pos are mandatory");
+    need(s->n + n < (int64_t)1 << 31, "too many particles");
+    // duplicate ids: the reference refuses the whole list (src/storage/Storage.py:281-290)
+    std::vector<int32_t> newids(id, id + n);
+    int maxid = s->nid - 1;
+    for (int64_t k = 0; k < n; ++k) {
+      need(id[k] >= 0, "particle ids must be non-negative");
+      maxid = std::max(maxid, (int)id[k]);
+    }
+    {
+      std::vector<unsigned char> seen((size_t)maxid + 1, 0);
+      for (int32_t e : s->host_ids) seen[(size_t)e] = 1;
+      for (int64_t k = 0; k < n; ++k) {
+        if (seen[(size_t)id[k]]) fail(ESPP_ERR_ARG, "particle " + std::to_string(id[k]) + " already exists");
+        seen[(size_t)id[k]] = 1;
+      }
+    }
+    std::vector<double4> hp((size_t)n), hv((size_t)n);
+    std::vector<int4> hi((size_t)n);
+    std::vector<double> zero((size_t)n, 0.0);
+    for (int64_t k = 0; k < n; ++k) {
+      int t = type ? type[k] : 0;
+      need(t >= 0, "particle types must be non-negative");
+      s->maxtype = std::max(s->maxtype, t);
+      double x[3] = {pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]};
+      int im[3] = {0, 0, 0};
+      // Storage::addParticle folds on insertion (src/storage/Storage.cpp:299)
+      for (int d = 0; d < 3; ++d) {
+        if (!(x[d] == x[d])) fail(ESPP_ERR_NAN, "particle position is nan");
+        int tmp = (int)std::floor(x[d] * s->g.invL[d]);
+        im[d] += tmp;
+        x[d] -= (double)tmp * s->g.L[d];
+      }
+      double m = mass ? mass[k] : 1.0;
+      need(m > 0.0, "particle mass must be positive");
+      hp[(size_t)k] = make_double4(x[0], x[1], x[2], (double)t);
+      hv[(size_t)k] = make_double4(vel ? vel[3 * k] : 0.0, vel ? vel[3 * k + 1] : 0.0, vel ? vel[3 * k + 2] : 0.0, m);
+      hi[(size_t)k] = make_int4(im[0], im[1], im[2], 0);
+    }
+    s->reserve_particles(s->n + n);
+    size_t off = (size_t)s->n;
+    hip_check(hipMemcpy(s->pos.p + off, hp.data(), (size_t)n * sizeof(double4), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(s->vel.p + off, hv.data(), (size_t)n * sizeof(double4), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(s->img.p + off, hi.data(), (size_t)n * sizeof(int4), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(s->id.p + off, id, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(s->fx.p + off, zero.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(s->fy.p + off, zero.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(s->fz.p + off, zero.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice), "upload");
+    s->n += n;
+    s->host_ids.insert(s->host_ids.end(), newids.begin(), newids.end());
+    s->nid = std::max(s->nid, maxid + 1);
+    s->sorted = false;
+  });
+}
+
+int espp_decompose(espp_sim* s) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    s->decompose();
+  });
+}
+
+int espp_num_particles(espp_sim* s, int64_t* n) {
+  return guarded([&] {
+    need(s && n, "NULL argument");
+    *n = s->n;
+  });
+}
+
+int espp_get_ids(espp_sim* s, int32_t* out) {
+  return guarded([&] {
+    need(s && (out || s->n == 0), "NULL argument");
+    std::vector<int32_t> ids = s->host_ids;
+    std::sort(ids.begin(), ids.end());
+    std::copy(ids.begin(), ids.end(), out);
+  });
+}
+
+int espp_get_particles(espp_sim* s, int32_t field, double* out) {
+  return guarded([&] {
+    need(s && (out || s->n == 0), "NULL argument");
+    size_t n = (size_t)s->n;
+    if (n == 0) return;
+    s->sync();
+    std::vector<int> ids = download(s->id, n);
+    std::vector<int64_t> order = s->slots_by_id(ids);
+    if (field == ESPP_FIELD_POS || field == ESPP_FIELD_TYPE) {
+      std::vector<double4> p = download(s->pos, n);
+      for (size_t k = 0; k < n; ++k) {
+        const double4& q = p[(size_t)order[k]];
+        if (field == ESPP_FIELD_POS) { out[3 * k] = q.x; out[3 * k + 1] = q.y; out[3 * k + 2] = q.z; }
+        else out[k] = q.w;
+      }
+    } else if (field == ESPP_FIELD_VEL || field == ESPP_FIELD_MASS) {
+      std::vector<double4> v = download(s->vel, n);
+      for (size_t k = 0; k < n; ++k) {
+        const double4& q = v[(size_t)order[k]];
+        if (field == ESPP_FIELD_VEL) { out[3 * k] = q.x; out[3 * k + 1] = q.y; out[3 * k + 2] = q.z; }
+        else out[k] = q.w;
+      }
+    } else if (field == ESPP_FIELD_FORCE) {
+      std::vector<double> a = download(s->fx, n), b = download(s->fy, n), c = download(s->fz, n);
+      for (size_t k = 0; k < n; ++k) {
+        size_t o = (size_t)order[k];
+        out[3 * k] = a[o]; out[3 * k + 1] = b[o]; out[3 * k + 2] = c[o];
+      }
+    } else if (field == ESPP_FIELD_IMAGE) {
+      std::vector<int4> im = download(s->img, n);
+      for (size_t k = 0; k < n; ++k) {
+        const int4& q = im[(size_t)order[k]];
+        out[3 * k] = q.x; out[3 * k + 1] = q.y; out[3 * k + 2] = q.z;
+      }
+    } else {
+      fail(ESPP_ERR_ARG, "unknown particle field");
+    }
+  });
+}
+
+int espp_set_particles(espp_sim* s, int32_t field, int64_t m, const int32_t* ids, const double* values) {
+  return guarded([&] {
+    need(s && (m == 0 || (ids && values)), "NULL argument");
+    need(m >= 0, "negative count");
+    if (m == 0) return;
+    size_t n = (size_t)s->n;
+    s->sync();
+    std::vector<int> dev_ids = download(s->id, n);
+    std::vector<int64_t> slot_of((size_t)std::max(s->nid, 1), -1);
+    for (size_t k = 0; k < n; ++k) slot_of[(size_t)dev_ids[k]] = (int64_t)k;
+    auto slot = [&](int32_t pid) {
+      if (pid < 0 || pid >= s->nid || slot_of[(size_t)pid] < 0)
+        fail(ESPP_ERR_ARG, "particle " + std::to_string(pid) + " does not exist");
+      return (size_t)slot_of[(size_t)pid];
+    };
+    if (field == ESPP_FIELD_POS || field == ESPP_FIELD_TYPE) {
+      std::vector<double4> p = download(s->pos, n);
+      for (int64_t k = 0; k < m; ++k) {
+        double4& q = p[slot(ids[k])];
+        if (field == ESPP_FIELD_POS) { q.x = values[3 * k]; q.y = values[3 * k + 1]; q.z = values[3 * k + 2]; }
+        else {
+          need(values[k] >= 0.0, "particle types must be non-negative");
+          q.w = std::floor(values[k]);
+          s->maxtype = std::max(s->maxtype, (int)q.w);
+        }
+      }
+      hip_check(hipMemcpy(s->pos.p, p.data(), n * sizeof(double4), hipMemcpyHostToDevice), "upload");
+      // re-binning refreshes the cell order, the fixed-point mirror and the lists
+      s->sorted = false;
+    } else if (field == ESPP_FIELD_VEL || field == ESPP_FIELD_MASS) {
+      std::vector<double4> v = download(s->vel, n);
+      for (int64_t k = 0; k < m; ++k) {
+        double4& q = v[slot(ids[k])];
+        if (field == ESPP_FIELD_VEL) { q.x = values[3 * k]; q.y = values[3 * k + 1]; q.z = values[3 * k + 2]; }
+        else { need(values[k] > 0.0, "particle mass must be positive"); q.w = values[k]; }
+      }
+      hip_check(hipMemcpy(s->vel.p, v.data(), n * sizeof(double4), hipMemcpyHostToDevice), "upload");
+    } else {
+      fail(ESPP_ERR_ARG, "field cannot be set");
+    }
+  });
+}
+
+int espp_vl_create(espp_sim* s, double cutoff, int32_t* vl) {
+  return guarded([&] {
+    need(s && vl, "NULL argument");
+    need(cutoff >= 0.0, "cutoff must be non-negative");
+    auto* v = new VerletListState();
+    v->cut = cutoff;
+    s->vls.push_back(v);
+    try {
+      s->ensure_sorted();
+      s->build_vl(*v);
+    } catch (...) {
+      s->vls.pop_back();
+      delete v;
+      throw;
+    }
+    *vl = (int32_t)s->vls.size() - 1;
+  });
+}
+
+static VerletListState& get_vl(espp_sim* s, int32_t vl) {
+  need(s != nullptr, "NULL handle");
+  need(vl >= 0 && vl < (int32_t)s->vls.size(), "no such verlet list");
+  return *s->vls[(size_t)vl];
+}
+
+int espp_vl_exclude(espp_sim* s, int32_t vl, int64_t m, const int32_t* pairs) {
+  return guarded([&] {
+    VerletListState& v = get_vl(s, vl);
+    need(m >= 0 && (m == 0 || pairs), "invalid exclusion list");
+    for (int64_t k = 0; k < m; ++k) v.excl.emplace_back(pairs[2 * k], pairs[2 * k + 1]);
+    v.has_excl = !v.excl.empty();
+    v.ex_nid = -1;
+    if (v.has_excl) s->upload_exclusions(v);
+    s->ensure_sorted();
+    s->build_vl(v);
+  });
+}
+
+int espp_vl_rebuild(espp_sim* s, int32_t vl) {
+  return guarded([&] {
+    VerletListState& v = get_vl(s, vl);
+    s->ensure_sorted();
+    s->build_vl(v);
+  });
+}
+
+int espp_vl_size(espp_sim* s, int32_t vl, int64_t* npairs) {
+  return guarded([&] {
+    VerletListState& v = get_vl(s, vl);
+    need(npairs != nullptr, "NULL argument");
+    s->ensure_lists();
+    s->sync();
+    std::vector<int> c = download(v.cnt, (size_t)s->n);
+    int64_t tot = 0;
+    for (int x : c) tot += x;
+    *npairs = tot / 2;
+  });
+}
+
+int espp_vl_builds(espp_sim* s, int32_t vl, int64_t* builds) {
+  return guarded([&] {
+    VerletListState& v = get_vl(s, vl);
+    need(builds != nullptr, "NULL argument");
+    *builds = v.builds;
+  });
+}
+
+int espp_vl_pairs(espp_sim* s, int32_t vl, int32_t* out, int64_t cap, int64_t* total) {
+  return guarded([&] {
+    VerletListState& v = get_vl(s, vl);
+    need(total != nullptr && (cap == 0 || out != nullptr), "NULL argument");
+    s->ensure_lists();
+    s->sync();
+    size_t n = (size_t)s->n;
+    std::vector<int> c = download(v.cnt, n), ids = download(s->id, n);
+    std::vector<int> nb = download(v.nbr, (size_t)((n + 63) / 64) * (size_t)v.cap * 64);
+    int64_t k = 0;
+    for (size_t i = 0; i < n; ++i) {
+      const int* row = nb.data() + (i / 64) * (size_t)v.cap * 64 + (i % 64);
+      for (int q = 0; q < c[i]; ++q) {
+        int j = row[(size_t)q * 64];
+        if (ids[i] < ids[(size_t)j]) {
+          if (k < cap) { out[2 * k] = ids[i]; out[2 * k + 1] = ids[(size_t)j]; }
+          ++k;
+        }
+      }
+    }
+    *total = k;
+  });
+}
+
+int espp_vl_cutoff(espp_sim* s, int32_t vl, double* cut) {
+  return guarded([&] {
+    VerletListState& v = get_vl(s, vl);
+    need(cut != nullptr, "NULL argument");
+    *cut = v.cut + s->skin;
+  });
+}
+
+int espp_fpl_create(espp_sim* s, int32_t* fpl) {
+  return guarded([&] {
+    need(s && fpl, "NULL argument");
+    s->fpls.push_back(new FixedPairListState());
+    *fpl = (int32_t)s->fpls.size() - 1;
+  });
+}
+
+int espp_fpl_add(espp_sim* s, int32_t fpl, int64_t m, const int32_t* pairs) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    need(fpl >= 0 && fpl < (int32_t)s->fpls.size(), "no such fixed pair list");
+    need(m >= 0 && (m == 0 || pairs), "invalid bond list");
+    FixedPairListState& f = *s->fpls[(size_t)fpl];
+    for (int64_t k = 0; k < m; ++k) {
+      need(pairs[2 * k] >= 0 && pairs[2 * k + 1] >= 0, "bond ids must be non-negative");
+      f.a.push_back(pairs[2 * k]);
+      f.b.push_back(pairs[2 * k + 1]);
+    }
+    f.dirty = true;
+  });
+}
+
+int espp_fpl_size(espp_sim* s, int32_t fpl, int64_t* m) {
+  return guarded([&] {
+    need(s && m, "NULL argument");
+    need(fpl >= 0 && fpl < (int32_t)s->fpls.size(), "no such fixed pair list");
+    *m = (int64_t)s->fpls[(size_t)fpl]->a.size();
+  });
+}
+
+int espp_lj_create(espp_sim* s, int32_t vl, int32_t* ia) {
+  return guarded([&] {
+    get_vl(s, vl);
+    need(ia != nullptr, "NULL argument");
+    auto* a = new InteractionState();
+    a->kind = 0;
+    a->list = vl;
+    s->ias.push_back(a);
+    *ia = (int32_t)s->ias.size() - 1;
+  });
+}
+
+static InteractionState& get_ia(espp_sim* s, int32_t ia) {
+  need(s != nullptr, "NULL handle");
+  need(ia >= 0 && ia < (int32_t)s->ias.size(), "no such interaction");
+  return *s->ias[(size_t)ia];
+}
+
+int espp_lj_set(espp_sim* s, int32_t ia, int32_t t1, int32_t t2, double eps, double sigma, double cutoff,
+                double shift) {
+  return guarded([&] {
+    InteractionState& a = get_ia(s, ia);
+    need(a.kind == 0, "not a Lennard-Jones interaction");
+    need(t1 >= 0 && t2 >= 0 && t1 < kMaxTypes && t2 < kMaxTypes, "particle type out of range");
+    LJParam p{};
+    // LennardJones::preset (src/interaction/LennardJones.hpp:79-87)
+    double sig2 = sigma * sigma, sig6 = sig2 * sig2 * sig2;
+    p.dff1 = 48.0 * eps * sig6 * sig6;
+    p.dff2 = 24.0 * eps * sig6;
+    p.drc2 = cutoff * cutoff;
+    p.sig2 = sig2;
+    p.eps4 = 4.0 * eps;
+    if (shift != shift) {
+      // setAutoShift (src/interaction/Potential.hpp:214-222)
+      if (std::isinf(cutoff)) p.dshift = 0.0;
+      else {
+        double frac2 = sigma * sigma / p.drc2, frac6 = frac2 * frac2 * frac2;
+        p.dshift = 4.0 * eps * (frac6 * frac6 - frac6);
+      }
+    } else {
+      p.dshift = shift;
+    }
+    p.ff1 = (float)p.dff1;
+    p.ff2 = (float)p.dff2;
+    p.rc2 = std::isinf(p.drc2) ? INFINITY : (float)p.drc2;
+    p.shift = (float)p.dshift;
+    p.set = 1;
+    a.tab[(size_t)t1 * kMaxTypes + t2] = p;
+    a.tab[(size_t)t2 * kMaxTypes + t1] = p;
+    a.maxset = std::max(a.maxset, std::max(t1, t2));
+    a.dirty = true;
+  });
+}
+
+int espp_fene_create(espp_sim* s, int32_t fpl, double K, double r0, double rMax, double cutoff, double shift,
+                     int32_t* ia) {
+  return guarded([&] {
+    need(s && ia, "NULL argument");
+    need(fpl >= 0 && fpl < (int32_t)s->fpls.size(), "no such fixed pair list");
+    auto* a = new InteractionState();
+    a->kind = 1;
+    a->list = fpl;
+    a->fp.K = K;
+    a->fp.r0 = r0;
+    a->fp.rMax = rMax;
+    a->fp.rMaxSqr = rMax * rMax;
+    a->fp.cutoffSqr = cutoff * cutoff;
+    a->fp.shift = shift;
+    s->ias.push_back(a);
+    *ia = (int32_t)s->ias.size() - 1;
+  });
+}
+
+int espp_ia_energy(espp_sim* s, int32_t ia, double* e) {
+  return guarded([&] {
+    InteractionState& a = get_ia(s, ia);
+    need(e != nullptr, "NULL argument");
+    double o[8];
+    s->interaction_sums(a, o);
+    *e = o[0];
+  });
+}
+
+int espp_ia_virial(espp_sim* s, int32_t ia, double* w, double* t6) {
+  return guarded([&] {
+    InteractionState& a = get_ia(s, ia);
+    need(w != nullptr, "NULL argument");
+    double o[8];
+    s->interaction_sums(a, o);
+    *w = o[1];
+    if (t6) for (int q = 0; q < 6; ++q) t6[q] = o[2 + q];
+  });
+}
+
+int espp_add_interaction(espp_sim* s, int32_t ia) {
+  return guarded([&] {
+    get_ia(s, ia);
+    for (int k : s->active) need(k != ia, "interaction already added");
+    s->active.push_back(ia);
+  });
+}
+
+int espp_compute_forces(espp_sim* s) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    s->calc_forces();
+    s->sync();
+  });
+}
+
+int espp_langevin(espp_sim* s, int32_t on, double gamma, double temperature, int64_t nexcl, const int32_t* excl) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    need(nexcl >= 0 && (nexcl == 0 || excl), "invalid exclusion list");
+    s->lang = on != 0;
+    s->gamma = gamma;
+    s->temperature = temperature;
+    for (int64_t k = 0; k < nexcl; ++k) s->lang_excl.push_back(excl[k]);
+    s->excl_dirty = true;
+  });
+}
+
+int espp_run(espp_sim* s, int32_t nsteps, double dt) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    need(nsteps >= 0, "nsteps must be non-negative");
+    // MDIntegrator::setTimeStep (src/integrator/MDIntegrator.cpp:51-63)
+    if (dt == 0.0) fail(ESPP_ERR_ARG, "Timestep 'dt' must be non-zero!");
+    s->run(nsteps, dt);
+  });
+}
+
+int espp_get_step(espp_sim* s, int64_t* step) {
+  return guarded([&] {
+    need(s && step, "NULL argument");
+    *step = s->step;
+  });
+}
+
+int espp_set_step(espp_sim* s, int64_t step) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    s->step = step;
+  });
+}
+
+int espp_num_resorts(espp_sim* s, int64_t* nr) {
+  return guarded([&] {
+    need(s && nr, "NULL argument");
+    *nr = s->nResorts;
+  });
+}
+
+int espp_timers(espp_sim* s, double out10[10]) {
+  return guarded([&] {
+    need(s && out10, "NULL argument");
+    std::memcpy(out10, s->timers, sizeof(s->timers));
+  });
+}
+
+static void kinetic_sums(espp_sim* s, unsigned long long mask, double out8[8]) {
+  for (int q = 0; q < 8; ++q) out8[q] = 0.0;
+  if (s->n == 0) return;
+  int nblk = kinetic_blocks((int)s->n);
+  s->partial.resize((size_t)nblk * 8);
+  launch_kinetic(s->stream, (int)s->n, s->vel.p, s->pos.p, mask, s->partial.p);
+  s->reduce(nblk, 8, out8);
+}
+
+int espp_kinetic(espp_sim* s, double* mv2) {
+  return guarded([&] {
+    need(s && mv2, "NULL argument");
+    double o[8];
+    kinetic_sums(s, ~0ull, o);
+    *mv2 = o[0];
+  });
+}
+
+int espp_kinetic_tensor(espp_sim* s, uint64_t type_mask, double* mv2, int64_t* count, double* tensor6) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    double o[8];
+    kinetic_sums(s, (unsigned long long)type_mask, o);
+    if (mv2) *mv2 = o[0];
+    if (count) *count = (int64_t)std::llround(o[1]);
+    if (tensor6) for (int q = 0; q < 6; ++q) tensor6[q] = o[2 + q];
+  });
+}
+
+// ------------------------------------------------------------------ instrumentation
+
+int espp_device_synchronize(void) {
+  return guarded([&] { hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
+}
+
+int espp_prof_enable(espp_sim* s, int32_t on) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    s->prof_collect();
+    s->prof = on != 0;
+  });
+}
+
+int espp_prof_reset(espp_sim* s) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    s->prof_collect();
+    for (int k = 0; k < 8; ++k) { s->prof_count[k] = 0; s->prof_ms[k] = 0.0; }
+  });
+}
+
+int espp_prof_stats(espp_sim* s, int32_t kernel, int64_t* launches, double* total_ms) {
+  return guarded([&] {
+    need(s && launches && total_ms, "NULL argument");
+    need(kernel >= 0 && kernel < 8, "kernel id out of range");
+    s->prof_collect();
+    *launches = s->prof_count[kernel];
+    *total_ms = s->prof_ms[kernel];
+  });
+}
+
+int espp_vl_stats(espp_sim* s, int32_t vl, int64_t* entries, int32_t* cap, int32_t* maxcount) {
+  return guarded([&] {
+    VerletListState& v = get_vl(s, vl);
+    s->ensure_lists();
+    s->sync();
+    std::vector<int> c = download(v.cnt, (size_t)s->n);
+    int64_t tot = 0;
+    for (int x : c) tot += x;
+    if (entries) *entries = tot;
+    if (cap) *cap = v.cap;
+    if (maxcount) *maxcount = v.maxcount;
+  });
+}
+
+}  // extern "C"

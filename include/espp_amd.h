@@ -1,0 +1,195 @@
+/*
+ * espp_amd.h -- C ABI of the MI355X-native short-range MD step of ESPResSo++.
+ *
+ * One handle (espp_sim) owns every device buffer of one periodic system: the
+ * cell-sorted particles, the Verlet lists, the bond lists, the interactions and
+ * the integrator state.  Caller arrays are borrowed and copied; no device
+ * pointer escapes.  Every call returns 0 on success and a negative code on
+ * failure; espp_last_error() then describes the failure (thread-local).
+ * Calls on one handle are not re-entrant; all work of a handle runs on one HIP
+ * stream of the current device.
+ *
+ * Each entry point names the reference interface it replaces (Boost.Python
+ * registration and C++ implementation, paths relative to the ESPResSo++ tree).
+ */
+#ifndef ESPP_AMD_H
+#define ESPP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct espp_sim espp_sim;
+
+/* error codes */
+#define ESPP_OK 0
+#define ESPP_ERR_ARG (-1)          /* invalid argument (ValueError in Python) */
+#define ESPP_ERR_STATE (-2)        /* call not valid in the current state */
+#define ESPP_ERR_DEVICE (-3)       /* HIP runtime failure or no GPU */
+#define ESPP_ERR_NAN (-4)          /* a particle position became NaN */
+#define ESPP_ERR_OOM (-5)          /* device allocation failed */
+
+/* precision of the pair path */
+#define ESPP_FP64 0                /* all-FP64 pair forces */
+#define ESPP_MIXED 1               /* FP32 pair math on 32-bit fixed-point positions,
+                                      FP64 integration, list build and reductions */
+
+/* particle fields for espp_get_particles / espp_set_particles */
+#define ESPP_FIELD_POS 0
+#define ESPP_FIELD_VEL 1
+#define ESPP_FIELD_FORCE 2
+#define ESPP_FIELD_IMAGE 3
+#define ESPP_FIELD_TYPE 4
+#define ESPP_FIELD_MASS 5
+
+/* Last error message of the calling thread ("" when none). */
+const char *espp_last_error(void);
+
+/* Library version string. */
+const char *espp_version(void);
+
+/* Number of usable GPUs (0 when no HIP device is present). */
+int espp_device_count(int32_t *count);
+
+/* System + OrthorhombicBC + storage::DomainDecomposition for one node.
+ * Replaces System() (src/System.cpp:206-226), bc::OrthorhombicBC (src/bc/OrthorhombicBC.cpp:35-50)
+ * and DomainDecomposition::DomainDecomposition / createCellGrid / initCellInteractions
+ * (src/storage/DomainDecomposition.cpp:70-170,325-374).  cell_grid is the inner
+ * cell grid (tools.decomp.cellGrid), skin is System.skin, seed the RNG seed.
+ * precision: ESPP_FP64 or ESPP_MIXED. */
+int espp_create(const double box[3], const int32_t cell_grid[3], double skin, uint64_t seed,
+                int32_t precision, espp_sim **out);
+
+/* Frees the handle and all its device memory. */
+int espp_destroy(espp_sim *sim);
+
+/* System::setSkin / getSkin (src/System.cpp:207). Takes effect at the next rebuild. */
+int espp_set_skin(espp_sim *sim, double skin);
+
+/* Storage::addParticle for n particles (src/storage/Storage.cpp:285-313, bound at
+ * src/storage/Storage.cpp:1037; Python loop at src/storage/Storage.py:233-330).
+ * Positions are folded on insertion.  type, vel and mass may be NULL (0, 0, 1). */
+int espp_add_particles(espp_sim *sim, int64_t n, const int32_t *id, const int32_t *type,
+                       const double *pos, const double *vel, const double *mass);
+
+/* Storage::decompose (src/storage/Storage.cpp:598-604, bound at :1049): fold,
+ * re-sort into cells and fire onParticlesChanged (rebuild every Verlet list). */
+int espp_decompose(espp_sim *sim);
+
+/* Number of particles (Storage::getNRealParticles). */
+int espp_num_particles(espp_sim *sim, int64_t *n);
+
+/* Ids of all particles in ascending order (Storage::getRealParticleIDs,
+ * src/storage/Storage.cpp:1050).  out holds espp_num_particles entries. */
+int espp_get_ids(espp_sim *sim, int32_t *out);
+
+/* Particle properties in ascending id order (ParticleAccess getters of
+ * src/Particle.py: pos, v, f, imageBox, type, mass).  out holds 3 doubles per
+ * particle for POS/VEL/FORCE/IMAGE and 1 for TYPE/MASS. */
+int espp_get_particles(espp_sim *sim, int32_t field, double *out);
+
+/* Set a property of n particles given by id (Particle.pos/.v/.type/.mass setters,
+ * src/Particle.py).  values: 3 per particle for POS/VEL, 1 for TYPE/MASS. */
+int espp_set_particles(espp_sim *sim, int32_t field, int64_t n, const int32_t *ids,
+                       const double *values);
+
+/* VerletList(system, cutoff) (src/VerletList.cpp:44-67, bound at :373-385):
+ * list of all pairs with d^2 <= (cutoff+skin)^2, rebuilt on every decompose. */
+int espp_vl_create(espp_sim *sim, double cutoff, int32_t *vl);
+
+/* VerletList::exclude for n id pairs (src/VerletList.cpp:327-332), then rebuild. */
+int espp_vl_exclude(espp_sim *sim, int32_t vl, int64_t n, const int32_t *pairs);
+
+/* VerletList::rebuild (src/VerletList.cpp:86-117). */
+int espp_vl_rebuild(espp_sim *sim, int32_t vl);
+
+/* VerletList::totalSize: number of distinct pairs (src/VerletList.cpp:302-310). */
+int espp_vl_size(espp_sim *sim, int32_t vl, int64_t *npairs);
+
+/* VerletList.builds (src/VerletList.cpp:378). */
+int espp_vl_builds(espp_sim *sim, int32_t vl, int64_t *builds);
+
+/* All pairs as (id1, id2) with id1 < id2 (VerletList::getPair / getAllPairs,
+ * src/VerletList.cpp:314-325).  Writes min(cap, total) pairs and returns the
+ * total in *n. */
+int espp_vl_pairs(espp_sim *sim, int32_t vl, int32_t *out, int64_t cap, int64_t *n);
+
+/* VerletList.getVerletCutoff (src/VerletList.cpp:69). */
+int espp_vl_cutoff(espp_sim *sim, int32_t vl, double *cut_verlet);
+
+/* FixedPairList(storage) (src/FixedPairList.cpp:52-66, bound at :346-355). */
+int espp_fpl_create(espp_sim *sim, int32_t *fpl);
+
+/* FixedPairList::add for n id pairs (src/FixedPairList.cpp:88-141). */
+int espp_fpl_add(espp_sim *sim, int32_t fpl, int64_t n, const int32_t *pairs);
+
+/* FixedPairList::totalSize (src/FixedPairList.cpp:326-333). */
+int espp_fpl_size(espp_sim *sim, int32_t fpl, int64_t *n);
+
+/* interaction.VerletListLennardJones(vl) (src/interaction/LennardJones.cpp:105-109). */
+int espp_lj_create(espp_sim *sim, int32_t vl, int32_t *ia);
+
+/* VerletListInteractionTemplate::setPotential(type1, type2, LennardJones(eps,
+ * sigma, cutoff, shift)) (src/interaction/VerletListInteractionTemplate.hpp:65-75,
+ * src/interaction/LennardJones.hpp:60-75).  shift = NaN selects 'auto'. */
+int espp_lj_set(espp_sim *sim, int32_t ia, int32_t type1, int32_t type2, double epsilon,
+                double sigma, double cutoff, double shift);
+
+/* interaction.FixedPairListFENE(system, fpl, FENE(K, r0, rMax, cutoff, shift))
+ * (src/interaction/FENE.cpp:39-53, src/interaction/FENE.hpp:40-112). */
+int espp_fene_create(espp_sim *sim, int32_t fpl, double K, double r0, double rMax,
+                     double cutoff, double shift, int32_t *ia);
+
+/* System::addInteraction (src/System.cpp:75-85, bound at :221): appends the
+ * interaction to the short-range list whose order is the force summation order. */
+int espp_add_interaction(espp_sim *sim, int32_t ia);
+
+/* Interaction::computeEnergy (src/interaction/Interaction.cpp:46). */
+int espp_ia_energy(espp_sim *sim, int32_t ia, double *energy);
+
+/* Interaction::computeVirial and computeVirialTensor (src/interaction/Interaction.cpp:52;
+ * src/interaction/VerletListInteractionTemplate.hpp:274-331).  tensor6 (may be NULL)
+ * receives xx, yy, zz, xy, xz, yz. */
+int espp_ia_virial(espp_sim *sim, int32_t ia, double *virial, double *tensor6);
+
+/* VelocityVerlet::calcForces without thermostat (src/integrator/VelocityVerlet.cpp:307-330). */
+int espp_compute_forces(espp_sim *sim);
+
+/* LangevinThermostat gamma/temperature/addExclusions and addExtension/disconnect
+ * (src/integrator/LangevinThermostat.cpp:40-110,224-233).  on=0 disconnects. */
+int espp_langevin(espp_sim *sim, int32_t on, double gamma, double temperature, int64_t nexcl,
+                  const int32_t *excl);
+
+/* MDIntegrator::run -> VelocityVerlet::run (src/integrator/MDIntegrator.cpp:103,
+ * src/integrator/VelocityVerlet.cpp:63-169) with time step dt. */
+int espp_run(espp_sim *sim, int32_t nsteps, double dt);
+
+/* MDIntegrator.step property (src/integrator/MDIntegrator.cpp:101). */
+int espp_get_step(espp_sim *sim, int64_t *step);
+int espp_set_step(espp_sim *sim, int64_t step);
+
+/* VelocityVerlet::getNumResorts (src/integrator/VelocityVerlet.cpp:434). */
+int espp_num_resorts(espp_sim *sim, int64_t *n);
+
+/* VelocityVerlet::getTimers (src/integrator/VelocityVerlet.cpp:165-204): Run,
+ * ForceComp[0..2], UpdateGhosts, CollectGhostForces, Integrate1, Integrate2,
+ * Resort, Lost (seconds of the last run()). */
+int espp_timers(espp_sim *sim, double out10[10]);
+
+/* Sum of m v^2 over all particles (src/analysis/Temperature.hpp:112-122). */
+int espp_kinetic(espp_sim *sim, double *mv2);
+
+/* Kinetic sums over particles whose type t < 64 has bit t set in type_mask
+ * (Temperature with add_type, src/analysis/Temperature.hpp:100-122; kinetic part of
+ * PressureTensor, src/analysis/PressureTensor.hpp:51-82): sum m v^2, the particle
+ * count and the tensor xx, yy, zz, xy, xz, yz.  Each output may be NULL. */
+int espp_kinetic_tensor(espp_sim *sim, uint64_t type_mask, double *mv2, int64_t *count,
+                        double *tensor6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ESPP_AMD_H */

@@ -1,0 +1,48 @@
+/*
+ * espp_amd_prof.h -- instrumentation entry points of libespp_amd.so.
+ *
+ * These have no counterpart in the reference: they serve measurement
+ * (benchmarks, roofline accounting) and are not needed by a drop-in user.
+ * Same conventions as espp_amd.h (0 on success, espp_last_error on failure).
+ */
+#ifndef ESPP_AMD_PROF_H
+#define ESPP_AMD_PROF_H
+
+#include <stdint.h>
+
+#include "espp_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* kernel ids for espp_prof_stats */
+#define ESPP_K_PAIR 0        /* Lennard-Jones pair-force kernel */
+#define ESPP_K_NBR_BUILD 1   /* Verlet-list build */
+#define ESPP_K_INTEGRATE1 2  /* half kick + drift + max displacement */
+#define ESPP_K_INTEGRATE2 3  /* thermostat + half kick */
+#define ESPP_K_BINNING 4     /* fold, counting sort and permutation (all kernels of one binning) */
+#define ESPP_K_BOND 5        /* FENE bond-force kernel */
+
+/* hipDeviceSynchronize on the current device. */
+int espp_device_synchronize(void);
+
+/* Record HIP events around every launch of the kernels above on the handle's
+ * stream (on != 0) or stop recording (on == 0). */
+int espp_prof_enable(espp_sim *sim, int32_t on);
+
+/* Clear the accumulated launch counts and times. */
+int espp_prof_reset(espp_sim *sim);
+
+/* Launch count and summed event time (ms) of one kernel id since the last reset. */
+int espp_prof_stats(espp_sim *sim, int32_t kernel, int64_t *launches, double *total_ms);
+
+/* Total neighbour entries of the full list (each pair counted from both ends),
+ * the padded row capacity and the largest row of the last build. */
+int espp_vl_stats(espp_sim *sim, int32_t vl, int64_t *entries, int32_t *cap, int32_t *maxcount);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ESPP_AMD_PROF_H */

@@ -1,0 +1,935 @@
+/*
+ * espp_oracle.c -- FP64 CPU restatement of ESPResSo++'s short-range MD step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * by nothing else.  The product path (espressopp_amd, libespp_amd.so) never
+ * links, loads or calls it.
+ *
+ * What it restates (reference paths are relative to the ESPResSo++ tree):
+ *   - cell grid geometry and particle->cell mapping
+ *       src/storage/CellGrid.cpp:40-60 (cellSize = (right-left)/n, inv = 1/cellSize)
+ *       src/storage/CellGrid.cpp:98-136 (mapPositionToCellChecked, truncation + clip)
+ *       src/esutil/Grid.hpp:68-71 (linear index x + nx*(y + ny*z))
+ *   - folding: src/bc/OrthorhombicBC.cpp:144-161 (tmp=floor(x*invL); img+=tmp; x-=tmp*L)
+ *   - minimum image for bonds: src/bc/OrthorhombicBC.cpp:86-105
+ *   - half-shell cell neighbours: src/storage/DomainDecomposition.cpp:325-374
+ *       (useForAllPairs = cell2Idx < cellIdx; the list uses the others)
+ *   - ghost shift: src/storage/DomainDecomposition.cpp:1320-1343 with
+ *       src/storage/NodeGrid.cpp:117-118 (right ghost frame = left cells + L)
+ *       and src/Particle.hpp:93-98 (ghost position = p + shift)
+ *   - Verlet list: src/VerletList.cpp:44-67 (cutsq=(rc+skin)^2),
+ *       src/VerletList.cpp:123-279 (_rebuildUsingBuffers: self cell p2>p1,
+ *       then half-shell buffer; include iff d.sqr() <= cutsq; exclusions both ways)
+ *   - Lennard-Jones: src/interaction/LennardJones.hpp:79-160,
+ *       src/interaction/Potential.hpp:190-235,360-430 (cut test distSqr>cutoffSqr)
+ *   - FENE: src/interaction/FENE.hpp:91-112
+ *   - pair loops with Newton-3, energy, virial, virial tensor:
+ *       src/interaction/VerletListInteractionTemplate.hpp:116-331
+ *       src/interaction/FixedPairListInteractionTemplate.hpp:112-262,400-462
+ *   - velocity Verlet loop and skin trigger: src/integrator/VelocityVerlet.cpp:63-169,236-305,307-371
+ *   - Langevin: src/integrator/LangevinThermostat.cpp:141-214 (uniform noise,
+ *       pref1=-gamma, pref2=sqrt(24 T gamma/dt), heatUp *sqrt(3) at run() start)
+ *   - observables: src/analysis/Temperature.hpp:60-125, src/analysis/Pressure.cpp:41-121
+ *
+ * Arithmetic follows the reference's operation order with no FMA contraction
+ * (the reference builds with -std=c++17, ISO mode, baseline x86-64): build this
+ * file with -ffp-contract=off.
+ *
+ * Deliberate differences, documented because they cannot be pinned:
+ *   - RNG: the reference uses boost::lagged_fibonacci607 (src/esutil/RNG.cpp:40-66).
+ *     Here the Langevin noise comes from Philox4x32-10 (Salmon et al., SC'11)
+ *     keyed (seed, thermostat call, particle id), the same stream the GPU path
+ *     uses, so thermostatted GPU and oracle runs can be compared step by step.
+ *   - particle order inside a cell is ascending insertion index; the reference
+ *     keeps migration order.  Pair sets do not depend on it; FP64 force sums
+ *     differ only by rounding.
+ *   - type pairs without a potential are skipped (the reference evaluates a
+ *     default LennardJones(eps=0, sigma=0) that yields zero force).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_MAXVL 16
+#define OR_MAXFPL 16
+#define OR_MAXIA 32
+#define OR_MAXT 64
+
+typedef struct {
+    int set;
+    double eps, sig, rc, shift;
+    double ff1, ff2, rc2;
+} or_lj;
+
+typedef struct {
+    double cut, cutsq;
+    int64_t npairs, cap;
+    int32_t *p1, *p2;
+    int8_t *sh;             /* 3 per pair: image shift applied to p2 */
+    int64_t builds;
+    uint64_t *excl;          /* sorted, packed (min id, max id) */
+    int64_t nexcl;
+    int max_type;
+    int64_t *cell_pstart;    /* pairs of inner cell c are [cell_pstart[c], cell_pstart[c+1]) */
+} or_vl;
+
+typedef struct {
+    int64_t n, cap;
+    int32_t *a, *b;          /* bond ids, stored as given (a, b) */
+} or_fpl;
+
+typedef struct {
+    int kind;                /* 0: VerletList LJ, 1: FixedPairList FENE */
+    int list;
+    or_lj lj[OR_MAXT * OR_MAXT];
+    double K, r0, rMax, rMaxSqr, cutoff, cutoffSqr, shift;
+} or_ia;
+
+typedef struct or_sim {
+    double L[3], invL[3], halfL[3];
+    int n[3];
+    double cs[3], ics[3];
+    int64_t ncell;
+    double skin;
+    uint64_t seed;
+
+    int64_t np, cap;
+    int32_t *id, *type, *img;
+    double *x, *v, *f, *m;
+    int32_t *loc_of_id;
+    int64_t nid;
+
+    int32_t *cell_of;
+    int64_t *cstart;
+    int32_t *clist;
+
+    or_vl vl[OR_MAXVL];
+    int nvl;
+    or_fpl fpl[OR_MAXFPL];
+    int nfpl;
+    or_ia *ia[OR_MAXIA];
+    int nia;
+
+    double dt;
+    int64_t step;
+    double maxDist;
+    int resortFlag;
+    int64_t nResorts;
+
+    int lang_on;
+    double gamma, temperature, pref1, pref2;
+    int32_t *lang_excl;
+    int64_t nlang_excl;
+    uint64_t therm_calls;
+
+    int nthreads;
+    double t_force, t_resort, t_int1, t_int2, t_run;
+    char err[256];
+} or_sim;
+
+static void assign_cells(or_sim *s, int64_t i0, int64_t i1);
+static void build_cell_lists(or_sim *s);
+
+static void or_seterr(or_sim *s, const char *msg) {
+    if (s) { strncpy(s->err, msg, sizeof(s->err) - 1); s->err[sizeof(s->err) - 1] = 0; }
+}
+
+const char *or_last_error(or_sim *s) { return s ? s->err : "null handle"; }
+
+/* ---------------------------------------------------------------- Philox */
+/* Philox4x32-10, Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+ * as 1, 2, 3", SC'11.  Multipliers and Weyl increments are the published ones. */
+static inline void philox_round(uint32_t c[4], const uint32_t k[2]) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k[0];
+    uint32_t n2 = hi0 ^ c[3] ^ k[1];
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+}
+
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    uint32_t k[2] = {key[0], key[1]};
+    for (int r = 0; r < 10; ++r) {
+        philox_round(c, k);
+        if (r < 9) { k[0] += 0x9E3779B9u; k[1] += 0xBB67AE85u; }
+    }
+    out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = c[3];
+}
+
+/* Three uniforms in (0,1) for (seed, call, particle id); shared with the GPU path. */
+static inline void langevin_uniforms(uint64_t seed, uint64_t call, int32_t pid, double u[3]) {
+    uint32_t ctr[4] = {(uint32_t)pid, (uint32_t)call, (uint32_t)(call >> 32), 0x4C414E47u};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    or_philox4x32_10(ctr, key, o);
+    for (int k = 0; k < 3; ++k) u[k] = ((double)o[k] + 0.5) * 2.3283064365386963e-10;
+}
+
+void or_langevin_uniforms(uint64_t seed, uint64_t call, int32_t pid, double *u3) {
+    langevin_uniforms(seed, call, pid, u3);
+}
+
+/* ---------------------------------------------------------------- lifecycle */
+or_sim *or_create(const double *L, const int *cells, double skin, uint64_t seed) {
+    or_sim *s = (or_sim *)calloc(1, sizeof(or_sim));
+    if (!s) return NULL;
+    for (int d = 0; d < 3; ++d) {
+        s->L[d] = L[d];
+        s->invL[d] = 1.0 / L[d];
+        s->halfL[d] = 0.5 * L[d];
+        s->n[d] = cells[d];
+        /* CellGrid: cellSize = (myRight - myLeft) / n with myLeft = 0, myRight = L */
+        s->cs[d] = (L[d] - 0.0) / (double)cells[d];
+        s->ics[d] = 1.0 / s->cs[d];
+    }
+    s->ncell = (int64_t)cells[0] * cells[1] * cells[2];
+    s->skin = skin;
+    s->seed = seed;
+    s->dt = 0.005;
+    s->resortFlag = 1;
+    s->nthreads = 1;
+    s->cstart = (int64_t *)calloc((size_t)s->ncell + 1, sizeof(int64_t));
+    return s;
+}
+
+static void vl_free(or_vl *v) {
+    free(v->p1); free(v->p2); free(v->sh); free(v->excl); free(v->cell_pstart);
+    memset(v, 0, sizeof(*v));
+}
+
+void or_destroy(or_sim *s) {
+    if (!s) return;
+    free(s->id); free(s->type); free(s->img); free(s->x); free(s->v); free(s->f); free(s->m);
+    free(s->loc_of_id); free(s->cell_of); free(s->cstart); free(s->clist);
+    for (int i = 0; i < s->nvl; ++i) vl_free(&s->vl[i]);
+    for (int i = 0; i < s->nfpl; ++i) { free(s->fpl[i].a); free(s->fpl[i].b); }
+    for (int i = 0; i < s->nia; ++i) free(s->ia[i]);
+    free(s->lang_excl);
+    free(s);
+}
+
+void or_set_threads(or_sim *s, int n) { s->nthreads = n > 0 ? n : 1; }
+void or_set_skin(or_sim *s, double skin) { s->skin = skin; }
+void or_set_dt(or_sim *s, double dt) { s->dt = dt; }
+int64_t or_get_step(or_sim *s) { return s->step; }
+void or_set_step(or_sim *s, int64_t st) { s->step = st; }
+int64_t or_num_resorts(or_sim *s) { return s->nResorts; }
+int64_t or_num_particles(or_sim *s) { return s->np; }
+
+/* fold one coordinate: src/bc/OrthorhombicBC.cpp:144-161 */
+static inline void fold_coord(const or_sim *s, double *x, int32_t *img, int d) {
+    int tmp = (int)floor(x[d] * s->invL[d]);
+    img[d] += tmp;
+    x[d] -= tmp * s->L[d];
+}
+
+/* inner cell coordinate of a folded position: CellGrid::mapPositionToCellChecked
+ * for a single node (myLeft = 0), returning the clipped inner coordinate. */
+static inline int cell_coord(const or_sim *s, double xd, int d) {
+    double lpos = xd - 0.0;
+    int c = (int)(lpos * s->ics[d]) + 1;           /* + frame */
+    if (c < 1) c = 1;
+    else if (c >= s->n[d] + 2 - 1) c = s->n[d] + 2 - 1 - 1;
+    return c - 1;
+}
+
+static int ensure_cap(or_sim *s, int64_t need) {
+    if (need <= s->cap) return 0;
+    int64_t nc = s->cap ? s->cap : 1024;
+    while (nc < need) nc *= 2;
+#define GROW(p, T, k) do { void *q = realloc(p, (size_t)nc * (k) * sizeof(T)); if (!q) return -1; p = (T *)q; } while (0)
+    GROW(s->id, int32_t, 1); GROW(s->type, int32_t, 1); GROW(s->img, int32_t, 3);
+    GROW(s->x, double, 3); GROW(s->v, double, 3); GROW(s->f, double, 3); GROW(s->m, double, 1);
+    GROW(s->cell_of, int32_t, 1); GROW(s->clist, int32_t, 1);
+#undef GROW
+    s->cap = nc;
+    return 0;
+}
+
+static int rebuild_id_map(or_sim *s) {
+    int64_t maxid = -1;
+    for (int64_t i = 0; i < s->np; ++i) if (s->id[i] > maxid) maxid = s->id[i];
+    free(s->loc_of_id);
+    s->nid = maxid + 1;
+    s->loc_of_id = (int32_t *)malloc((size_t)(s->nid > 0 ? s->nid : 1) * sizeof(int32_t));
+    if (!s->loc_of_id) return -1;
+    for (int64_t i = 0; i < s->nid; ++i) s->loc_of_id[i] = -1;
+    for (int64_t i = 0; i < s->np; ++i) s->loc_of_id[s->id[i]] = (int32_t)i;
+    return 0;
+}
+
+/* Storage::addParticle (src/storage/Storage.cpp:285-313): positions are folded on insertion. */
+int or_add_particles(or_sim *s, int64_t n, const int32_t *id, const int32_t *type,
+                     const double *pos, const double *vel, const double *mass) {
+    if (ensure_cap(s, s->np + n)) { or_seterr(s, "out of memory"); return -1; }
+    for (int64_t k = 0; k < n; ++k) {
+        if (id[k] < 0) { or_seterr(s, "negative particle id"); return -2; }
+        if (id[k] < s->nid && s->loc_of_id[id[k]] >= 0) { or_seterr(s, "particle id already exists"); return -3; }
+        int64_t i = s->np + k;
+        s->id[i] = id[k];
+        s->type[i] = type ? type[k] : 0;
+        for (int d = 0; d < 3; ++d) {
+            s->x[3 * i + d] = pos[3 * k + d];
+            s->v[3 * i + d] = vel ? vel[3 * k + d] : 0.0;
+            s->f[3 * i + d] = 0.0;
+            s->img[3 * i + d] = 0;
+        }
+        s->m[i] = mass ? mass[k] : 1.0;
+        for (int d = 0; d < 3; ++d) fold_coord(s, &s->x[3 * i], &s->img[3 * i], d);
+    }
+    int64_t first = s->np;
+    s->np += n;
+    assign_cells(s, first, s->np);
+    build_cell_lists(s);
+    if (rebuild_id_map(s)) { or_seterr(s, "out of memory"); return -1; }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- cells */
+static void assign_cells(or_sim *s, int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
+        const double *x = &s->x[3 * i];
+        int cx = cell_coord(s, x[0], 0), cy = cell_coord(s, x[1], 1), cz = cell_coord(s, x[2], 2);
+        s->cell_of[i] = (int32_t)(cx + s->n[0] * (cy + s->n[1] * cz));
+    }
+}
+
+/* cell member lists from cell_of, members in ascending insertion index */
+static void build_cell_lists(or_sim *s) {
+    int64_t nc = s->ncell;
+    memset(s->cstart, 0, (size_t)(nc + 1) * sizeof(int64_t));
+    for (int64_t i = 0; i < s->np; ++i) s->cstart[s->cell_of[i] + 1]++;
+    for (int64_t c = 0; c < nc; ++c) s->cstart[c + 1] += s->cstart[c];
+    int64_t *fill = (int64_t *)malloc((size_t)nc * sizeof(int64_t));
+    memcpy(fill, s->cstart, (size_t)nc * sizeof(int64_t));
+    for (int64_t i = 0; i < s->np; ++i) s->clist[fill[s->cell_of[i]]++] = (int32_t)i;
+    free(fill);
+}
+
+static void bin_particles(or_sim *s) {
+    assign_cells(s, 0, s->np);
+    build_cell_lists(s);
+}
+
+/* half-shell offsets in the order DomainDecomposition::initCellInteractions
+ * enumerates neighbours (z outer, then y, then x), keeping those whose framed
+ * linear index is larger than the cell's own (useForAllPairs == false). */
+static const int HALF_SHELL[13][3] = {
+    {1, 0, 0},
+    {-1, 1, 0}, {0, 1, 0}, {1, 1, 0},
+    {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
+    {-1, 0, 1}, {0, 0, 1}, {1, 0, 1},
+    {-1, 1, 1}, {0, 1, 1}, {1, 1, 1},
+};
+
+static int is_excluded(const or_vl *v, int32_t a, int32_t b) {
+    if (!v->nexcl) return 0;
+    uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
+    uint64_t key = ((uint64_t)lo << 32) | hi;
+    int64_t l = 0, r = v->nexcl - 1;
+    while (l <= r) {
+        int64_t mid = (l + r) / 2;
+        if (v->excl[mid] == key) return 1;
+        if (v->excl[mid] < key) l = mid + 1; else r = mid - 1;
+    }
+    return 0;
+}
+
+typedef struct { int64_t n, cap; int32_t *p1, *p2; int8_t *sh; int max_type; } pairbuf;
+
+static void pb_push(pairbuf *b, int32_t i, int32_t j, const int sh[3]) {
+    if (b->n == b->cap) {
+        b->cap = b->cap ? 2 * b->cap : 256;
+        b->p1 = (int32_t *)realloc(b->p1, (size_t)b->cap * sizeof(int32_t));
+        b->p2 = (int32_t *)realloc(b->p2, (size_t)b->cap * sizeof(int32_t));
+        b->sh = (int8_t *)realloc(b->sh, (size_t)b->cap * 3);
+    }
+    b->p1[b->n] = i; b->p2[b->n] = j;
+    b->sh[3 * b->n] = (int8_t)sh[0]; b->sh[3 * b->n + 1] = (int8_t)sh[1]; b->sh[3 * b->n + 2] = (int8_t)sh[2];
+    b->n++;
+}
+
+/* VerletList::_rebuildUsingBuffers for one real cell */
+static void vl_build_cell(const or_sim *s, const or_vl *v, int64_t c, pairbuf *out) {
+    int cx = (int)(c % s->n[0]), cy = (int)((c / s->n[0]) % s->n[1]), cz = (int)(c / ((int64_t)s->n[0] * s->n[1]));
+    int64_t b0 = s->cstart[c], b1 = s->cstart[c + 1];
+    const int zero[3] = {0, 0, 0};
+    for (int64_t a = b0; a < b1; ++a) {
+        int32_t i = s->clist[a];
+        const double *x1 = &s->x[3 * i];
+        /* self loop: CheckPair(p1, p2), d = p1 - p2 */
+        for (int64_t b = a + 1; b < b1; ++b) {
+            int32_t j = s->clist[b];
+            const double *x2 = &s->x[3 * j];
+            double dx = x1[0] - x2[0], dy = x1[1] - x2[1], dz = x1[2] - x2[2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > v->cutsq) continue;
+            if (is_excluded(v, s->id[i], s->id[j])) continue;
+            pb_push(out, i, j, zero);
+            int t = s->type[i] > s->type[j] ? s->type[i] : s->type[j];
+            if (t > out->max_type) out->max_type = t;
+        }
+        /* neighbour loop over the half-shell buffer (ghost copies carry p + shift) */
+        for (int k = 0; k < 13; ++k) {
+            int q[3] = {cx + HALF_SHELL[k][0], cy + HALF_SHELL[k][1], cz + HALF_SHELL[k][2]};
+            int sh[3];
+            for (int d = 0; d < 3; ++d) {
+                sh[d] = 0;
+                if (q[d] < 0) { q[d] += s->n[d]; sh[d] = -1; }
+                else if (q[d] >= s->n[d]) { q[d] -= s->n[d]; sh[d] = 1; }
+            }
+            int64_t c2 = q[0] + (int64_t)s->n[0] * (q[1] + (int64_t)s->n[1] * q[2]);
+            double shift[3] = {sh[0] * s->L[0], sh[1] * s->L[1], sh[2] * s->L[2]};
+            for (int64_t b = s->cstart[c2]; b < s->cstart[c2 + 1]; ++b) {
+                int32_t j = s->clist[b];
+                const double *x2 = &s->x[3 * j];
+                double g0 = x2[0] + shift[0], g1 = x2[1] + shift[1], g2 = x2[2] + shift[2];
+                double dx = x1[0] - g0, dy = x1[1] - g1, dz = x1[2] - g2;
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 > v->cutsq) continue;
+                if (is_excluded(v, s->id[i], s->id[j])) continue;
+                int t = s->type[i] > s->type[j] ? s->type[i] : s->type[j];
+                if (t > out->max_type) out->max_type = t;
+                pb_push(out, i, j, sh);
+            }
+        }
+    }
+}
+
+static int vl_rebuild(or_sim *s, or_vl *v) {
+    double cutVerlet = v->cut + s->skin;
+    v->cutsq = cutVerlet * cutVerlet;
+    int64_t nc = s->ncell;
+    pairbuf *bufs = (pairbuf *)calloc((size_t)nc, sizeof(pairbuf));
+    if (!bufs) return -1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(s->nthreads)
+    for (int64_t c = 0; c < nc; ++c) vl_build_cell(s, v, c, &bufs[c]);
+    int64_t tot = 0;
+    for (int64_t c = 0; c < nc; ++c) tot += bufs[c].n;
+    if (tot > v->cap) {
+        v->cap = tot + tot / 4 + 16;
+        v->p1 = (int32_t *)realloc(v->p1, (size_t)v->cap * sizeof(int32_t));
+        v->p2 = (int32_t *)realloc(v->p2, (size_t)v->cap * sizeof(int32_t));
+        v->sh = (int8_t *)realloc(v->sh, (size_t)v->cap * 3);
+    }
+    free(v->cell_pstart);
+    v->cell_pstart = (int64_t *)malloc((size_t)(nc + 1) * sizeof(int64_t));
+    int64_t o = 0;
+    v->max_type = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+        v->cell_pstart[c] = o;
+        if (bufs[c].n) {
+            memcpy(v->p1 + o, bufs[c].p1, (size_t)bufs[c].n * sizeof(int32_t));
+            memcpy(v->p2 + o, bufs[c].p2, (size_t)bufs[c].n * sizeof(int32_t));
+            memcpy(v->sh + 3 * o, bufs[c].sh, (size_t)bufs[c].n * 3);
+        }
+        if (bufs[c].max_type > v->max_type) v->max_type = bufs[c].max_type;
+        o += bufs[c].n;
+        free(bufs[c].p1); free(bufs[c].p2); free(bufs[c].sh);
+    }
+    v->cell_pstart[nc] = o;
+    v->npairs = o;
+    v->builds++;
+    free(bufs);
+    return 0;
+}
+
+/* Storage::decompose for one node: fold x, y, z, re-sort into cells, then the
+ * onParticlesChanged signal rebuilds every Verlet list. */
+int or_decompose(or_sim *s) {
+    for (int64_t i = 0; i < s->np; ++i) {
+        double *x = &s->x[3 * i];
+        if (x[0] != x[0] || x[1] != x[1] || x[2] != x[2]) {
+            or_seterr(s, "particle has moved to outer space (one or more coordinates are nan)");
+            return -4;
+        }
+        for (int d = 0; d < 3; ++d) fold_coord(s, x, &s->img[3 * i], d);
+    }
+    bin_particles(s);
+    for (int k = 0; k < s->nvl; ++k)
+        if (vl_rebuild(s, &s->vl[k])) { or_seterr(s, "out of memory"); return -1; }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- lists */
+int or_vl_create(or_sim *s, double cutoff) {
+    if (s->nvl >= OR_MAXVL) { or_seterr(s, "too many verlet lists"); return -1; }
+    or_vl *v = &s->vl[s->nvl];
+    memset(v, 0, sizeof(*v));
+    v->cut = cutoff;
+    if (vl_rebuild(s, v)) return -1;
+    return s->nvl++;
+}
+
+static int cmp_u64(const void *a, const void *b) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+int or_vl_exclude(or_sim *s, int vi, int64_t n, const int32_t *pairs) {
+    or_vl *v = &s->vl[vi];
+    v->excl = (uint64_t *)realloc(v->excl, (size_t)(v->nexcl + n) * sizeof(uint64_t));
+    for (int64_t k = 0; k < n; ++k) {
+        uint32_t a = (uint32_t)pairs[2 * k], b = (uint32_t)pairs[2 * k + 1];
+        uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        v->excl[v->nexcl++] = ((uint64_t)lo << 32) | hi;
+    }
+    qsort(v->excl, (size_t)v->nexcl, sizeof(uint64_t), cmp_u64);
+    return vl_rebuild(s, v);
+}
+
+int64_t or_vl_size(or_sim *s, int vi) { return s->vl[vi].npairs; }
+int64_t or_vl_builds(or_sim *s, int vi) { return s->vl[vi].builds; }
+
+/* pairs as particle ids in the reference's orientation (first = real particle) */
+int64_t or_vl_pairs(or_sim *s, int vi, int32_t *out, int64_t cap) {
+    or_vl *v = &s->vl[vi];
+    int64_t n = v->npairs < cap ? v->npairs : cap;
+    for (int64_t k = 0; k < n; ++k) { out[2 * k] = s->id[v->p1[k]]; out[2 * k + 1] = s->id[v->p2[k]]; }
+    return v->npairs;
+}
+
+int or_fpl_create(or_sim *s) {
+    if (s->nfpl >= OR_MAXFPL) { or_seterr(s, "too many fixed pair lists"); return -1; }
+    memset(&s->fpl[s->nfpl], 0, sizeof(or_fpl));
+    return s->nfpl++;
+}
+
+int or_fpl_add(or_sim *s, int li, int64_t n, const int32_t *pairs) {
+    or_fpl *p = &s->fpl[li];
+    if (p->n + n > p->cap) {
+        p->cap = (p->n + n) * 2;
+        p->a = (int32_t *)realloc(p->a, (size_t)p->cap * sizeof(int32_t));
+        p->b = (int32_t *)realloc(p->b, (size_t)p->cap * sizeof(int32_t));
+    }
+    for (int64_t k = 0; k < n; ++k) { p->a[p->n] = pairs[2 * k]; p->b[p->n] = pairs[2 * k + 1]; p->n++; }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- interactions */
+static void lj_preset(or_lj *p) {
+    double sig2 = p->sig * p->sig;
+    double sig6 = sig2 * sig2 * sig2;
+    p->ff1 = 48.0 * p->eps * sig6 * sig6;
+    p->ff2 = 24.0 * p->eps * sig6;
+    p->rc2 = p->rc * p->rc;
+}
+
+static double lj_energy_raw(const or_lj *p, double distSqr) {
+    double frac2 = p->sig * p->sig / distSqr;
+    double frac6 = frac2 * frac2 * frac2;
+    return 4.0 * p->eps * (frac6 * frac6 - frac6);
+}
+
+int or_ia_lj_create(or_sim *s, int vl) {
+    if (s->nia >= OR_MAXIA) { or_seterr(s, "too many interactions"); return -1; }
+    or_ia *a = (or_ia *)calloc(1, sizeof(or_ia));
+    a->kind = 0; a->list = vl;
+    s->ia[s->nia] = a;
+    return s->nia++;
+}
+
+/* shift: NaN selects 'auto' (LennardJones::setAutoShift, src/interaction/Potential.hpp:214-222) */
+int or_ia_lj_set(or_sim *s, int ii, int t1, int t2, double eps, double sig, double rc, double shift) {
+    if (t1 < 0 || t2 < 0 || t1 >= OR_MAXT || t2 >= OR_MAXT) { or_seterr(s, "type out of range"); return -1; }
+    or_lj p;
+    memset(&p, 0, sizeof(p));
+    p.set = 1; p.eps = eps; p.sig = sig; p.rc = rc;
+    lj_preset(&p);
+    if (shift != shift) p.shift = isinf(rc) ? 0.0 : lj_energy_raw(&p, p.rc2);
+    else p.shift = shift;
+    s->ia[ii]->lj[t1 * OR_MAXT + t2] = p;
+    s->ia[ii]->lj[t2 * OR_MAXT + t1] = p;
+    return 0;
+}
+
+double or_lj_energy(double eps, double sig, double rc, double shift, double r) {
+    or_lj p; memset(&p, 0, sizeof(p));
+    p.eps = eps; p.sig = sig; p.rc = rc; lj_preset(&p);
+    if (shift != shift) shift = isinf(rc) ? 0.0 : lj_energy_raw(&p, p.rc2);
+    double d2 = r * r;
+    if (d2 > p.rc2) return 0.0;
+    return lj_energy_raw(&p, d2) - shift;
+}
+
+static inline int lj_force(const or_lj *p, const double d[3], double F[3]) {
+    double distSqr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (distSqr > p->rc2) return 0;
+    double frac2 = 1.0 / distSqr;
+    double frac6 = frac2 * frac2 * frac2;
+    double ffactor = frac6 * (p->ff1 * frac6 - p->ff2) * frac2;
+    F[0] = d[0] * ffactor; F[1] = d[1] * ffactor; F[2] = d[2] * ffactor;
+    return 1;
+}
+
+void or_lj_force(double eps, double sig, double rc, const double *d, double *F) {
+    or_lj p; memset(&p, 0, sizeof(p));
+    p.eps = eps; p.sig = sig; p.rc = rc; lj_preset(&p);
+    if (!lj_force(&p, d, F)) F[0] = F[1] = F[2] = 0.0;
+}
+
+int or_ia_fene_create(or_sim *s, int fpl, double K, double r0, double rMax, double cutoff, double shift) {
+    if (s->nia >= OR_MAXIA) { or_seterr(s, "too many interactions"); return -1; }
+    or_ia *a = (or_ia *)calloc(1, sizeof(or_ia));
+    a->kind = 1; a->list = fpl;
+    a->K = K; a->r0 = r0; a->rMax = rMax; a->rMaxSqr = rMax * rMax;
+    a->cutoff = cutoff; a->cutoffSqr = cutoff * cutoff; a->shift = shift;
+    s->ia[s->nia] = a;
+    return s->nia++;
+}
+
+static double fene_energy_raw(const or_ia *a, double distSqr) {
+    double t = (sqrt(distSqr) - a->r0) / a->rMax;
+    return -0.5 * a->rMaxSqr * a->K * log(1 - pow(t, 2));
+}
+
+static inline int fene_force(const or_ia *a, const double d[3], double F[3]) {
+    double distSqr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (distSqr > a->cutoffSqr) return 0;
+    double ffactor;
+    if (a->r0 > 2.220446049250313e-16) {
+        double r = sqrt(distSqr);
+        ffactor = -a->K * (r - a->r0) / (r * (1 - ((r - a->r0) * (r - a->r0) / a->rMaxSqr)));
+    } else {
+        ffactor = -a->K / (1.0 - distSqr / a->rMaxSqr);
+    }
+    F[0] = d[0] * ffactor; F[1] = d[1] * ffactor; F[2] = d[2] * ffactor;
+    return 1;
+}
+
+double or_fene_energy(double K, double r0, double rMax, double r) {
+    or_ia a; memset(&a, 0, sizeof(a));
+    a.K = K; a.r0 = r0; a.rMax = rMax; a.rMaxSqr = rMax * rMax;
+    return fene_energy_raw(&a, r * r);
+}
+
+void or_fene_force(double K, double r0, double rMax, const double *d, double *F) {
+    or_ia a; memset(&a, 0, sizeof(a));
+    a.K = K; a.r0 = r0; a.rMax = rMax; a.rMaxSqr = rMax * rMax; a.cutoffSqr = INFINITY;
+    fene_force(&a, d, F);
+}
+
+/* OrthorhombicBC::getMinimumImageVectorBox */
+static inline void min_image_box(const or_sim *s, const double *p1, const double *p2, double d[3]) {
+    for (int k = 0; k < 3; ++k) {
+        d[k] = p1[k] - p2[k];
+        if (d[k] < -s->halfL[k]) d[k] += s->L[k];
+        else if (d[k] > s->halfL[k]) d[k] -= s->L[k];
+    }
+}
+
+static inline void pair_dist(const or_sim *s, const or_vl *v, int64_t k, double d[3]) {
+    const double *x1 = &s->x[3 * v->p1[k]], *x2 = &s->x[3 * v->p2[k]];
+    const int8_t *sh = &v->sh[3 * k];
+    for (int c = 0; c < 3; ++c) d[c] = x1[c] - (x2[c] + sh[c] * s->L[c]);
+}
+
+/* LJ over the half list with Newton-3.  Cells are processed in z-layer phases
+ * (even layers, odd layers, then a lone last layer when nz is odd) so that
+ * threads never write the same particle; the per-particle summation order is
+ * the same for every thread count. */
+static void lj_add_forces(or_sim *s, const or_ia *a) {
+    const or_vl *v = &s->vl[a->list];
+    int nz = s->n[2];
+    int64_t per_layer = (int64_t)s->n[0] * s->n[1];
+    int nph = (nz % 2) ? 3 : 2;
+    for (int ph = 0; ph < nph; ++ph) {
+        int zfirst = ph == 2 ? nz - 1 : ph;
+        int zstep = ph == 2 ? 1 : 2;
+        int zend = (ph < 2 && (nz % 2)) ? nz - 1 : nz;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(s->nthreads)
+        for (int z = zfirst; z < zend; z += zstep) {
+            for (int64_t c = z * per_layer; c < (z + 1) * per_layer; ++c) {
+                for (int64_t k = v->cell_pstart[c]; k < v->cell_pstart[c + 1]; ++k) {
+                    int32_t i = v->p1[k], j = v->p2[k];
+                    const or_lj *p = &a->lj[s->type[i] * OR_MAXT + s->type[j]];
+                    if (!p->set) continue;
+                    double d[3], F[3];
+                    pair_dist(s, v, k, d);
+                    if (lj_force(p, d, F)) {
+                        for (int c3 = 0; c3 < 3; ++c3) { s->f[3 * i + c3] += F[c3]; s->f[3 * j + c3] -= F[c3]; }
+                    }
+                }
+            }
+        }
+    }
+}
+
+static int fene_bond_dist(or_sim *s, const or_fpl *p, int64_t k, int32_t *i, int32_t *j, double d[3]) {
+    int32_t a = p->a[k], b = p->b[k];
+    if (a >= s->nid || b >= s->nid || s->loc_of_id[a] < 0 || s->loc_of_id[b] < 0) return 0;
+    *i = s->loc_of_id[a]; *j = s->loc_of_id[b];
+    min_image_box(s, &s->x[3 * *i], &s->x[3 * *j], d);
+    return 1;
+}
+
+static void fene_add_forces(or_sim *s, const or_ia *a) {
+    const or_fpl *p = &s->fpl[a->list];
+    for (int64_t k = 0; k < p->n; ++k) {
+        int32_t i, j; double d[3], F[3];
+        if (!fene_bond_dist(s, p, k, &i, &j, d)) continue;
+        if (fene_force(a, d, F))
+            for (int c = 0; c < 3; ++c) { s->f[3 * i + c] += F[c]; s->f[3 * j + c] -= F[c]; }
+    }
+}
+
+static void thermalize(or_sim *s) {
+    uint64_t call = s->therm_calls++;
+#pragma omp parallel for schedule(static) num_threads(s->nthreads)
+    for (int64_t i = 0; i < s->np; ++i) {
+        int32_t pid = s->id[i];
+        if (s->nlang_excl) {
+            int64_t l = 0, r = s->nlang_excl - 1; int hit = 0;
+            while (l <= r) { int64_t mid = (l + r) / 2; if (s->lang_excl[mid] == pid) { hit = 1; break; } if (s->lang_excl[mid] < pid) l = mid + 1; else r = mid - 1; }
+            if (hit) continue;
+        }
+        double u[3];
+        langevin_uniforms(s->seed, call, pid, u);
+        double mass = s->m[i], massf = sqrt(mass);
+        for (int c = 0; c < 3; ++c) {
+            double ran = u[c] - 0.5;
+            s->f[3 * i + c] += (s->pref1 * s->v[3 * i + c]) * mass + (s->pref2 * ran) * massf;
+        }
+    }
+}
+
+/* VelocityVerlet::calcForces + updateForces + aftCalcF (thermostat) */
+static void update_forces(or_sim *s) {
+    memset(s->f, 0, (size_t)s->np * 3 * sizeof(double));
+    for (int k = 0; k < s->nia; ++k) {
+        if (s->ia[k]->kind == 0) lj_add_forces(s, s->ia[k]);
+        else fene_add_forces(s, s->ia[k]);
+    }
+    if (s->lang_on) thermalize(s);
+}
+
+/* forces of all interactions, no thermostat */
+int or_calc_forces(or_sim *s) {
+    memset(s->f, 0, (size_t)s->np * 3 * sizeof(double));
+    for (int k = 0; k < s->nia; ++k) {
+        if (s->ia[k]->kind == 0) lj_add_forces(s, s->ia[k]);
+        else fene_add_forces(s, s->ia[k]);
+    }
+    return 0;
+}
+
+static int cmp_i32(const void *a, const void *b) {
+    int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    return x < y ? -1 : x > y;
+}
+
+int or_langevin(or_sim *s, int on, double gamma, double T, int64_t nexcl, const int32_t *excl) {
+    s->lang_on = on; s->gamma = gamma; s->temperature = T;
+    if (nexcl > 0) {
+        s->lang_excl = (int32_t *)realloc(s->lang_excl, (size_t)(s->nlang_excl + nexcl) * sizeof(int32_t));
+        memcpy(s->lang_excl + s->nlang_excl, excl, (size_t)nexcl * sizeof(int32_t));
+        s->nlang_excl += nexcl;
+        qsort(s->lang_excl, (size_t)s->nlang_excl, sizeof(int32_t), cmp_i32);
+    }
+    return 0;
+}
+
+void or_set_therm_calls(or_sim *s, uint64_t c) { s->therm_calls = c; }
+uint64_t or_get_therm_calls(or_sim *s) { return s->therm_calls; }
+
+static double integrate1(or_sim *s) {
+    double maxSq = 0.0;
+    double dt = s->dt;
+#pragma omp parallel for schedule(static) reduction(max : maxSq) num_threads(s->nthreads)
+    for (int64_t i = 0; i < s->np; ++i) {
+        double dtfm = 0.5 * dt / s->m[i];
+        double *v = &s->v[3 * i], *x = &s->x[3 * i];
+        const double *f = &s->f[3 * i];
+        double dp[3];
+        for (int c = 0; c < 3; ++c) v[c] += dtfm * f[c];
+        for (int c = 0; c < 3; ++c) { dp[c] = v[c]; dp[c] *= dt; x[c] += dp[c]; }
+        double sq = 0.0;
+        sq += dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+        if (sq > maxSq) maxSq = sq;
+    }
+    return sqrt(maxSq);
+}
+
+static void integrate2(or_sim *s) {
+    double half_dt = 0.5 * s->dt;
+#pragma omp parallel for schedule(static) num_threads(s->nthreads)
+    for (int64_t i = 0; i < s->np; ++i) {
+        double dtfm = half_dt / s->m[i];
+        for (int c = 0; c < 3; ++c) s->v[3 * i + c] += dtfm * s->f[3 * i + c];
+    }
+    s->step++;
+}
+
+static double now_s(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return (double)clock() / CLOCKS_PER_SEC;
+#endif
+}
+
+/* VelocityVerlet::run (src/integrator/VelocityVerlet.cpp:63-169) */
+int or_run(or_sim *s, int nsteps) {
+    if (s->dt == 0.0) { or_seterr(s, "timestep must be non-zero"); return -5; }
+    double t0 = now_s();
+    double skinHalf = 0.5 * s->skin;
+    s->nResorts = 0;
+    if (s->lang_on) { s->pref1 = -s->gamma; s->pref2 = sqrt(24.0 * s->temperature * s->gamma / s->dt); }
+    if (s->resortFlag) {
+        int rc = or_decompose(s); if (rc) return rc;
+        s->maxDist = 0.0; s->resortFlag = 0;
+    }
+    double pref2buffer = s->pref2;
+    s->pref2 *= sqrt(3.0);
+    update_forces(s);
+    s->pref2 = pref2buffer;
+    for (int i = 0; i < nsteps; ++i) {
+        double ta = now_s();
+        s->maxDist += integrate1(s);
+        double tb = now_s();
+        s->t_int1 += tb - ta;
+        if (s->maxDist > skinHalf) s->resortFlag = 1;
+        if (s->resortFlag) {
+            int rc = or_decompose(s); if (rc) return rc;
+            s->maxDist = 0.0; s->resortFlag = 0; s->nResorts++;
+        }
+        double tc = now_s();
+        s->t_resort += tc - tb;
+        update_forces(s);
+        double td = now_s();
+        s->t_force += td - tc;
+        integrate2(s);
+        s->t_int2 += now_s() - td;
+    }
+    s->t_run += now_s() - t0;
+    return 0;
+}
+
+/* ---------------------------------------------------------------- observables */
+double or_ia_energy(or_sim *s, int ii) {
+    const or_ia *a = s->ia[ii];
+    double es = 0.0;
+    if (a->kind == 0) {
+        const or_vl *v = &s->vl[a->list];
+        for (int64_t k = 0; k < v->npairs; ++k) {
+            const or_lj *p = &a->lj[s->type[v->p1[k]] * OR_MAXT + s->type[v->p2[k]]];
+            if (!p->set) continue;
+            double d[3]; pair_dist(s, v, k, d);
+            double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            es += (d2 > p->rc2) ? 0.0 : lj_energy_raw(p, d2) - p->shift;
+        }
+    } else {
+        const or_fpl *p = &s->fpl[a->list];
+        for (int64_t k = 0; k < p->n; ++k) {
+            int32_t i, j; double d[3];
+            if (!fene_bond_dist(s, p, k, &i, &j, d)) continue;
+            double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            es += (d2 > a->cutoffSqr) ? 0.0 : fene_energy_raw(a, d2) - a->shift;
+        }
+    }
+    return es;
+}
+
+/* w6 = xx, yy, zz, xy, xz, yz (Tensor order, src/Tensor.hpp) */
+double or_ia_virial(or_sim *s, int ii, double *w6) {
+    const or_ia *a = s->ia[ii];
+    double w = 0.0, t[6] = {0, 0, 0, 0, 0, 0};
+    if (a->kind == 0) {
+        const or_vl *v = &s->vl[a->list];
+        for (int64_t k = 0; k < v->npairs; ++k) {
+            const or_lj *p = &a->lj[s->type[v->p1[k]] * OR_MAXT + s->type[v->p2[k]]];
+            if (!p->set) continue;
+            double d[3], F[3]; pair_dist(s, v, k, d);
+            if (lj_force(p, d, F)) {
+                w = w + (d[0] * F[0] + d[1] * F[1] + d[2] * F[2]);
+                t[0] += d[0] * F[0]; t[1] += d[1] * F[1]; t[2] += d[2] * F[2];
+                t[3] += d[0] * F[1]; t[4] += d[0] * F[2]; t[5] += d[1] * F[2];
+            }
+        }
+    } else {
+        const or_fpl *p = &s->fpl[a->list];
+        for (int64_t k = 0; k < p->n; ++k) {
+            int32_t i, j; double d[3], F[3];
+            if (!fene_bond_dist(s, p, k, &i, &j, d)) continue;
+            if (fene_force(a, d, F)) {
+                w = w + (d[0] * F[0] + d[1] * F[1] + d[2] * F[2]);
+                t[0] += d[0] * F[0]; t[1] += d[1] * F[1]; t[2] += d[2] * F[2];
+                t[3] += d[0] * F[1]; t[4] += d[0] * F[2]; t[5] += d[1] * F[2];
+            }
+        }
+    }
+    if (w6) memcpy(w6, t, sizeof(t));
+    return w;
+}
+
+double or_temperature(or_sim *s) {
+    double v2sum = 0.0;
+    for (int64_t i = 0; i < s->np; ++i) {
+        const double *v = &s->v[3 * i];
+        v2sum += s->m[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    return v2sum / (3.0 * (double)s->np);
+}
+
+double or_pressure(or_sim *s) {
+    double tripleV = 3.0 * s->L[0] * s->L[1] * s->L[2];
+    double v2 = 0.0;
+    for (int64_t i = 0; i < s->np; ++i) {
+        const double *v = &s->v[3 * i];
+        v2 += s->m[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    double w = 0.0;
+    for (int k = 0; k < s->nia; ++k) w += or_ia_virial(s, k, NULL);
+    return (v2 + w) / tripleV;
+}
+
+/* ---------------------------------------------------------------- accessors */
+/* field: 0 pos, 1 vel, 2 force, 3 image(int as double) ; output in ascending id order */
+int or_get_by_id(or_sim *s, int field, double *out) {
+    int64_t o = 0;
+    for (int64_t pid = 0; pid < s->nid; ++pid) {
+        int32_t i = s->loc_of_id[pid];
+        if (i < 0) continue;
+        for (int c = 0; c < 3; ++c) {
+            double val = field == 0 ? s->x[3 * i + c] : field == 1 ? s->v[3 * i + c]
+                       : field == 2 ? s->f[3 * i + c] : (double)s->img[3 * i + c];
+            out[3 * o + c] = val;
+        }
+        o++;
+    }
+    return 0;
+}
+
+int or_get_ids(or_sim *s, int32_t *out) {
+    int64_t o = 0;
+    for (int64_t pid = 0; pid < s->nid; ++pid) if (s->loc_of_id[pid] >= 0) out[o++] = (int32_t)pid;
+    return 0;
+}
+
+/* field: 0 pos, 1 vel; input in ascending id order */
+int or_set_by_id(or_sim *s, int field, const double *in) {
+    int64_t o = 0;
+    for (int64_t pid = 0; pid < s->nid; ++pid) {
+        int32_t i = s->loc_of_id[pid];
+        if (i < 0) continue;
+        for (int c = 0; c < 3; ++c) {
+            if (field == 0) s->x[3 * i + c] = in[3 * o + c];
+            else s->v[3 * i + c] = in[3 * o + c];
+        }
+        o++;
+    }
+    return 0;
+}
+
+void or_timers(or_sim *s, double *t5) {
+    t5[0] = s->t_run; t5[1] = s->t_force; t5[2] = s->t_int1; t5[3] = s->t_int2; t5[4] = s->t_resort;
+}

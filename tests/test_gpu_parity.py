@@ -1,0 +1,305 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Bars: neighbour pair sets bit-exact; FP64 path forces within 1e-12 relative;
+mixed path per-particle relative force error <= 1e-5 (relative to max(|F_i|,
+0.05 * RMS|F|), the floor covers particles whose net force nearly cancels),
+energies and virials within 1e-6 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+
+from _systems import (cells_for, force_rel_err, gpu_system, jittered_lattice, mb_velocities, pair_set,
+                      random_walk_chains, read_fixture)
+
+pytestmark = pytest.mark.gpu
+
+import espressopp_amd as espressopp  # noqa: E402
+
+
+def build_lj(pos, box, cells, rc, skin, precision, ids=None, types=None, vel=None, lj=((0, 0, 1.0, 1.0),),
+             shift="auto"):
+    n = len(pos)
+    ids = np.arange(n, dtype=np.int32) if ids is None else np.asarray(ids, dtype=np.int32)
+    system = gpu_system(box, cells, skin, precision)
+    system.storage.addParticlesArray(ids, pos, types, vel)
+    system.storage.decompose()
+    vl = espressopp.VerletList(system, cutoff=rc)
+    ia = espressopp.interaction.VerletListLennardJones(vl)
+    for t1, t2, eps, sig in lj:
+        ia.setPotential(t1, t2, espressopp.interaction.LennardJones(eps, sig, rc, shift))
+    system.addInteraction(ia)
+    ref = OracleSim(box, cells, skin)
+    ref.add_particles(ids, pos, types=types, vel=vel)
+    ref.decompose()
+    rvl = ref.vl_create(rc)
+    ria = ref.lj_create(rvl)
+    for t1, t2, eps, sig in lj:
+        ref.lj_set(ria, t1, t2, eps, sig, rc, shift)
+    return system, vl, ia, ref, rvl, ria
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pairs_fixture(gpu, precision):
+    pid, pos, _, box = read_fixture()
+    rc, skin = 2.5, 0.3
+    cells = cells_for(box, rc, skin)
+    system, vl, _, ref, rvl, _ = build_lj(pos, box, cells, rc, skin, precision, ids=pid)
+    got = pair_set(vl.pairs_array())
+    want = pair_set(ref.vl_pairs(rvl))
+    assert vl.totalSize() == ref.vl_size(rvl) == len(want)
+    assert got == want
+
+
+@pytest.mark.parametrize("cut,count", [(0.0, 0), (1.0, 3 * 216), (math.sqrt(2.0), 9 * 216),
+                                       (math.sqrt(3.0), 13 * 216)])
+def test_pairs_lattice_counts(gpu, cut, count):
+    # testsuite/unittest/testVerletList.py:38-109
+    N, SIZE = 6, 6.0
+    system = gpu_system((SIZE,) * 3, (3, 3, 3), 0.001)
+    g = (np.arange(N) + 0.5) / N * SIZE
+    pos = np.array([[x, y, z] for x in g for y in g for z in g])
+    system.storage.addParticlesArray(np.arange(N ** 3), pos)
+    system.storage.decompose()
+    vl = espressopp.VerletList(system, cut)
+    assert vl.totalSize() == count
+
+
+@pytest.mark.parametrize("seed,box,rc,skin", [(11, (9.0, 9.0, 9.0), 2.5, 0.3),
+                                              (12, (8.7, 11.3, 14.9), 2.5, 0.4),
+                                              (13, (6.1, 6.1, 9.3), 1.12, 0.3),
+                                              (14, (20.0, 7.5, 33.0), 1.5, 0.2)])
+def test_pairs_random_boxes(gpu, seed, box, rc, skin):
+    rng = np.random.default_rng(seed)
+    n = 3000
+    pos = rng.uniform(0, 1, (n, 3)) * np.asarray(box)
+    # particles on the box faces and one ulp inside them exercise folding and the ghost shift
+    pos[:6] = 0.0
+    pos[6:12] = np.nextafter(np.asarray(box), 0.0)
+    pos[12, 0] = -1e-300
+    cells = cells_for(box, rc, skin)
+    system, vl, _, ref, rvl, _ = build_lj(pos, box, cells, rc, skin, "fp64")
+    assert pair_set(vl.pairs_array()) == pair_set(ref.vl_pairs(rvl))
+
+
+def test_pairs_borderline_distances(gpu):
+    # pairs placed at (rc+skin) +- a few ulp across the periodic boundary: membership must
+    # follow the reference's FP64 test in the reference's orientation
+    box = (10.0, 10.0, 10.0)
+    rc, skin = 2.5, 0.3
+    cut = rc + skin
+    pos = []
+    for k in range(40):
+        y = 0.2 + 0.24 * k
+        x0 = 0.05 + 0.001 * k
+        for m, dz in enumerate((-2, -1, 0, 1, 2)):
+            z = 0.3 + 2.0 * m
+            d = cut + dz * 4e-16
+            pos.append([x0, y, z])
+            pos.append([x0 - d + 10.0, y, z])
+    pos = np.asarray(pos)
+    system, vl, _, ref, rvl, _ = build_lj(pos, box, (3, 3, 3), rc, skin, "fp64")
+    assert pair_set(vl.pairs_array()) == pair_set(ref.vl_pairs(rvl))
+
+
+def test_forces_fp64_fixture(gpu):
+    pid, pos, _, box = read_fixture()
+    rc, skin = 2.5, 0.3
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells_for(box, rc, skin), rc, skin, "fp64", ids=pid)
+    system.integrator = espressopp.integrator.VelocityVerlet(system)
+    system.integrator.run(0)
+    ref.calc_forces()
+    f = system.storage.getParticleArrays()["f"]
+    assert force_rel_err(f, ref.get("f"), 1e-3).max() < 1e-12
+    assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=1e-12)
+    w, t = ref.virial(ria)
+    assert ia.computeVirial() == pytest.approx(w, rel=1e-12)
+    assert np.allclose(ia.computeVirialTensor(), t, rtol=1e-11, atol=1e-9)
+
+
+@pytest.mark.parametrize("n,seed", [(2048, 0), (32000, 1)])
+def test_forces_mixed(gpu, n, seed):
+    pos, box = jittered_lattice(n, 0.8442, seed)
+    rc, skin = 2.5, 0.3
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells_for(box, rc, skin), rc, skin, "mixed")
+    espressopp.integrator.VelocityVerlet(system).run(0)
+    ref.calc_forces()
+    f = system.storage.getParticleArrays()["f"]
+    err = force_rel_err(f, ref.get("f"), 0.05)
+    print("mixed n=%d: max %.3e  p99.9 %.3e  median %.3e" % (n, err.max(), np.quantile(err, 0.999), np.median(err)))
+    assert err.max() <= 1e-5
+    assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=1e-6)
+    assert ia.computeVirial() == pytest.approx(ref.virial(ria)[0], rel=1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_multitype_forces(gpu, precision):
+    pid, pos, _, box = read_fixture()
+    rc, skin = 2.5, 0.4
+    types = (pid % 3).astype(np.int32)
+    lj = ((0, 0, 0.1, 1.0), (0, 1, (0.1 * 0.3) ** 0.5, 0.8), (1, 1, 0.9, 0.6), (1, 2, 1.0, 1.1), (2, 2, 0.5, 1.0))
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells_for(box, rc, skin), rc, skin, precision,
+                                             ids=pid, types=types, lj=lj, shift=0.0)
+    espressopp.integrator.VelocityVerlet(system).run(0)
+    ref.calc_forces()
+    f = system.storage.getParticleArrays()["f"]
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert force_rel_err(f, ref.get("f"), 0.05).max() <= tol
+    assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=tol if precision == "fp64" else 1e-6)
+
+
+def test_golden_nve_temperature_fp64(gpu):
+    # testsuite/potential_array/potential_array_test.py:37-163 on the GPU FP64 path
+    pid, pos, _, box = read_fixture()
+    rc, skin = 2.5, 0.4
+    cells = espressopp.tools.decomp.cellGrid(box, (1, 1, 1), rc, skin)
+    system = gpu_system(box, cells, skin, "fp64")
+    system.storage.addParticles([[int(p), int(p) % 2, espressopp.Real3D(*x)] for p, x in zip(pid, pos)],
+                                "id", "type", "pos")
+    system.storage.decompose()
+    integrator = espressopp.integrator.VelocityVerlet(system)
+    integrator.dt = 0.005
+    vl = espressopp.VerletList(system, rc)
+    ia = espressopp.interaction.VerletListLennardJones(vl)
+    LJ = espressopp.interaction.LennardJones
+    ia.setPotential(type1=0, type2=0, potential=LJ(epsilon=0.1, sigma=1.0, cutoff=rc, shift=0.0))
+    ia.setPotential(type1=0, type2=1, potential=LJ(epsilon=(0.1 * 0.3) ** 0.5, sigma=0.8, cutoff=rc, shift=0.0))
+    ia.setPotential(type1=1, type2=1, potential=LJ(epsilon=0.9, sigma=0.6, cutoff=rc, shift=0.0))
+    system.addInteraction(ia)
+    integrator.run(100)
+    n = len(pid)
+    system.storage.addParticles([[n + k, 6, espressopp.Real3D(*pos[k])] for k in range(10)], "id", "type", "pos")
+    system.storage.decompose()
+    integrator.run(100)
+    T = espressopp.analysis.Temperature(system).compute()
+    assert abs(T - 0.160074388303) < 1.0e-8
+
+
+def test_nve_trajectory_matches_oracle(gpu):
+    pos, box = jittered_lattice(4000, 0.8442, seed=4)
+    vel = mb_velocities(4000, 1.0, seed=5)
+    rc, skin = 2.5, 0.3
+    cells = cells_for(box, rc, skin)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells, rc, skin, precision, vel=vel)
+        integ = espressopp.integrator.VelocityVerlet(system)
+        integ.dt = 0.005
+        integ.run(50)
+        out[precision] = (system.storage.getParticleArrays(), integ.getNumResorts(), vl.builds)
+    ref.run(50, dt=0.005)
+    rp = ref.get("pos") + ref.get("image") * np.asarray(box)
+    for precision, tol in (("fp64", 1e-9), ("mixed", 1e-4)):
+        a, resorts, builds = out[precision]
+        gp = a["pos"] + a["image"] * np.asarray(box)
+        assert np.abs(gp - rp).max() < tol, precision
+    assert out["fp64"][1] == ref.resorts
+
+
+def test_langevin_matches_oracle_and_exclusion(gpu):
+    pos, box = jittered_lattice(1000, 0.8442, seed=6)
+    rc, skin = 2.5, 0.3
+    cells = cells_for(box, rc, skin)
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells, rc, skin, "fp64")
+    integ = espressopp.integrator.VelocityVerlet(system)
+    integ.dt = 0.005
+    lang = espressopp.integrator.LangevinThermostat(system)
+    lang.gamma = 1.0
+    lang.temperature = 1.0
+    lang.addExclusions([3])
+    integ.addExtension(lang)
+    integ.run(20)
+    ref.langevin(1.0, 1.0, [3])
+    ref.run(20, dt=0.005)
+    a = system.storage.getParticleArrays()
+    assert np.abs(a["pos"] - ref.get("pos")).max() < 1e-9
+    assert np.abs(a["v"] - ref.get("v")).max() < 1e-8
+
+
+def test_langevin_excluded_particle_does_not_move(gpu):
+    # testsuite/LangevinThermostat/testLangevinThermostat.py:84-92
+    box = (10.0, 10.0, 10.0)
+    system = gpu_system(box, (5, 5, 5), 0.3)
+    system.storage.addParticles([(1, 1, espressopp.Real3D(5.5, 5.0, 5.0), 1.0),
+                                 (2, 1, espressopp.Real3D(6.5, 5.0, 5.0), 1.0),
+                                 (3, 1, espressopp.Real3D(7.5, 5.0, 5.0), 1.0)], "id", "type", "pos", "mass")
+    system.storage.decompose()
+    espressopp.VerletList(system, cutoff=1.5)
+    integ = espressopp.integrator.VelocityVerlet(system)
+    integ.dt = 0.01
+    lang = espressopp.integrator.LangevinThermostat(system)
+    lang.gamma = 1.0
+    lang.temperature = 1.0
+    lang.addExclusions([1])
+    integ.addExtension(lang)
+    before = [system.storage.getParticle(i).pos[j] for i in range(1, 4) for j in range(3)]
+    integ.run(10)
+    after = [system.storage.getParticle(i).pos[j] for i in range(1, 4) for j in range(3)]
+    assert before[:3] == after[:3]
+    assert all(b != a for b, a in zip(before[3:], after[3:]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fene_wca_melt(gpu, precision):
+    pos, box, bonds = random_walk_chains(40, 50, 0.85, seed=7)
+    rc, skin = 2 ** (1 / 6), 0.3
+    cells = cells_for(box, rc, skin)
+    n = len(pos)
+    system = gpu_system(box, cells, skin, precision)
+    system.storage.addParticlesArray(np.arange(n), pos)
+    system.storage.decompose()
+    vl = espressopp.VerletList(system, cutoff=rc)
+    wca = espressopp.interaction.VerletListLennardJones(vl)
+    wca.setPotential(0, 0, espressopp.interaction.LennardJones(1.0, 1.0, rc, "auto"))
+    system.addInteraction(wca)
+    fpl = espressopp.FixedPairList(system.storage)
+    fpl.addBonds(bonds.tolist())
+    fene = espressopp.interaction.FixedPairListFENE(system, fpl, espressopp.interaction.FENE(30.0, 0.0, 1.5))
+    system.addInteraction(fene)
+    espressopp.integrator.VelocityVerlet(system).run(0)
+    ref = OracleSim(box, cells, skin)
+    ref.add_particles(np.arange(n), pos)
+    ref.decompose()
+    ria = ref.lj_create(ref.vl_create(rc))
+    ref.lj_set(ria, 0, 0, 1.0, 1.0, rc, "auto")
+    rfpl = ref.fpl_create()
+    ref.fpl_add(rfpl, bonds)
+    rfene = ref.fene_create(rfpl, 30.0, 0.0, 1.5)
+    ref.calc_forces()
+    f = system.storage.getParticleArrays()["f"]
+    tol = 1e-11 if precision == "fp64" else 1e-5
+    assert force_rel_err(f, ref.get("f"), 0.05).max() <= tol
+    etol = 1e-11 if precision == "fp64" else 1e-6
+    assert fene.computeEnergy() == pytest.approx(ref.energy(rfene), rel=etol)
+    assert wca.computeEnergy() == pytest.approx(ref.energy(ria), rel=etol)
+    assert fene.computeVirial() == pytest.approx(ref.virial(rfene)[0], rel=etol)
+
+
+def test_pressure_and_temperature(gpu):
+    pid, pos, vel, box = read_fixture()
+    rc, skin = 2.5, 0.3
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells_for(box, rc, skin), rc, skin, "fp64", ids=pid, vel=vel)
+    assert espressopp.analysis.Temperature(system).compute() == pytest.approx(ref.temperature(), rel=1e-12)
+    assert espressopp.analysis.Pressure(system).compute() == pytest.approx(ref.pressure(), rel=1e-12)
+
+
+def test_empty_system_and_errors(gpu):
+    system = gpu_system((10.0, 10.0, 10.0), (3, 3, 3), 0.3)
+    system.storage.decompose()
+    vl = espressopp.VerletList(system, 2.5)
+    assert vl.totalSize() == 0
+    integ = espressopp.integrator.VelocityVerlet(system)
+    integ.run(5)
+    with pytest.raises(ValueError):
+        integ.dt = 0.0
+    system.storage.addParticlesArray([0, 1], [[1.0, 1.0, 1.0], [2.0, 2.0, 2.0]])
+    system.storage.decompose()
+    system.storage.setParticleArrays("pos", [1], [[float("nan"), 2.0, 2.0]])
+    with pytest.raises(RuntimeError):
+        system.storage.decompose()
+    with pytest.raises(ValueError):
+        gpu_system((10.0, 10.0, 10.0), (5, 5, 5), 0.3)
+        espressopp.VerletList(gpu_system((10.0, 10.0, 10.0), (5, 5, 5), 0.3), 2.5)

@@ -1,0 +1,73 @@
+"""Host-side logic of the drop-in package (no GPU needed)."""
+import math
+
+import numpy as np
+import pytest
+
+import espressopp_amd as espressopp
+from espressopp_amd.tools import decomp, lattice, velocities
+
+from _systems import FIXTURE_2048
+
+
+def test_cell_grid_truncates():
+    # src/tools/decomp.py:322-342 -> Int3D(float) truncation
+    assert tuple(decomp.cellGrid((13.4367695311,) * 3, (1, 1, 1), 2.5, 0.4)) == (4, 4, 4)
+    assert tuple(decomp.cellGrid((170.636,) * 3, (1, 1, 1), 2.5, 0.3)) == (60, 60, 60)
+    with pytest.raises(decomp.Error):
+        decomp.cellGrid((2.0, 10.0, 10.0), (1, 1, 1), 2.5, 0.3)
+
+
+def test_node_grid_single_gpu():
+    assert tuple(decomp.nodeGrid(1, (10, 10, 10), 2.5, 0.3)) == (1, 1, 1)
+
+
+def test_int3d_real3d():
+    assert tuple(espressopp.Int3D(4.9, 3.2, 1.0)) == (4, 3, 1)
+    r = espressopp.Real3D(1.0, 2.0, 2.0)
+    assert r.abs() == 3.0 and r.sqr() == 9.0
+    assert (r - espressopp.Real3D(1.0)) * espressopp.Real3D(0, 1, 1) == 2.0
+
+
+def test_readxyz_fixture():
+    pid, t, x, y, z, vx, vy, vz, Lx, Ly, Lz = espressopp.tools.readxyz(FIXTURE_2048)
+    assert len(pid) == 2048 and Lx == 13.4367695311
+    assert abs(2048 / (Lx * Ly * Lz) - 0.8442) < 1e-4
+
+
+def test_create_cubic_lattice():
+    x, y, z, L, _, _ = lattice.createCubic(1000, 0.8442, perfect=True)
+    assert len(x) == 1000 and abs(L - (1000 / 0.8442) ** (1 / 3)) < 1e-12
+    assert min(x) > 0 and max(x) < L
+
+
+def test_gaussian_velocities_exact_temperature():
+    vx, vy, vz = velocities.gaussian(1.5, 4000, zero_momentum=False)
+    v2 = np.sum(np.array(vx) ** 2 + np.array(vy) ** 2 + np.array(vz) ** 2)
+    assert v2 / (3 * 4000) == pytest.approx(1.5, rel=1e-12)
+
+
+def test_potential_objects():
+    lj = espressopp.interaction.LennardJones(1.0, 1.0, cutoff=2.5)
+    assert lj.shift == pytest.approx(4 * (2.5 ** -12 - 2.5 ** -6))
+    assert lj.computeEnergy(2.6) == 0.0
+    lj2 = espressopp.interaction.LennardJones(1.0, 1.0, cutoff=math.inf)
+    assert lj2.computeEnergy(2 ** (1 / 6)) == pytest.approx(-1.0)
+    fene = espressopp.interaction.FENE(K=30.0, r0=0.0, rMax=1.5)
+    assert fene.computeEnergy(0.97) == pytest.approx(18.279, rel=1e-4)
+    assert tuple(fene.computeForce((0.9, 0.1, 0.1))) == pytest.approx((-42.782, -4.7535, -4.7535), rel=1e-4)
+
+
+def test_folding_helper():
+    bc = espressopp.bc.OrthorhombicBC(None, (10.0, 10.0, 10.0))
+    p, img = bc.getFoldedPosition((12.5, -0.5, 3.0))
+    assert tuple(p) == pytest.approx((2.5, 9.5, 3.0)) and img == [1, -1, 0]
+    assert tuple(bc.getMinimumImageVector((9.5, 0, 0), (0.5, 0, 0))) == pytest.approx((-1.0, 0, 0))
+
+
+def test_run_requires_integer_steps():
+    class Dummy(espressopp.integrator.VelocityVerlet):
+        def __init__(self):
+            self._dt = 0.005
+    with pytest.raises(ValueError):
+        Dummy().run(1.5)

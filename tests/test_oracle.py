@@ -1,0 +1,188 @@
+"""Pins of the FP64 CPU oracle against the reference's own known answers.
+
+Each check uses a value the reference's test suite or the mathematics fixes;
+none restates the oracle's own formula.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from oracle.oracle import OracleSim
+
+from _systems import cells_for, jittered_lattice, read_fixture
+
+
+def test_philox_known_answers():
+    # Random123 kat_vectors for philox4x32-10 (Salmon et al., SC'11)
+    assert [hex(x) for x in oracle.philox4x32_10([0, 0, 0, 0], [0, 0])] == \
+        ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+    assert [hex(x) for x in oracle.philox4x32_10([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)] == \
+        ["0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
+    assert [hex(x) for x in oracle.philox4x32_10([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                                                 [0xA4093822, 0x299F31D0])] == \
+        ["0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
+
+
+def test_langevin_uniforms_moments():
+    u = np.array([oracle.langevin_uniforms(7, c, p) for c in range(40) for p in range(250)]).ravel()
+    assert np.all((u > 0) & (u < 1))
+    assert abs(u.mean() - 0.5) < 0.01
+    assert abs(u.var() - 1.0 / 12.0) < 0.002
+
+
+def test_fene_known_answers():
+    # testsuite/interaction_potentials/unittest/TestFENE.cpp:35-37 and :64-68 (tolerance 0.01 %)
+    for r, e in ((0.97, 18.279), (0.80, 11.296), (1.40, 69.147)):
+        assert oracle.fene_energy(30.0, 0.0, 1.5, r) == pytest.approx(e, rel=1e-4)
+    f = oracle.fene_force(30.0, 0.0, 1.5, [0.9, 0.1, 0.1])
+    assert f == pytest.approx([-42.782, -4.7535, -4.7535], rel=1e-4)
+
+
+def test_lj_known_answers():
+    # testsuite/interaction_potentials/unittest/TestLennardJones.cpp:30-49
+    inf = float("inf")
+    assert oracle.lj_energy(1.0, 1.0, inf, 0.0, 2 ** (1 / 6)) == pytest.approx(-1.0, rel=1e-4)
+    assert abs(oracle.lj_energy(1.0, 1.0, inf, 0.0, 2.5)) < 5.0
+    assert abs(oracle.lj_energy(1.0, 1.0, 2.0, "auto", 1.9999)) < 0.01
+    assert abs(oracle.lj_energy(1.0, 1.0, 2.0, "auto", 2.0)) < 0.01
+    assert oracle.lj_energy(1.0, 1.0, 2.0, "auto", 10.0) == 0.0
+    # testsuite/interaction_potentials/unittest/testLennardJones.py: eps=sigma=2 -> root at 2,
+    # minimum -2 at 2*2^(1/6) with zero force there
+    assert oracle.lj_energy(2.0, 2.0, inf, 0.0, 2.0) == pytest.approx(0.0, abs=1e-12)
+    rmin = 2.0 * 2 ** (1 / 6)
+    assert oracle.lj_energy(2.0, 2.0, inf, 0.0, rmin) == pytest.approx(-2.0, rel=1e-12)
+    assert np.allclose(oracle.lj_force(2.0, 2.0, inf, [rmin, 0.0, 0.0]), 0.0, atol=1e-12)
+    # force is minus the derivative of the energy (central difference)
+    r, h = 1.3, 1e-6
+    dE = (oracle.lj_energy(1.0, 1.0, inf, 0.0, r + h) - oracle.lj_energy(1.0, 1.0, inf, 0.0, r - h)) / (2 * h)
+    assert oracle.lj_force(1.0, 1.0, inf, [r, 0.0, 0.0])[0] == pytest.approx(-dE, rel=1e-7)
+
+
+def _lattice_sim(cut):
+    # testsuite/unittest/testVerletList.py:38-109: 6^3 simple cubic lattice, box 6, skin 0.001
+    N, SIZE = 6, 6.0
+    s = OracleSim((SIZE,) * 3, (3, 3, 3), 0.001)
+    g = (np.arange(N) + 0.5) / N * SIZE
+    pos = np.array([[x, y, z] for x in g for y in g for z in g])
+    s.add_particles(np.arange(N ** 3), pos)
+    s.decompose()
+    return s, s.vl_create(cut)
+
+
+@pytest.mark.parametrize("cut,count", [(0.0, 0), (1.0, 3 * 216), (math.sqrt(2.0), 9 * 216),
+                                       (math.sqrt(3.0), 13 * 216)])
+def test_verlet_list_lattice_counts(cut, count):
+    s, vl = _lattice_sim(cut)
+    assert s.vl_size(vl) == count
+
+
+def _brute_force_pairs(pos, box, cutsq):
+    L = np.asarray(box)
+    n = len(pos)
+    out = set()
+    for i in range(n):
+        d = pos[i] - pos[i + 1:]
+        d -= L * np.round(d / L)
+        r2 = np.sum(d * d, axis=1)
+        for j in np.nonzero(r2 <= cutsq)[0]:
+            out.add((i, i + 1 + int(j)))
+    return out
+
+
+@pytest.mark.parametrize("seed,box,rc,skin", [(1, (9.0, 9.0, 9.0), 2.5, 0.3),
+                                              (2, (8.7, 11.3, 14.9), 2.5, 0.4),
+                                              (3, (6.1, 6.1, 9.3), 1.12, 0.3)])
+def test_verlet_list_brute_force(seed, box, rc, skin):
+    # PTestVerletList.cpp:152-212 pattern: random particles vs O(N^2) pair count
+    rng = np.random.default_rng(seed)
+    n = 500
+    pos = rng.uniform(0, 1, (n, 3)) * np.asarray(box)
+    s = OracleSim(box, cells_for(box, rc, skin), skin)
+    s.add_particles(np.arange(n), pos)
+    s.decompose()
+    vl = s.vl_create(rc)
+    got = set(map(tuple, np.sort(s.vl_pairs(vl), axis=1).tolist()))
+    assert got == _brute_force_pairs(pos, box, (rc + skin) ** 2)
+
+
+def test_exclusions_remove_pairs():
+    rng = np.random.default_rng(5)
+    box = (8.0, 8.0, 8.0)
+    pos = rng.uniform(0, 8, (300, 3))
+    s = OracleSim(box, (2, 2, 2), 0.3)
+    s.add_particles(np.arange(300), pos)
+    s.decompose()
+    vl = s.vl_create(2.5)
+    pairs = s.vl_pairs(vl)
+    excl = pairs[:25]
+    s.vl_exclude(vl, excl[:, ::-1])          # either orientation excludes
+    after = set(map(tuple, np.sort(s.vl_pairs(vl), axis=1).tolist()))
+    assert len(after) == len(pairs) - 25
+    assert not (set(map(tuple, np.sort(excl, axis=1).tolist())) & after)
+
+
+def test_golden_nve_temperature():
+    # testsuite/potential_array/potential_array_test.py:37-163: T = 0.160074388303 +- 1e-8
+    pid, pos, _, box = read_fixture()
+    rc, skin = 2.5, 0.4
+    s = OracleSim(box, [int(b / (rc + skin)) for b in box], skin)
+    s.add_particles(pid, pos, types=pid % 2)
+    s.decompose()
+    ia = s.lj_create(s.vl_create(rc))
+    s.lj_set(ia, 0, 0, 0.1, 1.0, rc, 0.0)
+    s.lj_set(ia, 0, 1, (0.1 * 0.3) ** 0.5, 0.8, rc, 0.0)
+    s.lj_set(ia, 1, 1, 0.9, 0.6, rc, 0.0)
+    s.run(100, dt=0.005)
+    n = len(pid)
+    s.add_particles(np.arange(n, n + 10), pos[:10], types=np.full(10, 6))
+    s.decompose()
+    s.run(100)
+    assert abs(s.temperature() - 0.160074388303) < 1.0e-8
+
+
+def test_free_flight_returns_to_lattice():
+    # testsuite/integrator/PTestVelocityVerlet.cpp:222-256: free particles cross the box and return
+    pos, box = jittered_lattice(125, 0.5, seed=3, jitter=0.0)
+    s = OracleSim(box, (2, 2, 2), 0.3)
+    v = np.tile([1.0, -0.5, 0.25], (125, 1))
+    s.add_particles(np.arange(125), pos, vel=v)
+    s.decompose()
+    L = box[0]
+    dt = L / 400.0
+    s.run(1600, dt=dt)                      # x moves 4 L, y -2 L, z 1 L
+    p = s.get("pos") + s.get("image") * np.asarray(box)
+    assert np.allclose(p, pos + np.array([4 * L, -2 * L, L]), atol=1e-6)
+
+
+def test_harmonic_like_fene_energy_conservation():
+    # two bonded particles oscillating in a FENE well: NVE energy is conserved to O(dt^2)
+    box = (10.0, 10.0, 10.0)
+    s = OracleSim(box, (2, 2, 2), 0.3)
+    s.add_particles([0, 1], [[5.0, 5.0, 5.0], [6.1, 5.0, 5.0]])
+    s.decompose()
+    fpl = s.fpl_create()
+    s.fpl_add(fpl, [[0, 1]])
+    ia = s.fene_create(fpl, 30.0, 0.0, 1.5)
+    s.run(0, dt=0.001)
+    e0 = s.energy(ia)
+    for _ in range(20):
+        s.run(50)
+        v = s.get("v")
+        etot = s.energy(ia) + 0.5 * np.sum(v * v)
+        assert abs(etot - e0) < 1e-4 * abs(e0)
+
+
+def test_thread_count_does_not_change_results():
+    pid, pos, _, box = read_fixture()
+    out = []
+    for threads in (1, 4):
+        s = OracleSim(box, (4, 4, 4), 0.3, threads=threads)
+        s.add_particles(pid, pos)
+        s.decompose()
+        ia = s.lj_create(s.vl_create(2.5))
+        s.lj_set(ia, 0, 0, 1.0, 1.0, 2.5, "auto")
+        s.run(20, dt=0.005)
+        out.append(s.get("pos"))
+    assert np.array_equal(out[0], out[1])
