@@ -1,8 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark of the short-range MD step on one MI355X.
 
-Workload: Lennard-Jones fluid, N = 4,194,304 (128^3 jittered simple-cubic
-lattice), density 0.8442, eps = sigma = 1, rc = 2.5 (shift auto), skin 0.3,
+Workload: Lennard-Jones fluid, N = 4,194,304 (first N sites of a 162^3
+simple-cubic lattice, jittered U(-0.05, 0.05)), density 0.8442, eps = sigma = 1, rc = 2.5 (shift auto), skin 0.3,
 dt = 0.005.  Warm-up: W Langevin steps at T = 0.72, gamma = 1 (untimed), then
 K timed NVE steps.  A step is one velocity-Verlet step: integrate1, skin check,
 rebuild when triggered, pair forces, integrate2.
@@ -117,7 +117,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--n", type=int, default=128 ** 3)
+    ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")

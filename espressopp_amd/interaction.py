@@ -177,7 +177,9 @@ class Interaction:
         return self._h.out_dbl("espp_ia_energy", self._id)
 
     def computeVirial(self):
-        return self._h.out_dbl("espp_ia_virial", self._id, None)
+        w = ctypes.c_double(0.0)
+        self._h.call("espp_ia_virial", self._id, ctypes.byref(w), None)
+        return w.value
 
     def computeVirialTensor(self):
         """xx, yy, zz, xy, xz, yz (src/interaction/VerletListInteractionTemplate.hpp:306-331)."""
