@@ -35,13 +35,23 @@ struct FENEParam {
   double K, r0, rMax, rMaxSqr, cutoffSqr, shift;
 };
 
+// One contiguous range of cells of a tile's staged region: the particles of
+// cells [cell_a, cell_b] are slots [slot_start, slot_start + count) and sit at
+// LDS offset lds_off of the tile's staging buffer.
+struct TileRun {
+  int slot_start, count, lds_off, cell_a, cell_b, pad0, pad1, pad2;
+};
+constexpr int kMaxRuns = 64;      // runs per tile
+constexpr int kMaxPieces = 128;   // row segments examined while planning a tile
+
 // Per-call control block kept on the device.
 struct Ctl {
   unsigned long long maxsq_bits;   // max |dx|^2 of the last integrate1 (non-negative double bits)
   int overflow;                    // neighbour list capacity exceeded
   int maxcount;                    // largest neighbour count of the last build
   int nanpos;                      // a NaN position was met while folding
-  int pad;
+  int plan_fail;                   // a tile exceeded the run or LDS limits
+  int max_region;                  // largest staged region (particles) of the last plan
 };
 
 }  // namespace espp

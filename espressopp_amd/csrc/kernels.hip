@@ -204,6 +204,8 @@ __global__ void k_reset_ctl(Ctl* ctl) {
     ctl->overflow = 0;
     ctl->maxcount = 0;
     ctl->nanpos = 0;
+    ctl->plan_fail = 0;
+    ctl->max_region = 0;
   }
 }
 
@@ -486,6 +488,387 @@ __global__ __launch_bounds__(256) void k_lj_fp64(
     for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
     block_reduce_store<kNPart>(acc, partial);
   }
+}
+
+// ------------------------------------------------------------------ tiles
+//
+// A tile is T consecutive particles (T = blockDim.x of the tiled kernels, one
+// particle per thread).  Its staged region is the union of the 27 neighbour
+// cells of every cell the tile touches, kept as runs of consecutive cell
+// indices; the particles of a run are consecutive slots, so staging is a set of
+// coalesced copies into LDS.  Neighbour entries of the tiled list are 16-bit
+// offsets into that LDS buffer, two per 32-bit word.
+
+__device__ __forceinline__ int wrap_dim(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// One thread per tile.  Runs are sorted by cell index and never overlap.
+__global__ void k_tile_plan(int n, int T, int ntiles, const int* __restrict__ cell_of_slot,
+                            const int* __restrict__ cstart, Geom g, int budget,
+                            TileRun* __restrict__ runs, int* __restrict__ nruns, Ctl* ctl) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int s0 = t * T, s1 = min(s0 + T, n) - 1;
+  const int c0 = cell_of_slot[s0], c1 = cell_of_slot[s1];
+  const int r0 = c0 / nx, r1 = c1 / nx;
+  int pa[kMaxPieces], pb[kMaxPieces];
+  int np = 0;
+  bool bad = (r1 - r0 + 1) * 18 > kMaxPieces;
+  for (int r = r0; r <= r1 && !bad; ++r) {
+    const int y = r % ny, z = r / ny;
+    const int xa = (r == r0) ? c0 % nx : 0, xb = (r == r1) ? c1 % nx : nx - 1;
+    const int lo = xa - 1, hi = xb + 1;
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int zz = wrap_dim(z + dz, nz);
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = wrap_dim(y + dy, ny);
+        const int rb = (yy + ny * zz) * nx;
+        if (hi - lo + 1 >= nx) {
+          pa[np] = rb; pb[np] = rb + nx - 1; ++np;
+        } else if (lo < 0) {
+          pa[np] = rb + lo + nx; pb[np] = rb + nx - 1; ++np;
+          pa[np] = rb; pb[np] = rb + hi; ++np;
+        } else if (hi >= nx) {
+          pa[np] = rb + lo; pb[np] = rb + nx - 1; ++np;
+          pa[np] = rb; pb[np] = rb + hi - nx; ++np;
+        } else {
+          pa[np] = rb + lo; pb[np] = rb + hi; ++np;
+        }
+      }
+    }
+  }
+  int nr = 0, total = 0;
+  if (!bad) {
+    // insertion sort of the pieces by first cell, then merge overlapping or adjacent ranges
+    for (int a = 1; a < np; ++a) {
+      const int ka = pa[a], kb = pb[a];
+      int m = a - 1;
+      while (m >= 0 && pa[m] > ka) { pa[m + 1] = pa[m]; pb[m + 1] = pb[m]; --m; }
+      pa[m + 1] = ka; pb[m + 1] = kb;
+    }
+    int ca = pa[0], cb = pb[0];
+    for (int a = 1; a <= np && !bad; ++a) {
+      if (a < np && pa[a] <= cb + 1) { cb = max(cb, pb[a]); continue; }
+      if (nr >= kMaxRuns) { bad = true; break; }
+      TileRun R;
+      R.cell_a = ca; R.cell_b = cb;
+      R.slot_start = cstart[ca];
+      R.count = cstart[cb + 1] - R.slot_start;
+      R.lds_off = total;
+      R.pad0 = R.pad1 = R.pad2 = 0;
+      runs[(size_t)t * kMaxRuns + nr] = R;
+      total += R.count;
+      ++nr;
+      if (a < np) { ca = pa[a]; cb = pb[a]; }
+    }
+  }
+  if (bad || total > budget || total > 65535) {
+    ctl->plan_fail = 1;
+    nr = 0;
+  }
+  nruns[t] = nr;
+  atomicMax(&ctl->max_region, total);
+}
+
+// run holding cell c (runs sorted by cell_a, c guaranteed inside one of them)
+__device__ __forceinline__ int find_run_by_cell(const TileRun* sr, int nr, int c) {
+  int lo = 0, hi = nr - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sr[mid].cell_a <= c) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// slot of LDS entry e (runs sorted by lds_off)
+__device__ __forceinline__ int slot_of_lds(const TileRun* sr, int nr, int e) {
+  int lo = 0, hi = nr - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sr[mid].lds_off <= e) lo = mid; else hi = mid - 1;
+  }
+  return sr[lo].slot_start + (e - sr[lo].lds_off);
+}
+
+template <typename V>
+__device__ __forceinline__ int stage_tile(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
+                                          const V* __restrict__ src, V* lds, TileRun* sr) {
+  const int t = blockIdx.x;
+  const int nr = nruns[t];
+  if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)t * kMaxRuns + threadIdx.x];
+  __syncthreads();
+  for (int r = 0; r < nr; ++r) {
+    const int cnt = sr[r].count, off = sr[r].lds_off, ss = sr[r].slot_start;
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) lds[off + q] = src[ss + q];
+  }
+  __syncthreads();
+  return nr;
+}
+
+// Tiled Verlet-list build.  Candidates of the 27 neighbour cells are read from
+// the staged fixed-point positions; pairs whose FP32 r^2 is within `win` of
+// (rc+skin)^2 are settled by the reference's FP64 test (same orientation rule
+// as k_nbr_build), all others are decided by the FP32 value.
+template <bool HAS_EXCL>
+__global__ __launch_bounds__(256) void k_nbr_build_tiled(
+    int n, const uint4* __restrict__ posq, const double4* __restrict__ pos, const int* __restrict__ id,
+    const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g, double cutsq, float cut_lo,
+    float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint32_t* __restrict__ nbr,
+    int* __restrict__ cnt, int capw, Ctl* ctl) {
+#pragma clang fp contract(off)
+  extern __shared__ uint4 lds[];
+  __shared__ TileRun sr[kMaxRuns];
+  const int nr = stage_tile<uint4>(runs, nruns, posq, lds, sr);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = 0;
+  if (i < n && nr > 0) {
+    const uint4 qi = posq[i];
+    const int c = cell_of_slot[i];
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    const int myid = HAS_EXCL ? id[i] : 0;
+    uint32_t* out = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
+    uint32_t pending = 0;
+    const int cap = 2 * capw;
+    for (int oz = -1; oz <= 1; ++oz) {
+      int qz = cz + oz, sz = 0;
+      if (qz < 0) { qz += nz; sz = -1; } else if (qz >= nz) { qz -= nz; sz = 1; }
+      for (int oy = -1; oy <= 1; ++oy) {
+        int qy = cy + oy, sy = 0;
+        if (qy < 0) { qy += ny; sy = -1; } else if (qy >= ny) { qy -= ny; sy = 1; }
+        for (int ox = -1; ox <= 1; ++ox) {
+          int qx = cx + ox, sx = 0;
+          if (qx < 0) { qx += nx; sx = -1; } else if (qx >= nx) { qx -= nx; sx = 1; }
+          const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
+          const bool unshifted = (sx | sy | sz) == 0;
+          const int c2 = qx + nx * (qy + ny * qz);
+          const TileRun R = sr[find_run_by_cell(sr, nr, c2)];
+          const int jb = cstart[c2], je = cstart[c2 + 1];
+          const int base = R.lds_off + (jb - R.slot_start);
+          for (int j = jb; j < je; ++j) {
+            if (j == i && unshifted) continue;
+            const int e = base + (j - jb);
+            const uint4 qj = lds[e];
+            const float dx = (float)(int)(qi.x - qj.x) * g.qs[0];
+            const float dy = (float)(int)(qi.y - qj.y) * g.qs[1];
+            const float dz = (float)(int)(qi.z - qj.z) * g.qs[2];
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > cut_hi) continue;
+            bool in = r2 < cut_lo;
+            if (!in) {
+              const double4 pi = pos[i], pj = pos[j];
+              const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
+              double ex, ey, ez;
+              if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
+              else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
+              in = (ex * ex + ey * ey + ez * ez) <= cutsq;
+            }
+            if (!in) continue;
+            if (HAS_EXCL && is_excluded(myid, id[j], ex_start, ex_ids)) continue;
+            if (k & 1) {
+              if (k < cap) out[(size_t)(k >> 1) * 64] = pending | ((uint32_t)e << 16);
+            } else {
+              pending = (uint32_t)e;
+            }
+            ++k;
+          }
+        }
+      }
+    }
+    if ((k & 1) && k < cap) out[(size_t)(k >> 1) * 64] = pending;
+    cnt[i] = k;
+    if (k > cap) ctl->overflow = 1;
+  } else if (i < n) {
+    cnt[i] = 0;
+  }
+  int m = k;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(&ctl->maxcount, m);
+}
+
+// Mixed-precision LJ over the tiled list: the same arithmetic as k_lj_mixed
+// with the neighbour positions read from LDS.
+template <bool ONETYPE, bool ENERGY, bool ACCUM, int kChunk>
+__global__ __launch_bounds__(256) void k_lj_tiled_mixed(
+    int n, const uint4* __restrict__ posq, const double4* __restrict__ pos,
+    const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int capw, Geom g,
+    const TileRun* __restrict__ runs, const int* __restrict__ nruns, const LJParam* __restrict__ prm,
+    int ntypes, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    double* __restrict__ partial) {
+  extern __shared__ uint4 lds[];
+  __shared__ TileRun sr[kMaxRuns];
+  __shared__ float4 tab[kMaxTypes * kMaxTypes];
+  __shared__ float2 etab[kMaxTypes * kMaxTypes];
+  if (!ONETYPE) {
+    for (int t = threadIdx.x; t < ntypes * ntypes; t += blockDim.x) {
+      const LJParam p = prm[t];
+      tab[t] = make_float4(p.ff1, p.ff2, p.set ? p.rc2 : -1.0f, p.shift);
+      etab[t] = make_float2((float)p.sig2, (float)p.eps4);
+    }
+  }
+  const int nr = stage_tile<uint4>(runs, nruns, posq, lds, sr);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kNPart];
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
+  if (i < n) {
+    const uint4 qi = posq[i];
+    const uint32_t* np = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
+    const int c = nr > 0 ? min(cnt[i], 2 * capw) : 0;
+    float ff1 = prm[0].ff1, ff2 = prm[0].ff2, rc2 = prm[0].rc2, shift = prm[0].shift;
+    float sig2 = (float)prm[0].sig2, eps4 = (float)prm[0].eps4;
+    double drc2 = prm[0].drc2;
+    const int rowi = (int)qi.w * ntypes;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    double Ax = 0.0, Ay = 0.0, Az = 0.0;
+    float e32 = 0.f, w32 = 0.f;
+    float t32[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint32_t word = 0;
+    for (int k = 0; k < c; ++k) {
+      if ((k & 1) == 0) word = np[(size_t)(k >> 1) * 64];
+      const int e = (k & 1) ? (int)(word >> 16) : (int)(word & 0xFFFFu);
+      const uint4 qj = lds[e];
+      const float dx = (float)(int)(qi.x - qj.x) * g.qs[0];
+      const float dy = (float)(int)(qi.y - qj.y) * g.qs[1];
+      const float dz = (float)(int)(qi.z - qj.z) * g.qs[2];
+      if (!ONETYPE) {
+        const float4 t = tab[rowi + (int)qj.w];
+        ff1 = t.x; ff2 = t.y; rc2 = t.z; shift = t.w;
+        if (ENERGY) { const float2 et = etab[rowi + (int)qj.w]; sig2 = et.x; eps4 = et.y; }
+      }
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      bool in = r2 <= rc2;
+      if (__builtin_fabsf(r2 - rc2) <= 1e-5f * rc2 && rc2 > 0.f) {
+        if (!ONETYPE) drc2 = prm[rowi + (int)qj.w].drc2;
+        const int j = slot_of_lds(sr, nr, e);
+        const double4 pi = pos[i], pj = pos[j];
+        double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+        ex = __builtin_fma(-g.L[0], __builtin_rint(ex * g.invL[0]), ex);
+        ey = __builtin_fma(-g.L[1], __builtin_rint(ey * g.invL[1]), ey);
+        ez = __builtin_fma(-g.L[2], __builtin_rint(ez * g.invL[2]), ez);
+        in = (ex * ex + ey * ey + ez * ez) <= drc2;
+      }
+      const float inv = rcp_nr(r2);
+      const float r6 = inv * inv * inv;
+      float ff = r6 * (ff1 * r6 - ff2) * inv;
+      ff = in ? ff : 0.0f;
+      ax = __builtin_fmaf(dx, ff, ax);
+      ay = __builtin_fmaf(dy, ff, ay);
+      az = __builtin_fmaf(dz, ff, az);
+      if (ENERGY) {
+        const float f2 = sig2 * inv;
+        const float f6 = f2 * f2 * f2;
+        const float en = eps4 * (f6 * f6 - f6) - shift;
+        e32 += in ? en : 0.0f;
+        w32 += r2 * ff;
+        t32[0] += dx * dx * ff; t32[1] += dy * dy * ff; t32[2] += dz * dz * ff;
+        t32[3] += dx * dy * ff; t32[4] += dx * dz * ff; t32[5] += dy * dz * ff;
+      }
+      if ((k % kChunk) == kChunk - 1) {
+        Ax += (double)ax; Ay += (double)ay; Az += (double)az;
+        ax = ay = az = 0.f;
+        if (ENERGY) {
+          acc[0] += (double)e32; acc[1] += (double)w32;
+#pragma unroll
+          for (int q = 0; q < 6; ++q) { acc[2 + q] += (double)t32[q]; t32[q] = 0.f; }
+          e32 = w32 = 0.f;
+        }
+      }
+    }
+    Ax += (double)ax; Ay += (double)ay; Az += (double)az;
+    if (ENERGY) {
+      acc[0] += (double)e32; acc[1] += (double)w32;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc[2 + q] += (double)t32[q];
+    }
+    if (!ENERGY) {
+      if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
+      else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
+    }
+  }
+  if (ENERGY) {
+#pragma unroll
+    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
+    block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
+// All-FP64 LJ over the tiled list; LDS holds FP64 positions with the type in w.
+template <bool ONETYPE, bool ENERGY, bool ACCUM>
+__global__ __launch_bounds__(256) void k_lj_tiled_fp64(
+    int n, const double4* __restrict__ pos, const uint32_t* __restrict__ nbr,
+    const int* __restrict__ cnt, int capw, Geom g, const TileRun* __restrict__ runs,
+    const int* __restrict__ nruns, const LJParam* __restrict__ prm, int ntypes,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    double* __restrict__ partial) {
+  extern __shared__ double4 ldsd[];
+  __shared__ TileRun sr[kMaxRuns];
+  const int nr = stage_tile<double4>(runs, nruns, pos, ldsd, sr);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kNPart];
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
+  if (i < n) {
+    const double4 pi = pos[i];
+    const uint32_t* np = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
+    const int c = nr > 0 ? min(cnt[i], 2 * capw) : 0;
+    const int rowi = (int)pi.w * ntypes;
+    double ff1 = prm[0].dff1, ff2 = prm[0].dff2, rc2 = prm[0].drc2, shift = prm[0].dshift;
+    double sig2 = prm[0].sig2, eps4 = prm[0].eps4;
+    int set = prm[0].set;
+    double Ax = 0.0, Ay = 0.0, Az = 0.0;
+    uint32_t word = 0;
+    for (int k = 0; k < c; ++k) {
+      if ((k & 1) == 0) word = np[(size_t)(k >> 1) * 64];
+      const int e = (k & 1) ? (int)(word >> 16) : (int)(word & 0xFFFFu);
+      const double4 pj = ldsd[e];
+      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      dx = __builtin_fma(-g.L[0], __builtin_rint(dx * g.invL[0]), dx);
+      dy = __builtin_fma(-g.L[1], __builtin_rint(dy * g.invL[1]), dy);
+      dz = __builtin_fma(-g.L[2], __builtin_rint(dz * g.invL[2]), dz);
+      if (!ONETYPE) {
+        const LJParam& p = prm[rowi + (int)pj.w];
+        ff1 = p.dff1; ff2 = p.dff2; rc2 = p.drc2; shift = p.dshift; sig2 = p.sig2; eps4 = p.eps4;
+        set = p.set;
+      }
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const bool in = set && r2 <= rc2;
+      const double inv = rcp_nr(r2);
+      const double r6 = inv * inv * inv;
+      double ff = r6 * (ff1 * r6 - ff2) * inv;
+      ff = in ? ff : 0.0;
+      Ax = __builtin_fma(dx, ff, Ax);
+      Ay = __builtin_fma(dy, ff, Ay);
+      Az = __builtin_fma(dz, ff, Az);
+      if (ENERGY) {
+        const double f2 = sig2 * inv;
+        const double f6 = f2 * f2 * f2;
+        acc[0] += in ? eps4 * (f6 * f6 - f6) - shift : 0.0;
+        acc[1] += r2 * ff;
+        acc[2] += dx * dx * ff; acc[3] += dy * dy * ff; acc[4] += dz * dz * ff;
+        acc[5] += dx * dy * ff; acc[6] += dx * dz * ff; acc[7] += dy * dz * ff;
+      }
+    }
+    if (!ENERGY) {
+      if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
+      else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
+    }
+  }
+  if (ENERGY) {
+#pragma unroll
+    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
+    block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
+// fixed-point mirror of the FP64 positions (refresh before a tiled build of an FP64 system)
+__global__ void k_posq(int n, const double4* __restrict__ pos, uint4* __restrict__ posq, Geom g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 p = pos[i];
+  posq[i] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]), (uint32_t)(int)p.w);
 }
 
 // ------------------------------------------------------------------ FENE bonds
@@ -789,6 +1172,55 @@ void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum,
 #undef LJM
 #undef LJD
 }
+
+
+void launch_tile_plan(hipStream_t s, int n, int T, int ntiles, const int* cell_of_slot, const int* cstart,
+                      const Geom& g, int budget, TileRun* runs, int* nruns, Ctl* ctl) {
+  if (ntiles > 0)
+    hipLaunchKernelGGL(k_tile_plan, dim3(nblocks(ntiles, 64)), dim3(64), 0, s, n, T, ntiles, cell_of_slot, cstart, g,
+                       budget, runs, nruns, ctl);
+}
+
+void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Geom& g) {
+  if (n > 0) hipLaunchKernelGGL(k_posq, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, posq, g);
+}
+
+void launch_nbr_build_tiled(hipStream_t s, int n, int T, int lds_bytes, const uint4* posq, const double4* pos,
+                            const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
+                            float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
+                            const int* ex_ids, uint32_t* nbr, int* cnt, int capw, Ctl* ctl) {
+  if (n <= 0) return;
+  dim3 grid(nblocks(n, T)), block(T);
+  if (ex_start)
+    hipLaunchKernelGGL(k_nbr_build_tiled<true>, grid, block, lds_bytes, s, n, posq, pos, id, cell_of_slot, cstart, g,
+                       cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capw, ctl);
+  else
+    hipLaunchKernelGGL(k_nbr_build_tiled<false>, grid, block, lds_bytes, s, n, posq, pos, id, cell_of_slot, cstart, g,
+                       cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capw, ctl);
+}
+
+void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
+                     const uint4* posq, const double4* pos, const uint32_t* nbr, const int* cnt, int capw,
+                     const Geom& g, const TileRun* runs, const int* nruns, const LJParam* prm, int ntypes,
+                     double* fx, double* fy, double* fz, double* partial) {
+  if (n <= 0) return;
+  dim3 grid(nblocks(n, T)), block(T);
+#define LJM(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, 8>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial)
+#define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64<OT, EN, AC>), grid, block, lds_bytes, s, n, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial)
+  if (mixed) {
+    if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }
+    else if (onetype) { if (accum) LJM(true, false, true); else LJM(true, false, false); }
+    else { if (accum) LJM(false, false, true); else LJM(false, false, false); }
+  } else {
+    if (energy) { if (onetype) LJD(true, true, false); else LJD(false, true, false); }
+    else if (onetype) { if (accum) LJD(true, false, true); else LJD(true, false, false); }
+    else { if (accum) LJD(false, false, true); else LJD(false, false, false); }
+  }
+#undef LJM
+#undef LJD
+}
+
+int lj_tiled_blocks(int n, int T) { return nblocks(n, T); }
 
 void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, const double4* pos,
                  const uint4* posq, const int* id, const int* slot_of_id, int nid, const int* bstart,

@@ -85,7 +85,8 @@ struct DBuf {
 
 struct VerletListState {
   double cut = 0.0;
-  int cap = 0;
+  int cap = 0;          // entries per particle row (untiled) or 2 x words per row (tiled)
+  bool tiled = false;   // entries are 16-bit offsets into the tile's staged region
   DBuf<int> nbr, cnt;
   int64_t builds = 0;
   int maxcount = 0;
@@ -134,6 +135,14 @@ struct espp_sim {
   DBuf<int> cell_of, rank, order, cell_of_slot, cell_count, cstart;
   DBuf<int> slot_of_id;
   int nid = 0;
+
+  // tile plan of the current slot order (tiled list build and force kernels)
+  int lds_cap = 65536;          // dynamic+static LDS bytes one workgroup may use
+  int T = 0;                    // particles per tile; 0 when no plan fits
+  bool plan_valid = false;
+  int max_region = 0;           // largest staged region (particles)
+  DBuf<TileRun> runs;
+  DBuf<int> nruns;
   std::vector<int32_t> host_ids;  // ids present (unsorted)
   bool sorted = false;
 
@@ -236,7 +245,7 @@ struct espp_sim {
     pos2.resize(m); vel2.resize(m); fx2.resize(m); fy2.resize(m); fz2.resize(m);
     id2.resize(m); img2.resize(m);
     cell_of.resize(m); rank.resize(m); order.resize(m); cell_of_slot.grow_keep(m, keep, stream);
-    if (mixed()) posq.grow_keep(m, keep, stream);
+    posq.grow_keep(m, keep, stream);
   }
 
   int64_t ncell() const { return (int64_t)g.n[0] * g.n[1] * g.n[2]; }
@@ -266,12 +275,12 @@ struct espp_sim {
       launch_scatter(stream, ni, cell_of.p, rank.p, cstart.p, order.p);
       launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, id.p);
       launch_permute(stream, ni, order.p, cell_of.p, pos.p, vel.p, fx.p, fy.p, fz.p, id.p, img.p, pos2.p,
-                     vel2.p, fx2.p, fy2.p, fz2.p, id2.p, img2.p, cell_of_slot.p, slot_of_id.p,
-                     mixed() ? posq.p : nullptr, g);
+                     vel2.p, fx2.p, fy2.p, fz2.p, id2.p, img2.p, cell_of_slot.p, slot_of_id.p, posq.p, g);
       pos.swap(pos2); vel.swap(vel2); fx.swap(fx2); fy.swap(fy2); fz.swap(fz2);
       id.swap(id2); img.swap(img2);
     }
     prof_end(4, pe);
+    plan_valid = false;
     for (auto* v : vls) v->stale = true;
     if (do_fold) {
       read_ctl();
@@ -282,6 +291,35 @@ struct espp_sim {
 
   void ensure_sorted() {
     if (!sorted) bin(false);
+  }
+
+  // Tile plan for the current slot order: the largest tile size whose staged
+  // regions fit the LDS budget; T = 0 selects the untiled kernels.
+  void ensure_plan() {
+    if (plan_valid) return;
+    T = 0;
+    max_region = 0;
+    plan_valid = true;
+    if (n == 0) return;
+    const int elem = mixed() ? 16 : 32;
+    const int budget = (lds_cap - 12288) / elem;
+    const int sizes_mixed[3] = {256, 128, 64}, sizes_fp64[2] = {128, 64};
+    const int* sizes = mixed() ? sizes_mixed : sizes_fp64;
+    const int nsz = mixed() ? 3 : 2;
+    for (int q = 0; q < nsz; ++q) {
+      const int tt = sizes[q];
+      const int ntiles = (int)((n + tt - 1) / tt);
+      runs.resize((size_t)ntiles * kMaxRuns);
+      nruns.resize((size_t)ntiles);
+      launch_reset_ctl(stream, ctl.p);
+      launch_tile_plan(stream, (int)n, tt, ntiles, cell_of_slot.p, cstart.p, g, budget, runs.p, nruns.p, ctl.p);
+      read_ctl();
+      if (!hctl->plan_fail) {
+        T = tt;
+        max_region = hctl->max_region;
+        return;
+      }
+    }
   }
 
   // neighbour lists hold slot indices: rebuild every list the last binning invalidated
@@ -302,6 +340,7 @@ struct espp_sim {
                                " is smaller than cutoff+skin " + std::to_string(cutVerlet));
     }
     if (v.has_excl && v.ex_nid != nid) upload_exclusions(v);
+    ensure_plan();
     const int ngroups = (int)((n + 63) / 64);
     if (v.cap == 0) {
       double vol = g.L[0] * g.L[1] * g.L[2];
@@ -311,13 +350,27 @@ struct espp_sim {
       v.cap = (cap + 7) & ~7;
     }
     v.cnt.resize((size_t)std::max<int64_t>(n, 1));
+    v.tiled = T > 0;
+    if (v.tiled && !mixed()) launch_posq(stream, (int)n, pos.p, posq.p, g);
+    // FP32 prefilter window: FP32 rounding plus fixed-point quantisation of both positions
+    double qmax = std::max(g.L[0], std::max(g.L[1], g.L[2])) / 4294967296.0;
+    double win = 1e-5 + 8.0 * qmax / std::max(cutVerlet, 1e-30);
+    const float cut_lo = (float)(cutsq * (1.0 - win)), cut_hi = (float)(cutsq * (1.0 + win));
     for (int attempt = 0; attempt < 4; ++attempt) {
-      v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
       launch_reset_ctl(stream, ctl.p);
       hipEvent_t pe = prof_begin();
-      launch_nbr_build(stream, (int)n, pos.p, id.p, cell_of_slot.p, cstart.p, g, cutsq,
-                       v.has_excl ? v.ex_start.p : nullptr, v.has_excl ? v.ex_ids.p : nullptr, v.nbr.p,
-                       v.cnt.p, v.cap, ctl.p);
+      if (v.tiled) {
+        const int capw = v.cap / 2;
+        v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)capw * 64);
+        launch_nbr_build_tiled(stream, (int)n, T, max_region * 16, posq.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
+                               cutsq, cut_lo, cut_hi, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
+                               v.has_excl ? v.ex_ids.p : nullptr, (uint32_t*)v.nbr.p, v.cnt.p, capw, ctl.p);
+      } else {
+        v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
+        launch_nbr_build(stream, (int)n, pos.p, id.p, cell_of_slot.p, cstart.p, g, cutsq,
+                         v.has_excl ? v.ex_start.p : nullptr, v.has_excl ? v.ex_ids.p : nullptr, v.nbr.p,
+                         v.cnt.p, v.cap, ctl.p);
+      }
       prof_end(1, pe);
       read_ctl();
       v.maxcount = hctl->maxcount;
@@ -403,8 +456,13 @@ struct espp_sim {
       upload_lj(a);
       bool onetype = a.dtab_nt == 1;
       hipEvent_t pe = energy ? nullptr : prof_begin();
-      launch_lj(stream, mixed(), onetype, energy, accum, (int)n, posq.p, pos.p, v.nbr.p, v.cnt.p, v.cap, g,
-                a.dtab.p, a.dtab_nt, fx.p, fy.p, fz.p, partial_out);
+      if (v.tiled)
+        launch_lj_tiled(stream, mixed(), onetype, energy, accum, (int)n, T, max_region * (mixed() ? 16 : 32), posq.p,
+                        pos.p, (const uint32_t*)v.nbr.p, v.cnt.p, v.cap / 2, g, runs.p, nruns.p, a.dtab.p, a.dtab_nt,
+                        fx.p, fy.p, fz.p, partial_out);
+      else
+        launch_lj(stream, mixed(), onetype, energy, accum, (int)n, posq.p, pos.p, v.nbr.p, v.cnt.p, v.cap, g,
+                  a.dtab.p, a.dtab_nt, fx.p, fy.p, fz.p, partial_out);
       prof_end(0, pe);
     } else {
       FixedPairListState& f = *fpls[(size_t)a.list];
@@ -447,7 +505,8 @@ struct espp_sim {
 
   void interaction_sums(InteractionState& a, double out8[8]) {
     ensure_lists();
-    int nblk = std::max(lj_blocks((int)n), 1);
+    bool tiled = a.kind == 0 && vls[(size_t)a.list]->tiled;
+    int nblk = std::max(tiled ? lj_tiled_blocks((int)n, T) : lj_blocks((int)n), 1);
     partial.resize((size_t)nblk * 8);
     if (n == 0) { for (int q = 0; q < 8; ++q) out8[q] = 0.0; return; }
     run_interaction(a, true, false, partial.p);
@@ -618,6 +677,9 @@ int espp_create(const double box[3], const int32_t cell_grid[3], double skin, ui
       s->seed = seed;
       s->precision = precision;
       hip_check(hipGetDevice(&s->device), "hipGetDevice");
+      int lds = 0;
+      if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device) == hipSuccess && lds > 0)
+        s->lds_cap = std::min(lds, 65536);
       hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
       s->ctl.resize(1);
       hip_check(hipHostMalloc((void**)&s->hctl, sizeof(Ctl), hipHostMallocDefault), "hipHostMalloc");
@@ -894,12 +956,30 @@ int espp_vl_pairs(espp_sim* s, int32_t vl, int32_t* out, int64_t cap, int64_t* t
     s->sync();
     size_t n = (size_t)s->n;
     std::vector<int> c = download(v.cnt, n), ids = download(s->id, n);
-    std::vector<int> nb = download(v.nbr, (size_t)((n + 63) / 64) * (size_t)v.cap * 64);
+    const size_t rowlen = v.tiled ? (size_t)v.cap / 2 : (size_t)v.cap;
+    std::vector<int> nb = download(v.nbr, (size_t)((n + 63) / 64) * rowlen * 64);
+    std::vector<TileRun> runs;
+    std::vector<int> nruns;
+    if (v.tiled) {
+      size_t ntiles = (n + (size_t)s->T - 1) / (size_t)s->T;
+      runs = download(s->runs, ntiles * kMaxRuns);
+      nruns = download(s->nruns, ntiles);
+    }
+    auto entry_slot = [&](size_t i, int q) -> int {
+      const int* row = nb.data() + (i / 64) * rowlen * 64 + (i % 64);
+      if (!v.tiled) return row[(size_t)q * 64];
+      uint32_t w = (uint32_t)row[(size_t)(q / 2) * 64];
+      int e = (q & 1) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
+      size_t t = i / (size_t)s->T;
+      const TileRun* sr = runs.data() + t * kMaxRuns;
+      int r = 0;
+      while (r + 1 < nruns[t] && sr[r + 1].lds_off <= e) ++r;
+      return sr[r].slot_start + (e - sr[r].lds_off);
+    };
     int64_t k = 0;
     for (size_t i = 0; i < n; ++i) {
-      const int* row = nb.data() + (i / 64) * (size_t)v.cap * 64 + (i % 64);
       for (int q = 0; q < c[i]; ++q) {
-        int j = row[(size_t)q * 64];
+        int j = entry_slot(i, q);
         if (ids[i] < ids[(size_t)j]) {
           if (k < cap) { out[2 * k] = ids[i]; out[2 * k + 1] = ids[(size_t)j]; }
           ++k;
