@@ -167,7 +167,7 @@ def main():
         f32 = system.storage.getParticleArrays()["f"]
         f64 = sys64.storage.getParticleArrays()["f"]
         nref = np.linalg.norm(f64, axis=1)
-        floor = 0.05 * math.sqrt(np.mean(nref ** 2))
+        floor = 0.1 * math.sqrt(np.mean(nref ** 2))
         rel = np.linalg.norm(f32 - f64, axis=1) / np.maximum(nref, floor)
         e32, e64 = lj.computeEnergy(), lj64.computeEnergy()
         w32, w64 = lj.computeVirial(), lj64.computeVirial()
@@ -192,6 +192,10 @@ def main():
     achieved = b_force * n / avg_s / 1e9
     stats = {k: kernel_stats(h, k) for k in ("nbr_build", "integrate1", "integrate2", "binning")}
     resorts = integ.getNumResorts()
+    import ctypes
+    tile, region = ctypes.c_int32(0), ctypes.c_int32(0)
+    espressopp._lib.check(h.lib.espp_tile_stats(h.h, ctypes.byref(tile), ctypes.byref(region)))
+    log("tile plan: %d particles per tile, largest staged region %d particles" % (tile.value, region.value))
     log("mixed: %.3f ms/step, %.4g particle-steps/s, pair kernel %.3f ms avg (%d launches), nbar %.2f, "
         "cap %d, resorts %d" % (dt_run / args.steps * 1e3, value, avg_s * 1e3, launches, nbar, cap, resorts))
     for k, (c, ms) in stats.items():

@@ -20,6 +20,9 @@ struct Geom {
   double ics[3];                   // 1 / cellSize, cellSize = L / n (src/storage/CellGrid.cpp:55-56)
   double qinv[3];                  // 2^32 / L: FP64 position -> 32-bit fixed point
   float qs[3];                     // L / 2^32: fixed-point difference -> FP32 length
+  double qsd[3];                   // L / 2^32 in FP64
+  double qs2;                      // (L / 2^32)^2 for cubic boxes (tiled mixed kernel)
+  float qsf;                       // L / 2^32 as FP32 (cubic boxes)
 };
 
 // Lennard-Jones coefficients of one type pair (src/interaction/LennardJones.hpp:79-87).

@@ -597,9 +597,28 @@ __device__ __forceinline__ int stage_tile(const TileRun* __restrict__ runs, cons
   const int nr = nruns[t];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)t * kMaxRuns + threadIdx.x];
   __syncthreads();
-  for (int r = 0; r < nr; ++r) {
-    const int cnt = sr[r].count, off = sr[r].lds_off, ss = sr[r].slot_start;
-    for (int q = threadIdx.x; q < cnt; q += blockDim.x) lds[off + q] = src[ss + q];
+  const int total = nr > 0 ? sr[nr - 1].lds_off + sr[nr - 1].count : 0;
+  const int bs = blockDim.x;
+  // eight independent loads in flight per thread, then eight LDS stores
+  for (int base = 0; base < total; base += 8 * bs) {
+    V v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * bs + (int)threadIdx.x;
+      if (e < total) {
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (sr[mid].lds_off <= e) lo = mid; else hi = mid - 1;
+        }
+        v[u] = src[sr[lo].slot_start + (e - sr[lo].lds_off)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * bs + (int)threadIdx.x;
+      if (e < total) lds[e] = v[u];
+    }
   }
   __syncthreads();
   return nr;
@@ -637,41 +656,58 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
       for (int oy = -1; oy <= 1; ++oy) {
         int qy = cy + oy, sy = 0;
         if (qy < 0) { qy += ny; sy = -1; } else if (qy >= ny) { qy -= ny; sy = 1; }
-        for (int ox = -1; ox <= 1; ++ox) {
-          int qx = cx + ox, sx = 0;
+        // the three cells of this x-row: issue their cstart loads together
+        int c2s[3], sxs[3], jbs[3], jes[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          int qx = cx + a - 1, sx = 0;
           if (qx < 0) { qx += nx; sx = -1; } else if (qx >= nx) { qx -= nx; sx = 1; }
+          c2s[a] = qx + nx * (qy + ny * qz);
+          sxs[a] = sx;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { jbs[a] = cstart[c2s[a]]; jes[a] = cstart[c2s[a] + 1]; }
+        for (int a = 0; a < 3; ++a) {
+          const int ox = a - 1, sx = sxs[a], c2 = c2s[a];
           const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
           const bool unshifted = (sx | sy | sz) == 0;
-          const int c2 = qx + nx * (qy + ny * qz);
           const TileRun R = sr[find_run_by_cell(sr, nr, c2)];
-          const int jb = cstart[c2], je = cstart[c2 + 1];
+          const int jb = jbs[a], je = jes[a];
           const int base = R.lds_off + (jb - R.slot_start);
-          for (int j = jb; j < je; ++j) {
-            if (j == i && unshifted) continue;
-            const int e = base + (j - jb);
-            const uint4 qj = lds[e];
-            const float dx = (float)(int)(qi.x - qj.x) * g.qs[0];
-            const float dy = (float)(int)(qi.y - qj.y) * g.qs[1];
-            const float dz = (float)(int)(qi.z - qj.z) * g.qs[2];
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 > cut_hi) continue;
-            bool in = r2 < cut_lo;
-            if (!in) {
-              const double4 pi = pos[i], pj = pos[j];
-              const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
-              double ex, ey, ez;
-              if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
-              else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
-              in = (ex * ex + ey * ey + ez * ez) <= cutsq;
+          for (int j0 = jb; j0 < je; j0 += 4) {
+            uint4 qv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) qv[u] = (j0 + u < je) ? lds[base + (j0 + u - jb)] : qi;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int j = j0 + u;
+              if (j >= je) break;
+              if (j == i && unshifted) continue;
+              const int e = base + (j - jb);
+              const uint4 qj = qv[u];
+              const float dx = (float)(int)(qi.x - qj.x) * g.qs[0];
+              const float dy = (float)(int)(qi.y - qj.y) * g.qs[1];
+              const float dz = (float)(int)(qi.z - qj.z) * g.qs[2];
+              const float r2 = dx * dx + dy * dy + dz * dz;
+              if (r2 > cut_hi) continue;
+              bool in = r2 < cut_lo;
+              if (!in) {
+                const double4 pi = pos[i], pj = pos[j];
+                const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
+                double ex, ey, ez;
+                if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
+                else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
+                in = (ex * ex + ey * ey + ez * ez) <= cutsq;
+              }
+              if (!in) continue;
+              if (HAS_EXCL && is_excluded(myid, id[j], ex_start, ex_ids)) continue;
+              if (k & 1) {
+                if (k < cap) out[(size_t)(k >> 1) * 64] = pending | ((uint32_t)e << 16);
+              } else {
+                pending = (uint32_t)e;
+              }
+              ++k;
             }
-            if (!in) continue;
-            if (HAS_EXCL && is_excluded(myid, id[j], ex_start, ex_ids)) continue;
-            if (k & 1) {
-              if (k < cap) out[(size_t)(k >> 1) * 64] = pending | ((uint32_t)e << 16);
-            } else {
-              pending = (uint32_t)e;
-            }
-            ++k;
           }
         }
       }
@@ -690,7 +726,7 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
 
 // Mixed-precision LJ over the tiled list: the same arithmetic as k_lj_mixed
 // with the neighbour positions read from LDS.
-template <bool ONETYPE, bool ENERGY, bool ACCUM, int kChunk>
+template <bool ONETYPE, bool ENERGY, bool ACCUM, bool CUBIC>
 __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     int n, const uint4* __restrict__ posq, const double4* __restrict__ pos,
     const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int capw, Geom g,
@@ -725,63 +761,80 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     float e32 = 0.f, w32 = 0.f;
     float t32[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    uint32_t word = 0;
-    for (int k = 0; k < c; ++k) {
-      if ((k & 1) == 0) word = np[(size_t)(k >> 1) * 64];
-      const int e = (k & 1) ? (int)(word >> 16) : (int)(word & 0xFFFFu);
-      const uint4 qj = lds[e];
-      const float dx = (float)(int)(qi.x - qj.x) * g.qs[0];
-      const float dy = (float)(int)(qi.y - qj.y) * g.qs[1];
-      const float dz = (float)(int)(qi.z - qj.z) * g.qs[2];
-      if (!ONETYPE) {
-        const float4 t = tab[rowi + (int)qj.w];
-        ff1 = t.x; ff2 = t.y; rc2 = t.z; shift = t.w;
-        if (ENERGY) { const float2 et = etab[rowi + (int)qj.w]; sig2 = et.x; eps4 = et.y; }
-      }
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      bool in = r2 <= rc2;
-      if (__builtin_fabsf(r2 - rc2) <= 1e-5f * rc2 && rc2 > 0.f) {
-        if (!ONETYPE) drc2 = prm[rowi + (int)qj.w].drc2;
-        const int j = slot_of_lds(sr, nr, e);
-        const double4 pi = pos[i], pj = pos[j];
-        double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-        ex = __builtin_fma(-g.L[0], __builtin_rint(ex * g.invL[0]), ex);
-        ey = __builtin_fma(-g.L[1], __builtin_rint(ey * g.invL[1]), ey);
-        ez = __builtin_fma(-g.L[2], __builtin_rint(ez * g.invL[2]), ez);
-        in = (ex * ex + ey * ey + ez * ez) <= drc2;
-      }
-      const float inv = rcp_nr(r2);
-      const float r6 = inv * inv * inv;
-      float ff = r6 * (ff1 * r6 - ff2) * inv;
-      ff = in ? ff : 0.0f;
-      ax = __builtin_fmaf(dx, ff, ax);
-      ay = __builtin_fmaf(dy, ff, ay);
-      az = __builtin_fmaf(dz, ff, az);
-      if (ENERGY) {
-        const float f2 = sig2 * inv;
-        const float f6 = f2 * f2 * f2;
-        const float en = eps4 * (f6 * f6 - f6) - shift;
-        e32 += in ? en : 0.0f;
-        w32 += r2 * ff;
-        t32[0] += dx * dx * ff; t32[1] += dy * dy * ff; t32[2] += dz * dz * ff;
-        t32[3] += dx * dy * ff; t32[4] += dx * dz * ff; t32[5] += dy * dz * ff;
-      }
-      if ((k % kChunk) == kChunk - 1) {
-        Ax += (double)ax; Ay += (double)ay; Az += (double)az;
-        ax = ay = az = 0.f;
-        if (ENERGY) {
-          acc[0] += (double)e32; acc[1] += (double)w32;
+    const int nw = (c + 1) >> 1;
+    uint32_t wn[4];
 #pragma unroll
-          for (int q = 0; q < 6; ++q) { acc[2 + q] += (double)t32[q]; t32[q] = 0.f; }
-          e32 = w32 = 0.f;
+    for (int u = 0; u < 4; ++u) wn[u] = (u < nw) ? np[(size_t)u * 64] : 0u;
+    for (int kw = 0; kw < nw; kw += 4) {
+      uint32_t w[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w[u] = wn[u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) wn[u] = (kw + 4 + u < nw) ? np[(size_t)(kw + 4 + u) * 64] : 0u;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = 2 * kw + u;
+        if (k >= c) break;
+        const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
+        const uint4 qj = lds[e];
+        const int ix = (int)(qi.x - qj.x), iy = (int)(qi.y - qj.y), iz = (int)(qi.z - qj.z);
+        if (!ONETYPE) {
+          const float4 t = tab[rowi + (int)qj.w];
+          ff1 = t.x; ff2 = t.y; rc2 = t.z; shift = t.w;
+          if (ENERGY) { const float2 et = etab[rowi + (int)qj.w]; sig2 = et.x; eps4 = et.y; }
+        }
+        // r^2 from the exact integer differences in FP64, one rounding to FP32
+        const double ux = (double)ix, uy = (double)iy, uz = (double)iz;
+        float dx, dy, dz;
+        double r2d;
+        if (CUBIC) {
+          r2d = (ux * ux + uy * uy + uz * uz) * g.qs2;
+          dx = (float)ix; dy = (float)iy; dz = (float)iz;        // fixed-point units
+        } else {
+          const double ex = ux * g.qsd[0], ey = uy * g.qsd[1], ez = uz * g.qsd[2];
+          r2d = ex * ex + ey * ey + ez * ez;
+          dx = (float)ex; dy = (float)ey; dz = (float)ez;        // length units
+        }
+        const float r2 = (float)r2d;
+        bool in = r2 <= rc2;
+        if (__builtin_fabsf(r2 - rc2) <= 1e-5f * rc2 && rc2 > 0.f) {
+          if (!ONETYPE) drc2 = prm[rowi + (int)qj.w].drc2;
+          const int j = slot_of_lds(sr, nr, e);
+          const double4 pi = pos[i], pj = pos[j];
+          double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+          ex = __builtin_fma(-g.L[0], __builtin_rint(ex * g.invL[0]), ex);
+          ey = __builtin_fma(-g.L[1], __builtin_rint(ey * g.invL[1]), ey);
+          ez = __builtin_fma(-g.L[2], __builtin_rint(ez * g.invL[2]), ez);
+          in = (ex * ex + ey * ey + ez * ez) <= drc2;
+        }
+        const float inv = rcp_nr(r2);
+        const float r6 = inv * inv * inv;
+        float ff = r6 * (ff1 * r6 - ff2) * inv;
+        ff = in ? ff : 0.0f;
+        const float sff = CUBIC ? ff * g.qsf : ff;   // F = d_units * (s * ff) for cubic boxes
+        ax = __builtin_fmaf(dx, sff, ax);
+        ay = __builtin_fmaf(dy, sff, ay);
+        az = __builtin_fmaf(dz, sff, az);
+        if (ENERGY) {
+          const float f2 = sig2 * inv;
+          const float f6 = f2 * f2 * f2;
+          const float en = eps4 * (f6 * f6 - f6) - shift;
+          e32 += in ? en : 0.0f;
+          w32 += r2 * ff;
+          const float sx = CUBIC ? dx * g.qsf : dx, sy = CUBIC ? dy * g.qsf : dy, sz = CUBIC ? dz * g.qsf : dz;
+          t32[0] += sx * sx * ff; t32[1] += sy * sy * ff; t32[2] += sz * sz * ff;
+          t32[3] += sx * sy * ff; t32[4] += sx * sz * ff; t32[5] += sy * sz * ff;
         }
       }
-    }
-    Ax += (double)ax; Ay += (double)ay; Az += (double)az;
-    if (ENERGY) {
-      acc[0] += (double)e32; acc[1] += (double)w32;
+      // carry the FP32 partial sums of this batch into FP64
+      Ax += (double)ax; Ay += (double)ay; Az += (double)az;
+      ax = ay = az = 0.f;
+      if (ENERGY) {
+        acc[0] += (double)e32; acc[1] += (double)w32;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) acc[2 + q] += (double)t32[q];
+        for (int q = 0; q < 6; ++q) { acc[2 + q] += (double)t32[q]; t32[q] = 0.f; }
+        e32 = w32 = 0.f;
+      }
     }
     if (!ENERGY) {
       if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
@@ -819,36 +872,48 @@ __global__ __launch_bounds__(256) void k_lj_tiled_fp64(
     double sig2 = prm[0].sig2, eps4 = prm[0].eps4;
     int set = prm[0].set;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
-    uint32_t word = 0;
-    for (int k = 0; k < c; ++k) {
-      if ((k & 1) == 0) word = np[(size_t)(k >> 1) * 64];
-      const int e = (k & 1) ? (int)(word >> 16) : (int)(word & 0xFFFFu);
-      const double4 pj = ldsd[e];
-      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      dx = __builtin_fma(-g.L[0], __builtin_rint(dx * g.invL[0]), dx);
-      dy = __builtin_fma(-g.L[1], __builtin_rint(dy * g.invL[1]), dy);
-      dz = __builtin_fma(-g.L[2], __builtin_rint(dz * g.invL[2]), dz);
-      if (!ONETYPE) {
-        const LJParam& p = prm[rowi + (int)pj.w];
-        ff1 = p.dff1; ff2 = p.dff2; rc2 = p.drc2; shift = p.dshift; sig2 = p.sig2; eps4 = p.eps4;
-        set = p.set;
-      }
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      const bool in = set && r2 <= rc2;
-      const double inv = rcp_nr(r2);
-      const double r6 = inv * inv * inv;
-      double ff = r6 * (ff1 * r6 - ff2) * inv;
-      ff = in ? ff : 0.0;
-      Ax = __builtin_fma(dx, ff, Ax);
-      Ay = __builtin_fma(dy, ff, Ay);
-      Az = __builtin_fma(dz, ff, Az);
-      if (ENERGY) {
-        const double f2 = sig2 * inv;
-        const double f6 = f2 * f2 * f2;
-        acc[0] += in ? eps4 * (f6 * f6 - f6) - shift : 0.0;
-        acc[1] += r2 * ff;
-        acc[2] += dx * dx * ff; acc[3] += dy * dy * ff; acc[4] += dz * dz * ff;
-        acc[5] += dx * dy * ff; acc[6] += dx * dz * ff; acc[7] += dy * dz * ff;
+    const int nw = (c + 1) >> 1;
+    uint32_t wn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) wn[u] = (u < nw) ? np[(size_t)u * 64] : 0u;
+    for (int kw = 0; kw < nw; kw += 4) {
+      uint32_t w[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w[u] = wn[u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) wn[u] = (kw + 4 + u < nw) ? np[(size_t)(kw + 4 + u) * 64] : 0u;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = 2 * kw + u;
+        if (k >= c) break;
+        const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
+        const double4 pj = ldsd[e];
+        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        dx = __builtin_fma(-g.L[0], __builtin_rint(dx * g.invL[0]), dx);
+        dy = __builtin_fma(-g.L[1], __builtin_rint(dy * g.invL[1]), dy);
+        dz = __builtin_fma(-g.L[2], __builtin_rint(dz * g.invL[2]), dz);
+        if (!ONETYPE) {
+          const LJParam& p = prm[rowi + (int)pj.w];
+          ff1 = p.dff1; ff2 = p.dff2; rc2 = p.drc2; shift = p.dshift; sig2 = p.sig2; eps4 = p.eps4;
+          set = p.set;
+        }
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const bool in = set && r2 <= rc2;
+        const double inv = rcp_nr(r2);
+        const double r6 = inv * inv * inv;
+        double ff = r6 * (ff1 * r6 - ff2) * inv;
+        ff = in ? ff : 0.0;
+        Ax = __builtin_fma(dx, ff, Ax);
+        Ay = __builtin_fma(dy, ff, Ay);
+        Az = __builtin_fma(dz, ff, Az);
+        if (ENERGY) {
+          const double f2 = sig2 * inv;
+          const double f6 = f2 * f2 * f2;
+          acc[0] += in ? eps4 * (f6 * f6 - f6) - shift : 0.0;
+          acc[1] += r2 * ff;
+          acc[2] += dx * dx * ff; acc[3] += dy * dy * ff; acc[4] += dz * dz * ff;
+          acc[5] += dx * dy * ff; acc[6] += dx * dz * ff; acc[7] += dy * dz * ff;
+        }
       }
     }
     if (!ENERGY) {
@@ -1205,7 +1270,9 @@ void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool 
                      double* fx, double* fy, double* fz, double* partial) {
   if (n <= 0) return;
   dim3 grid(nblocks(n, T)), block(T);
-#define LJM(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, 8>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial)
+  const bool cubic = g.L[0] == g.L[1] && g.L[1] == g.L[2];
+#define LJM(OT, EN, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, true>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); \
+                             else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, false>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); } while (0)
 #define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64<OT, EN, AC>), grid, block, lds_bytes, s, n, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial)
   if (mixed) {
     if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }

@@ -624,7 +624,10 @@ void set_geom(Geom& g, const double box[3], const int32_t cells[3]) {
     g.ics[d] = 1.0 / cs;
     g.qinv[d] = 4294967296.0 / box[d];
     g.qs[d] = (float)(box[d] / 4294967296.0);
+    g.qsd[d] = box[d] / 4294967296.0;
   }
+  g.qs2 = (box[0] / 4294967296.0) * (box[0] / 4294967296.0);
+  g.qsf = (float)(box[0] / 4294967296.0);
 }
 
 template <class T>
@@ -1247,6 +1250,16 @@ int espp_prof_stats(espp_sim* s, int32_t kernel, int64_t* launches, double* tota
     s->prof_collect();
     *launches = s->prof_count[kernel];
     *total_ms = s->prof_ms[kernel];
+  });
+}
+
+int espp_tile_stats(espp_sim* s, int32_t* tile, int32_t* max_region) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    s->ensure_lists();
+    s->ensure_plan();
+    if (tile) *tile = s->T;
+    if (max_region) *max_region = s->max_region;
   });
 }
 

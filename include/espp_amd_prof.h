@@ -41,6 +41,10 @@ int espp_prof_stats(espp_sim *sim, int32_t kernel, int64_t *launches, double *to
  * the padded row capacity and the largest row of the last build. */
 int espp_vl_stats(espp_sim *sim, int32_t vl, int64_t *entries, int32_t *cap, int32_t *maxcount);
 
+/* Particles per tile of the current tile plan (0: untiled kernels) and the
+ * largest staged region in particles. */
+int espp_tile_stats(espp_sim *sim, int32_t *tile, int32_t *max_region);
+
 #ifdef __cplusplus
 }
 #endif
