@@ -2,8 +2,9 @@
 //
 //   binning          k_fold_cell, k_scan_cells, k_scatter, k_sort_cells, k_permute
 //   neighbour build  k_nbr_build   (full list, wavefront-padded ragged SoA)
-//   pair forces      k_lj_mixed    (32-bit fixed-point positions, FP32 pair math,
-//                                   FP32 chunks carried into FP64 accumulators)
+//   pair forces      k_lj_tiled_mixed / k_lj_mixed (32-bit fixed-point positions,
+//                                   FP32 reciprocal and LJ terms with an FP64
+//                                   residual correction, FP64 force accumulation)
 //                    k_lj_fp64     (all-FP64 comparison path)
 //   bonds            k_fene        (per-particle bond adjacency, no atomics)
 //   integrator       k_integrate1, k_thermo_integrate2
@@ -48,6 +49,23 @@ __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
+}
+
+
+// Mixed-precision LJ force factor for one pair.  r^2 arrives in FP64 from exact
+// fixed-point differences; the reciprocal, its powers and the two LJ terms are
+// FP32; one FP64 residual, 1 - r2d * inv, corrects to first order both the
+// rounding of r^2 to FP32 and the error of v_rcp_f32:
+//   ff(u) = ff1 u^7 - ff2 u^4,  u = inv (1 + eps)  =>  ff += eps (7 a - 4 b).
+__device__ __forceinline__ float lj_ff_mixed(double r2d, float ff1, float ff2, float* inv_out) {
+  const float r2 = (float)r2d;
+  const float inv = __builtin_amdgcn_rcpf(r2);
+  const float r6 = inv * inv * inv;
+  const float b = ff2 * r6 * inv;
+  const float a = ff1 * r6 * r6 * inv;
+  const float eps = (float)__builtin_fma(-r2d, (double)inv, 1.0);
+  *inv_out = inv;
+  return (a - b) + eps * (7.0f * a - 4.0f * b);
 }
 
 // ------------------------------------------------------------------ binning
@@ -328,7 +346,7 @@ __device__ void block_reduce_store(double (&v)[NV], double* __restrict__ partial
 // whose FP32 r^2 lies within 1e-5 relative of rc^2 re-decide the cutoff test in
 // FP64 so the set of interacting pairs equals the FP64 path's.
 // Potential: src/interaction/LennardJones.hpp:129-136 (force), :109-114 (energy).
-template <bool ONETYPE, bool ENERGY, bool ACCUM, int kChunk>
+template <bool ONETYPE, bool ENERGY, bool ACCUM>
 __global__ __launch_bounds__(256) void k_lj_mixed(
     int n, const uint4* __restrict__ posq, const double4* __restrict__ pos,
     const int* __restrict__ nbr, const int* __restrict__ cnt, int cap, Geom g,
@@ -356,65 +374,46 @@ __global__ __launch_bounds__(256) void k_lj_mixed(
     float sig2 = (float)prm[0].sig2, eps4 = (float)prm[0].eps4;
     double drc2 = prm[0].drc2;
     const int rowi = (int)qi.w * ntypes;
-    float ax = 0.f, ay = 0.f, az = 0.f;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
-    float e32 = 0.f, w32 = 0.f;
-    float t32[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int k = 0; k < c; ++k) {
       const int j = np[(size_t)k * 64];
       const uint4 qj = posq[j];
-      const float dx = (float)(int)(qi.x - qj.x) * g.qs[0];
-      const float dy = (float)(int)(qi.y - qj.y) * g.qs[1];
-      const float dz = (float)(int)(qi.z - qj.z) * g.qs[2];
       if (!ONETYPE) {
         const float4 t = tab[rowi + (int)qj.w];
         ff1 = t.x; ff2 = t.y; rc2 = t.z; shift = t.w;
         if (ENERGY) { const float2 et = etab[rowi + (int)qj.w]; sig2 = et.x; eps4 = et.y; }
       }
-      const float r2 = dx * dx + dy * dy + dz * dz;
+      const double ex = (double)(int)(qi.x - qj.x) * g.qsd[0];
+      const double ey = (double)(int)(qi.y - qj.y) * g.qsd[1];
+      const double ez = (double)(int)(qi.z - qj.z) * g.qsd[2];
+      const double r2d = ex * ex + ey * ey + ez * ez;
+      const float r2 = (float)r2d;
       bool in = r2 <= rc2;
       if (__builtin_fabsf(r2 - rc2) <= 1e-5f * rc2 && rc2 > 0.f) {
-        // rare: settle the cutoff test in FP64 from the FP64 positions
         if (!ONETYPE) drc2 = prm[rowi + (int)qj.w].drc2;
         const double4 pi = pos[i], pj = pos[j];
-        double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-        ex = __builtin_fma(-g.L[0], __builtin_rint(ex * g.invL[0]), ex);
-        ey = __builtin_fma(-g.L[1], __builtin_rint(ey * g.invL[1]), ey);
-        ez = __builtin_fma(-g.L[2], __builtin_rint(ez * g.invL[2]), ez);
-        in = (ex * ex + ey * ey + ez * ez) <= drc2;
+        double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
+        hx = __builtin_fma(-g.L[0], __builtin_rint(hx * g.invL[0]), hx);
+        hy = __builtin_fma(-g.L[1], __builtin_rint(hy * g.invL[1]), hy);
+        hz = __builtin_fma(-g.L[2], __builtin_rint(hz * g.invL[2]), hz);
+        in = (hx * hx + hy * hy + hz * hz) <= drc2;
       }
-      const float inv = rcp_nr(r2);
-      const float r6 = inv * inv * inv;
-      float ff = r6 * (ff1 * r6 - ff2) * inv;
+      float inv;
+      float ff = lj_ff_mixed(r2d, ff1, ff2, &inv);
       ff = in ? ff : 0.0f;
-      ax = __builtin_fmaf(dx, ff, ax);
-      ay = __builtin_fmaf(dy, ff, ay);
-      az = __builtin_fmaf(dz, ff, az);
+      const double fd = (double)ff;
+      Ax = __builtin_fma(ex, fd, Ax);
+      Ay = __builtin_fma(ey, fd, Ay);
+      Az = __builtin_fma(ez, fd, Az);
       if (ENERGY) {
         const float f2 = sig2 * inv;
         const float f6 = f2 * f2 * f2;
-        const float e = eps4 * (f6 * f6 - f6) - shift;
-        e32 += in ? e : 0.0f;
-        w32 += r2 * ff;
-        t32[0] += dx * dx * ff; t32[1] += dy * dy * ff; t32[2] += dz * dz * ff;
-        t32[3] += dx * dy * ff; t32[4] += dx * dz * ff; t32[5] += dy * dz * ff;
+        const float en = eps4 * (f6 * f6 - f6) - shift;
+        acc[0] += in ? (double)en : 0.0;
+        acc[1] += r2d * fd;
+        acc[2] += ex * ex * fd; acc[3] += ey * ey * fd; acc[4] += ez * ez * fd;
+        acc[5] += ex * ey * fd; acc[6] += ex * ez * fd; acc[7] += ey * ez * fd;
       }
-      if ((k % kChunk) == kChunk - 1) {
-        Ax += (double)ax; Ay += (double)ay; Az += (double)az;
-        ax = ay = az = 0.f;
-        if (ENERGY) {
-          acc[0] += (double)e32; acc[1] += (double)w32;
-#pragma unroll
-          for (int q = 0; q < 6; ++q) { acc[2 + q] += (double)t32[q]; t32[q] = 0.f; }
-          e32 = w32 = 0.f;
-        }
-      }
-    }
-    Ax += (double)ax; Ay += (double)ay; Az += (double)az;
-    if (ENERGY) {
-      acc[0] += (double)e32; acc[1] += (double)w32;
-#pragma unroll
-      for (int q = 0; q < 6; ++q) acc[2 + q] += (double)t32[q];
     }
     if (!ENERGY) {
       if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
@@ -423,7 +422,7 @@ __global__ __launch_bounds__(256) void k_lj_mixed(
   }
   if (ENERGY) {
 #pragma unroll
-    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;   // each pair is seen from both ends
+    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
     block_reduce_store<kNPart>(acc, partial);
   }
 }
@@ -757,10 +756,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     float sig2 = (float)prm[0].sig2, eps4 = (float)prm[0].eps4;
     double drc2 = prm[0].drc2;
     const int rowi = (int)qi.w * ntypes;
-    float ax = 0.f, ay = 0.f, az = 0.f;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
-    float e32 = 0.f, w32 = 0.f;
-    float t32[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     const int nw = (c + 1) >> 1;
     uint32_t wn[4];
 #pragma unroll
@@ -783,17 +779,14 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
           ff1 = t.x; ff2 = t.y; rc2 = t.z; shift = t.w;
           if (ENERGY) { const float2 et = etab[rowi + (int)qj.w]; sig2 = et.x; eps4 = et.y; }
         }
-        // r^2 from the exact integer differences in FP64, one rounding to FP32
-        const double ux = (double)ix, uy = (double)iy, uz = (double)iz;
-        float dx, dy, dz;
+        // r^2 from the exact integer differences in FP64
+        double ex = (double)ix, ey = (double)iy, ez = (double)iz;
         double r2d;
         if (CUBIC) {
-          r2d = (ux * ux + uy * uy + uz * uz) * g.qs2;
-          dx = (float)ix; dy = (float)iy; dz = (float)iz;        // fixed-point units
+          r2d = (ex * ex + ey * ey + ez * ez) * g.qs2;           // ex.. stay in fixed-point units
         } else {
-          const double ex = ux * g.qsd[0], ey = uy * g.qsd[1], ez = uz * g.qsd[2];
+          ex *= g.qsd[0]; ey *= g.qsd[1]; ez *= g.qsd[2];        // length units
           r2d = ex * ex + ey * ey + ez * ez;
-          dx = (float)ex; dy = (float)ey; dz = (float)ez;        // length units
         }
         const float r2 = (float)r2d;
         bool in = r2 <= rc2;
@@ -801,39 +794,30 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
           if (!ONETYPE) drc2 = prm[rowi + (int)qj.w].drc2;
           const int j = slot_of_lds(sr, nr, e);
           const double4 pi = pos[i], pj = pos[j];
-          double ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-          ex = __builtin_fma(-g.L[0], __builtin_rint(ex * g.invL[0]), ex);
-          ey = __builtin_fma(-g.L[1], __builtin_rint(ey * g.invL[1]), ey);
-          ez = __builtin_fma(-g.L[2], __builtin_rint(ez * g.invL[2]), ez);
-          in = (ex * ex + ey * ey + ez * ez) <= drc2;
+          double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
+          hx = __builtin_fma(-g.L[0], __builtin_rint(hx * g.invL[0]), hx);
+          hy = __builtin_fma(-g.L[1], __builtin_rint(hy * g.invL[1]), hy);
+          hz = __builtin_fma(-g.L[2], __builtin_rint(hz * g.invL[2]), hz);
+          in = (hx * hx + hy * hy + hz * hz) <= drc2;
         }
-        const float inv = rcp_nr(r2);
-        const float r6 = inv * inv * inv;
-        float ff = r6 * (ff1 * r6 - ff2) * inv;
+        float inv;
+        float ff = lj_ff_mixed(r2d, ff1, ff2, &inv);
         ff = in ? ff : 0.0f;
-        const float sff = CUBIC ? ff * g.qsf : ff;   // F = d_units * (s * ff) for cubic boxes
-        ax = __builtin_fmaf(dx, sff, ax);
-        ay = __builtin_fmaf(dy, sff, ay);
-        az = __builtin_fmaf(dz, sff, az);
+        const double sff = CUBIC ? (double)ff * g.qsd[0] : (double)ff;   // F = d * ff
+        Ax = __builtin_fma(ex, sff, Ax);
+        Ay = __builtin_fma(ey, sff, Ay);
+        Az = __builtin_fma(ez, sff, Az);
         if (ENERGY) {
           const float f2 = sig2 * inv;
           const float f6 = f2 * f2 * f2;
           const float en = eps4 * (f6 * f6 - f6) - shift;
-          e32 += in ? en : 0.0f;
-          w32 += r2 * ff;
-          const float sx = CUBIC ? dx * g.qsf : dx, sy = CUBIC ? dy * g.qsf : dy, sz = CUBIC ? dz * g.qsf : dz;
-          t32[0] += sx * sx * ff; t32[1] += sy * sy * ff; t32[2] += sz * sz * ff;
-          t32[3] += sx * sy * ff; t32[4] += sx * sz * ff; t32[5] += sy * sz * ff;
+          acc[0] += in ? (double)en : 0.0;
+          const double wx = CUBIC ? ex * g.qsd[0] : ex, wy = CUBIC ? ey * g.qsd[0] : ey, wz = CUBIC ? ez * g.qsd[0] : ez;
+          const double fd = (double)ff;
+          acc[1] += r2d * fd;
+          acc[2] += wx * wx * fd; acc[3] += wy * wy * fd; acc[4] += wz * wz * fd;
+          acc[5] += wx * wy * fd; acc[6] += wx * wz * fd; acc[7] += wy * wz * fd;
         }
-      }
-      // carry the FP32 partial sums of this batch into FP64
-      Ax += (double)ax; Ay += (double)ay; Az += (double)az;
-      ax = ay = az = 0.f;
-      if (ENERGY) {
-        acc[0] += (double)e32; acc[1] += (double)w32;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) { acc[2 + q] += (double)t32[q]; t32[q] = 0.f; }
-        e32 = w32 = 0.f;
       }
     }
     if (!ENERGY) {
@@ -1223,7 +1207,7 @@ void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum,
                double* partial) {
   if (n <= 0) return;
   dim3 grid(nblocks(n, 256)), block(256);
-#define LJM(OT, EN, AC) hipLaunchKernelGGL((k_lj_mixed<OT, EN, AC, 8>), grid, block, 0, s, n, posq, pos, nbr, cnt, cap, g, prm, ntypes, fx, fy, fz, partial)
+#define LJM(OT, EN, AC) hipLaunchKernelGGL((k_lj_mixed<OT, EN, AC>), grid, block, 0, s, n, posq, pos, nbr, cnt, cap, g, prm, ntypes, fx, fy, fz, partial)
 #define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_fp64<OT, EN, AC>), grid, block, 0, s, n, pos, nbr, cnt, cap, g, prm, ntypes, fx, fy, fz, partial)
   if (mixed) {
     if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }
@@ -1270,7 +1254,7 @@ void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool 
                      double* fx, double* fy, double* fz, double* partial) {
   if (n <= 0) return;
   dim3 grid(nblocks(n, T)), block(T);
-  const bool cubic = g.L[0] == g.L[1] && g.L[1] == g.L[2];
+  const bool cubic = g.qsd[0] == g.qsd[1] && g.qsd[1] == g.qsd[2];
 #define LJM(OT, EN, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, true>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); \
                              else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, false>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); } while (0)
 #define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64<OT, EN, AC>), grid, block, lds_bytes, s, n, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial)

@@ -614,6 +614,10 @@ void need(bool ok, const char* msg) {
   if (!ok) fail(ESPP_ERR_ARG, msg);
 }
 
+// Fixed-point mirror: x -> low 32 bits of x * 2^(32+k) / L.  Differences of
+// two mirrors are exact int32 and equal the minimum image modulo P = L / 2^k;
+// k is the largest value keeping P >= 6 cell sizes, so every candidate of the
+// 27 neighbour cells (|dx| < 2 cells plus drift) is represented unaliased.
 void set_geom(Geom& g, const double box[3], const int32_t cells[3]) {
   for (int d = 0; d < 3; ++d) {
     g.L[d] = box[d];
@@ -622,12 +626,15 @@ void set_geom(Geom& g, const double box[3], const int32_t cells[3]) {
     g.n[d] = cells[d];
     double cs = (box[d] - 0.0) / (double)cells[d];
     g.ics[d] = 1.0 / cs;
-    g.qinv[d] = 4294967296.0 / box[d];
-    g.qs[d] = (float)(box[d] / 4294967296.0);
-    g.qsd[d] = box[d] / 4294967296.0;
+    int kbits = 0;
+    while (kbits < 16 && (double)cells[d] / (double)(1 << (kbits + 1)) >= 6.0) ++kbits;
+    const double scale = std::ldexp(1.0, 32 + kbits);
+    g.qinv[d] = scale / box[d];
+    g.qs[d] = (float)(box[d] / scale);
+    g.qsd[d] = box[d] / scale;
   }
-  g.qs2 = (box[0] / 4294967296.0) * (box[0] / 4294967296.0);
-  g.qsf = (float)(box[0] / 4294967296.0);
+  g.qs2 = g.qsd[0] * g.qsd[0];
+  g.qsf = (float)g.qsd[0];
 }
 
 template <class T>
