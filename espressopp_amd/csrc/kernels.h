@@ -7,6 +7,12 @@ namespace espp {
 void launch_fold_cell(hipStream_t s, int n, double4* pos, int4* img, int* cell_of, int* rank,
                       int* cell_count, const Geom& g, Ctl* ctl, int do_fold);
 void launch_scan_cells(hipStream_t s, int ncell, const int* count, int* start);
+void launch_scan_multi(hipStream_t s, int ncell, const int* count, int* start, int* blocksum, int* blockoff);
+int scan_blocks(int ncell);
+void launch_step_fused(hipStream_t s, int n, const int* id, double4* pos, double4* vel, double* fx, double* fy,
+                       double* fz, int lang, double pref1, double pref2, unsigned long long seed,
+                       unsigned long long call, const unsigned char* excl, int nid, int do_int2, double dt,
+                       uint4* posq, const Geom& g, double* blockmax, Ctl* ctl);
 void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start,
                     int* order);
 void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const int* id);

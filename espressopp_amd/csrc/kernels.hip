@@ -12,6 +12,8 @@
 //
 // Launch wrappers at the bottom are the only entry points used by the host
 // runtime (sim.hip).
+#include <type_traits>
+
 #include "espp_dev.h"
 #include "kernels.h"
 
@@ -70,6 +72,26 @@ __device__ __forceinline__ float lj_ff_mixed(double r2d, float ff1, float ff2, f
 
 // ------------------------------------------------------------------ binning
 
+// Rank of this lane among all particles counted into cell `key`: lanes of one
+// wavefront holding the same key share one atomicAdd (a wavefront spans only a
+// few cells because particles arrive in cell order) and are ranked in lane order.
+__device__ __forceinline__ int wave_aggregated_rank(int key, bool live, int* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(live);
+  int rank = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lk = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(live && key == lk);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&count[lk], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (live && key == lk) rank = base + __popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return rank;
+}
+
 // Fold every coordinate (OrthorhombicBC::foldCoordinate, src/bc/OrthorhombicBC.cpp:144-161)
 // and map the folded position to its inner cell (CellGrid::mapPositionToCellChecked
 // for one node, src/storage/CellGrid.cpp:98-136).  Cell ranks come from atomics;
@@ -79,7 +101,8 @@ __global__ void k_fold_cell(int n, double4* __restrict__ pos, int4* __restrict__
                             int* __restrict__ cell_count, Geom g, Ctl* ctl, int do_fold) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const bool live = i < n;
+  if (!live) i = n - 1;          // inactive lanes mirror the last particle and write nothing
   double4 p = pos[i];
   int4 im = img[i];
   double x[3] = {p.x, p.y, p.z};
@@ -99,13 +122,15 @@ __global__ void k_fold_cell(int n, double4* __restrict__ pos, int4* __restrict__
     else if (cc >= g.n[d] + 1) cc = g.n[d];
     c[d] = cc - 1;
   }
+  int cell = c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+  const int r = wave_aggregated_rank(cell, live, cell_count);
+  if (!live) return;
   if (bad) ctl->nanpos = 1;
   p.x = x[0]; p.y = x[1]; p.z = x[2];
   pos[i] = p;
   img[i] = make_int4(iv[0], iv[1], iv[2], 0);
-  int cell = c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
   cell_of[i] = cell;
-  rank[i] = atomicAdd(&cell_count[cell], 1);
+  rank[i] = r;
 }
 
 // Exclusive scan of per-cell counts in one workgroup of 1024 threads: tiles of
@@ -155,6 +180,71 @@ __global__ __launch_bounds__(1024) void k_scan_cells(int ncell, const int* __res
   }
   if (tid == 0) start[ncell] = carry_s;
 }
+
+
+// Multi-block exclusive scan of the cell counts: per-chunk totals, a scan of the
+// totals (k_scan_cells on the short array) and a per-chunk scan with its offset.
+constexpr int kScanChunk = 4096;   // 1024 threads x 4 cells
+
+__device__ __forceinline__ int block_sum_1024(int v, int* wsum) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) wsum[wid] = v;
+  __syncthreads();
+  int t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < 16; ++w) t += wsum[w];
+  return t;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_reduce(int ncell, const int* __restrict__ count,
+                                                      int* __restrict__ blocksum) {
+  __shared__ int wsum[16];
+  const int idx = blockIdx.x * kScanChunk + threadIdx.x * 4;
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v += (idx + k < ncell) ? count[idx + k] : 0;
+  const int t = block_sum_1024(v, wsum);
+  if (threadIdx.x == 0) blocksum[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_down(int ncell, const int* __restrict__ count,
+                                                    const int* __restrict__ blockoff, int* __restrict__ start) {
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int idx = blockIdx.x * kScanChunk + tid * 4;
+  int v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = (idx + k < ncell) ? count[idx + k] : 0;
+  const int local = v[0] + v[1] + v[2] + v[3];
+  int incl = local;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  if (tid < 64) {
+    const int w = (tid < 16) ? wsum[tid] : 0;
+    int wi = w;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const int t = __shfl_up(wi, o, 64);
+      if (tid >= o) wi += t;
+    }
+    if (tid < 16) wsum[tid] = wi - w;
+  }
+  __syncthreads();
+  int run = blockoff[blockIdx.x] + wsum[wid] + incl - local;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (idx + k < ncell) start[idx + k] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && tid == 0) start[ncell] = blockoff[gridDim.x];
+}
+
 
 __global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
                           const int* __restrict__ start, int* __restrict__ order) {
@@ -623,21 +713,66 @@ __device__ __forceinline__ int stage_tile(const TileRun* __restrict__ runs, cons
   return nr;
 }
 
+template <typename S>
+__device__ __forceinline__ S narrow_slot(const uint4& v);
+template <>
+__device__ __forceinline__ uint4 narrow_slot<uint4>(const uint4& v) { return v; }
+template <>
+__device__ __forceinline__ uint3 narrow_slot<uint3>(const uint4& v) { return make_uint3(v.x, v.y, v.z); }
+
+// stage_tile for the fixed-point mirror, storing S (uint3 or uint4) per particle
+template <typename S>
+__device__ __forceinline__ int stage_tile_q(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
+                                            const uint4* __restrict__ src, S* lds, TileRun* sr) {
+  const int t = blockIdx.x;
+  const int nr = nruns[t];
+  if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)t * kMaxRuns + threadIdx.x];
+  __syncthreads();
+  const int total = nr > 0 ? sr[nr - 1].lds_off + sr[nr - 1].count : 0;
+  const int bs = blockDim.x;
+  for (int base = 0; base < total; base += 8 * bs) {
+    uint4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * bs + (int)threadIdx.x;
+      if (e < total) {
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (sr[mid].lds_off <= e) lo = mid; else hi = mid - 1;
+        }
+        v[u] = src[sr[lo].slot_start + (e - sr[lo].lds_off)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = base + u * bs + (int)threadIdx.x;
+      if (e < total) lds[e] = narrow_slot<S>(v[u]);
+    }
+  }
+  __syncthreads();
+  return nr;
+}
+
 // Tiled Verlet-list build.  Candidates of the 27 neighbour cells are read from
-// the staged fixed-point positions; pairs whose FP32 r^2 is within `win` of
+// the fixed-point mirror (lanes of one cell read the same candidates, so the
+// loads broadcast); pairs whose FP32 r^2 is within a small window of
 // (rc+skin)^2 are settled by the reference's FP64 test (same orientation rule
-// as k_nbr_build), all others are decided by the FP32 value.
+// as k_nbr_build), all others by the FP32 value.  Entries are written as the
+// candidate's offset in the tile's staged region, the layout the tiled force
+// kernels read.
 template <bool HAS_EXCL>
 __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     int n, const uint4* __restrict__ posq, const double4* __restrict__ pos, const int* __restrict__ id,
     const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g, double cutsq, float cut_lo,
-    float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,
+    float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,  // cut_lo/hi in units of qs[0]^2
     const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint32_t* __restrict__ nbr,
     int* __restrict__ cnt, int capw, Ctl* ctl) {
 #pragma clang fp contract(off)
-  extern __shared__ uint4 lds[];
   __shared__ TileRun sr[kMaxRuns];
-  const int nr = stage_tile<uint4>(runs, nruns, posq, lds, sr);
+  const int nr = nruns[blockIdx.x];
+  if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)blockIdx.x * kMaxRuns + threadIdx.x];
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int k = 0;
   if (i < n && nr > 0) {
@@ -649,6 +784,7 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     uint32_t* out = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
     uint32_t pending = 0;
     const int cap = 2 * capw;
+    const float3 qsr = make_float3(1.0f, (float)(g.qsd[1] / g.qsd[0]), (float)(g.qsd[2] / g.qsd[0]));
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nz; sz = -1; } else if (qz >= nz) { qz -= nz; sz = 1; }
@@ -676,7 +812,7 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
           for (int j0 = jb; j0 < je; j0 += 4) {
             uint4 qv[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) qv[u] = (j0 + u < je) ? lds[base + (j0 + u - jb)] : qi;
+            for (int u = 0; u < 4; ++u) qv[u] = (j0 + u < je) ? posq[j0 + u] : qi;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               const int j = j0 + u;
@@ -684,9 +820,10 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
               if (j == i && unshifted) continue;
               const int e = base + (j - jb);
               const uint4 qj = qv[u];
-              const float dx = (float)(int)(qi.x - qj.x) * g.qs[0];
-              const float dy = (float)(int)(qi.y - qj.y) * g.qs[1];
-              const float dz = (float)(int)(qi.z - qj.z) * g.qs[2];
+              // prefilter in fixed-point units: thresholds are pre-divided by the scale^2
+              const float dx = (float)(int)(qi.x - qj.x) * qsr.x;
+              const float dy = (float)(int)(qi.y - qj.y) * qsr.y;
+              const float dz = (float)(int)(qi.z - qj.z) * qsr.z;
               const float r2 = dx * dx + dy * dy + dz * dz;
               if (r2 > cut_hi) continue;
               bool in = r2 < cut_lo;
@@ -732,10 +869,13 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     const TileRun* __restrict__ runs, const int* __restrict__ nruns, const LJParam* __restrict__ prm,
     int ntypes, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
     double* __restrict__ partial) {
-  extern __shared__ uint4 lds[];
+  // one type: LDS holds x, y, z (12 bytes); several types: x, y, z, type
+  using Slot = typename std::conditional<ONETYPE, uint3, uint4>::type;
+  extern __shared__ uint4 lds_raw[];
+  Slot* lds = reinterpret_cast<Slot*>(lds_raw);
   __shared__ TileRun sr[kMaxRuns];
-  __shared__ float4 tab[kMaxTypes * kMaxTypes];
-  __shared__ float2 etab[kMaxTypes * kMaxTypes];
+  __shared__ float4 tab[ONETYPE ? 1 : kMaxTypes * kMaxTypes];
+  __shared__ float2 etab[ONETYPE ? 1 : kMaxTypes * kMaxTypes];
   if (!ONETYPE) {
     for (int t = threadIdx.x; t < ntypes * ntypes; t += blockDim.x) {
       const LJParam p = prm[t];
@@ -743,7 +883,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
       etab[t] = make_float2((float)p.sig2, (float)p.eps4);
     }
   }
-  const int nr = stage_tile<uint4>(runs, nruns, posq, lds, sr);
+  const int nr = stage_tile_q<Slot>(runs, nruns, posq, lds, sr);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[kNPart];
 #pragma unroll
@@ -772,12 +912,14 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
         const int k = 2 * kw + u;
         if (k >= c) break;
         const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
-        const uint4 qj = lds[e];
+        const Slot qj = lds[e];
         const int ix = (int)(qi.x - qj.x), iy = (int)(qi.y - qj.y), iz = (int)(qi.z - qj.z);
+        int tj = 0;
+        if constexpr (!ONETYPE) tj = (int)qj.w;
         if (!ONETYPE) {
-          const float4 t = tab[rowi + (int)qj.w];
+          const float4 t = tab[rowi + tj];
           ff1 = t.x; ff2 = t.y; rc2 = t.z; shift = t.w;
-          if (ENERGY) { const float2 et = etab[rowi + (int)qj.w]; sig2 = et.x; eps4 = et.y; }
+          if (ENERGY) { const float2 et = etab[rowi + tj]; sig2 = et.x; eps4 = et.y; }
         }
         // r^2 from the exact integer differences in FP64
         double ex = (double)ix, ey = (double)iy, ez = (double)iz;
@@ -791,7 +933,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
         const float r2 = (float)r2d;
         bool in = r2 <= rc2;
         if (__builtin_fabsf(r2 - rc2) <= 1e-5f * rc2 && rc2 > 0.f) {
-          if (!ONETYPE) drc2 = prm[rowi + (int)qj.w].drc2;
+          if (!ONETYPE) drc2 = prm[rowi + tj].drc2;
           const int j = slot_of_lds(sr, nr, e);
           const double4 pi = pos[i], pj = pos[j];
           double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
@@ -1106,6 +1248,79 @@ __global__ __launch_bounds__(256) void k_integrate1(
   }
 }
 
+// ------------------------------------------------------------------ fused step
+//
+// Thermostat and integrate2 of step s-1 followed by integrate1 of step s on the
+// same forces (VelocityVerlet::run, src/integrator/VelocityVerlet.cpp:121-165:
+// aftCalcF -> integrate2 -> befIntP -> integrate1), one pass over the particles.
+// Writes the block's maximum |dx|^2; k_finish_max reduces them.
+__global__ __launch_bounds__(256) void k_step_fused(
+    int n, const int* __restrict__ id, double4* __restrict__ pos, double4* __restrict__ vel,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz, int lang, double pref1,
+    double pref2, unsigned long long seed, unsigned long long call, const unsigned char* __restrict__ excl,
+    int nid, int do_int2, double dt, uint4* __restrict__ posq, Geom g, double* __restrict__ blockmax) {
+#pragma clang fp contract(off)
+  __shared__ double red[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double sq = 0.0;
+  if (i < n) {
+    double4 v = vel[i];
+    double f0 = fx[i], f1 = fy[i], f2 = fz[i];
+    if (lang) {
+      const int pid = id[i];
+      const bool skip = excl && pid < nid && excl[pid];
+      if (!skip) {
+        uint32_t o[4];
+        philox((uint32_t)pid, (uint32_t)call, (uint32_t)(call >> 32), 0x4C414E47u, (uint32_t)seed,
+               (uint32_t)(seed >> 32), o);
+        const double u0 = ((double)o[0] + 0.5) * 2.3283064365386963e-10;
+        const double u1 = ((double)o[1] + 0.5) * 2.3283064365386963e-10;
+        const double u2 = ((double)o[2] + 0.5) * 2.3283064365386963e-10;
+        const double m = v.w, massf = sqrt(m);
+        f0 += (pref1 * v.x) * m + (pref2 * (u0 - 0.5)) * massf;
+        f1 += (pref1 * v.y) * m + (pref2 * (u1 - 0.5)) * massf;
+        f2 += (pref1 * v.z) * m + (pref2 * (u2 - 0.5)) * massf;
+        fx[i] = f0; fy[i] = f1; fz[i] = f2;
+      }
+    }
+    const double dtfm = 0.5 * dt / v.w;
+    if (do_int2) {
+      v.x += dtfm * f0; v.y += dtfm * f1; v.z += dtfm * f2;
+    }
+    v.x += dtfm * f0; v.y += dtfm * f1; v.z += dtfm * f2;
+    double4 p = pos[i];
+    const double d0 = v.x * dt, d1 = v.y * dt, d2 = v.z * dt;
+    p.x += d0; p.y += d1; p.z += d2;
+    sq = d0 * d0 + d1 * d1 + d2 * d2;
+    vel[i] = v;
+    pos[i] = p;
+    if (posq)
+      posq[i] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]), (uint32_t)(int)p.w);
+  }
+  sq = wave_max(sq);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+    blockmax[blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_finish_max(int nblk, const double* __restrict__ blockmax, Ctl* ctl) {
+  __shared__ double red[16];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) m = fmax(m, blockmax[b]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t = fmax(t, red[w]);
+    ctl->maxsq_bits = (unsigned long long)__double_as_longlong(t);
+  }
+}
+
 // Kinetic sums per block: [sum m v^2, count, m vx vx, m vy vy, m vz vz, m vx vy,
 // m vx vz, m vy vz] over particles whose type bit is set in type_mask
 // (src/analysis/Temperature.hpp:100-122, src/analysis/PressureTensor.hpp:51-82).
@@ -1159,6 +1374,27 @@ void launch_fold_cell(hipStream_t s, int n, double4* pos, int4* img, int* cell_o
 
 void launch_scan_cells(hipStream_t s, int ncell, const int* count, int* start) {
   hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncell, count, start);
+}
+
+
+void launch_scan_multi(hipStream_t s, int ncell, const int* count, int* start, int* blocksum, int* blockoff) {
+  const int nb = (ncell + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(1024), 0, s, ncell, count, blocksum);
+  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, nb, blocksum, blockoff);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(1024), 0, s, ncell, count, blockoff, start);
+}
+
+int scan_blocks(int ncell) { return (ncell + kScanChunk - 1) / kScanChunk; }
+
+void launch_step_fused(hipStream_t s, int n, const int* id, double4* pos, double4* vel, double* fx, double* fy,
+                       double* fz, int lang, double pref1, double pref2, unsigned long long seed,
+                       unsigned long long call, const unsigned char* excl, int nid, int do_int2, double dt,
+                       uint4* posq, const Geom& g, double* blockmax, Ctl* ctl) {
+  if (n <= 0) return;
+  const int nb = nblocks(n, 256);
+  hipLaunchKernelGGL(k_step_fused, dim3(nb), dim3(256), 0, s, n, id, pos, vel, fx, fy, fz, lang, pref1, pref2, seed,
+                     call, excl, nid, do_int2, dt, posq, g, blockmax);
+  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, blockmax, ctl);
 }
 
 void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start, int* order) {
@@ -1241,10 +1477,10 @@ void launch_nbr_build_tiled(hipStream_t s, int n, int T, int lds_bytes, const ui
   if (n <= 0) return;
   dim3 grid(nblocks(n, T)), block(T);
   if (ex_start)
-    hipLaunchKernelGGL(k_nbr_build_tiled<true>, grid, block, lds_bytes, s, n, posq, pos, id, cell_of_slot, cstart, g,
+    hipLaunchKernelGGL(k_nbr_build_tiled<true>, grid, block, 0, s, n, posq, pos, id, cell_of_slot, cstart, g,
                        cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capw, ctl);
   else
-    hipLaunchKernelGGL(k_nbr_build_tiled<false>, grid, block, lds_bytes, s, n, posq, pos, id, cell_of_slot, cstart, g,
+    hipLaunchKernelGGL(k_nbr_build_tiled<false>, grid, block, 0, s, n, posq, pos, id, cell_of_slot, cstart, g,
                        cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capw, ctl);
 }
 

@@ -133,6 +133,8 @@ struct espp_sim {
   DBuf<int4> img, img2;
   DBuf<uint4> posq;
   DBuf<int> cell_of, rank, order, cell_of_slot, cell_count, cstart;
+  DBuf<int> scan_sum, scan_off;   // per-chunk totals and offsets of the cell scan
+  DBuf<double> blockmax;          // per-block max |dx|^2 of the fused step
   DBuf<int> slot_of_id;
   int nid = 0;
 
@@ -270,7 +272,7 @@ struct espp_sim {
       Geom gg = g;
       launch_fold_cell(stream, ni, pos.p, img.p, cell_of.p, rank.p, cell_count.p, gg, ctl.p, do_fold ? 1 : 0);
     }
-    launch_scan_cells(stream, (int)ncell(), cell_count.p, cstart.p);
+    launch_scan_multi(stream, (int)ncell(), cell_count.p, cstart.p, scan_sum.p, scan_off.p);
     if (ni > 0) {
       launch_scatter(stream, ni, cell_of.p, rank.p, cstart.p, order.p);
       launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, id.p);
@@ -355,7 +357,9 @@ struct espp_sim {
     // FP32 prefilter window: FP32 rounding plus fixed-point quantisation of both positions
     double qmax = std::max(g.L[0], std::max(g.L[1], g.L[2])) / 4294967296.0;
     double win = 1e-5 + 8.0 * qmax / std::max(cutVerlet, 1e-30);
-    const float cut_lo = (float)(cutsq * (1.0 - win)), cut_hi = (float)(cutsq * (1.0 + win));
+    // the tiled build compares in fixed-point units of the x scale
+    const double q2 = g.qsd[0] * g.qsd[0];
+    const float cut_lo_q = (float)(cutsq * (1.0 - win) / q2), cut_hi_q = (float)(cutsq * (1.0 + win) / q2);
     for (int attempt = 0; attempt < 4; ++attempt) {
       launch_reset_ctl(stream, ctl.p);
       hipEvent_t pe = prof_begin();
@@ -363,7 +367,7 @@ struct espp_sim {
         const int capw = v.cap / 2;
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)capw * 64);
         launch_nbr_build_tiled(stream, (int)n, T, max_region * 16, posq.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
-                               cutsq, cut_lo, cut_hi, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
+                               cutsq, cut_lo_q, cut_hi_q, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
                                v.has_excl ? v.ex_ids.p : nullptr, (uint32_t*)v.nbr.p, v.cnt.p, capw, ctl.p);
       } else {
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
@@ -457,7 +461,8 @@ struct espp_sim {
       bool onetype = a.dtab_nt == 1;
       hipEvent_t pe = energy ? nullptr : prof_begin();
       if (v.tiled)
-        launch_lj_tiled(stream, mixed(), onetype, energy, accum, (int)n, T, max_region * (mixed() ? 16 : 32), posq.p,
+        launch_lj_tiled(stream, mixed(), onetype, energy, accum, (int)n, T,
+                        max_region * (mixed() ? (onetype ? 12 : 16) : 32), posq.p,
                         pos.p, (const uint32_t*)v.nbr.p, v.cnt.p, v.cap / 2, g, runs.p, nruns.p, a.dtab.p, a.dtab_nt,
                         fx.p, fy.p, fz.p, partial_out);
       else
@@ -536,15 +541,20 @@ struct espp_sim {
       launch_thermo_integrate2(stream, (int)n, id.p, vel.p, fx.p, fy.p, fz.p, 1, pref1, pref2 * std::sqrt(3.0), seed,
                                therm_calls++, excl_flag.p, excl_nid, 0, 0.5 * dt);
     }
+    blockmax.resize((size_t)std::max(kinetic_blocks((int)n), 1));
     for (int i = 0; i < nsteps; ++i) {
       auto ta = clk::now();
-      launch_reset_ctl(stream, ctl.p);
+      // thermostat + integrate2 of the previous step fused with integrate1 of this one
+      const bool first = (i == 0);
       hipEvent_t pe1 = prof_begin();
-      launch_integrate1(stream, (int)n, pos.p, vel.p, fx.p, fy.p, fz.p, dt, mixed() ? posq.p : nullptr, g, ctl.p);
+      launch_step_fused(stream, (int)n, id.p, pos.p, vel.p, fx.p, fy.p, fz.p, (!first && lang) ? 1 : 0, pref1, pref2,
+                        seed, (!first && lang) ? therm_calls++ : 0ull, excl_flag.p, excl_nid, first ? 0 : 1, dt,
+                        mixed() ? posq.p : nullptr, g, blockmax.p, ctl.p);
       prof_end(2, pe1);
       read_ctl();
       double maxSq;
       std::memcpy(&maxSq, &hctl->maxsq_bits, sizeof(double));
+      if (n == 0) maxSq = 0.0;
       maxDist += std::sqrt(maxSq);
       auto tb = clk::now();
       t_int1 += std::chrono::duration<double>(tb - ta).count();
@@ -558,13 +568,16 @@ struct espp_sim {
       auto tc = clk::now();
       t_resort += std::chrono::duration<double>(tc - tb).count();
       calc_forces();
+      t_force += std::chrono::duration<double>(clk::now() - tc).count();
+      step++;
+    }
+    if (nsteps > 0) {
+      // thermostat + integrate2 of the last step
       auto td = clk::now();
-      t_force += std::chrono::duration<double>(td - tc).count();
       hipEvent_t pe2 = prof_begin();
       launch_thermo_integrate2(stream, (int)n, id.p, vel.p, fx.p, fy.p, fz.p, lang ? 1 : 0, pref1, pref2, seed,
                                lang ? therm_calls++ : 0ull, excl_flag.p, excl_nid, 1, 0.5 * dt);
       prof_end(3, pe2);
-      step++;
       t_int2 += std::chrono::duration<double>(clk::now() - td).count();
     }
     sync();
@@ -699,6 +712,8 @@ int espp_create(const double box[3], const int32_t cell_grid[3], double skin, ui
       need(nc < (int64_t)1 << 30, "cell grid too large");
       s->cell_count.resize((size_t)nc);
       s->cstart.resize((size_t)nc + 1);
+      s->scan_sum.resize((size_t)scan_blocks((int)nc) + 1);
+      s->scan_off.resize((size_t)scan_blocks((int)nc) + 1);
       s->reserve_particles(1024);
       s->ensure_id_map();
     } catch (...) {
