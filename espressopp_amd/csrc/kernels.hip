@@ -754,13 +754,15 @@ __device__ __forceinline__ int stage_tile_q(const TileRun* __restrict__ runs, co
   return nr;
 }
 
-// Tiled Verlet-list build.  Candidates of the 27 neighbour cells are read from
-// the fixed-point mirror (lanes of one cell read the same candidates, so the
-// loads broadcast); pairs whose FP32 r^2 is within a small window of
-// (rc+skin)^2 are settled by the reference's FP64 test (same orientation rule
-// as k_nbr_build), all others by the FP32 value.  Entries are written as the
-// candidate's offset in the tile's staged region, the layout the tiled force
-// kernels read.
+// Tiled Verlet-list build (cell grids with at least 3 cells per dimension).
+// For each of its 9 neighbour x-rows a particle scans the candidates of the
+// three cells (cx-1, cx, cx+1), one contiguous slot range or two when the row
+// wraps, read from the fixed-point mirror (lanes of one cell read the same
+// candidates, so the loads broadcast).  Acceptance bits of 32 candidates are
+// collected first and appended afterwards.  Pairs whose FP32 r^2 is within a
+// small window of (rc+skin)^2 are settled by the reference's FP64 test in the
+// reference's orientation (see k_nbr_build); all others by the FP32 value.
+// Entries are the candidate's offset in the tile's staged region.
 template <bool HAS_EXCL>
 __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     int n, const uint4* __restrict__ posq, const double4* __restrict__ pos, const int* __restrict__ id,
@@ -784,63 +786,69 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     uint32_t* out = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
     uint32_t pending = 0;
     const int cap = 2 * capw;
-    const float3 qsr = make_float3(1.0f, (float)(g.qsd[1] / g.qsd[0]), (float)(g.qsd[2] / g.qsd[0]));
+    const float ry = (float)(g.qsd[1] / g.qsd[0]), rz = (float)(g.qsd[2] / g.qsd[0]);
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nz; sz = -1; } else if (qz >= nz) { qz -= nz; sz = 1; }
       for (int oy = -1; oy <= 1; ++oy) {
         int qy = cy + oy, sy = 0;
         if (qy < 0) { qy += ny; sy = -1; } else if (qy >= ny) { qy -= ny; sy = 1; }
-        // the three cells of this x-row: issue their cstart loads together
-        int c2s[3], sxs[3], jbs[3], jes[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          int qx = cx + a - 1, sx = 0;
-          if (qx < 0) { qx += nx; sx = -1; } else if (qx >= nx) { qx -= nx; sx = 1; }
-          c2s[a] = qx + nx * (qy + ny * qz);
-          sxs[a] = sx;
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) { jbs[a] = cstart[c2s[a]]; jes[a] = cstart[c2s[a] + 1]; }
-        for (int a = 0; a < 3; ++a) {
-          const int ox = a - 1, sx = sxs[a], c2 = c2s[a];
-          const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
-          const bool unshifted = (sx | sy | sz) == 0;
-          const TileRun R = sr[find_run_by_cell(sr, nr, c2)];
-          const int jb = jbs[a], je = jes[a];
+        const int row = (qy + ny * qz) * nx;
+        // x pieces of the row segment (cx-1 .. cx+1), split where the row wraps
+        int pa0, pb0, pa1 = 0, pb1 = -1;
+        if (cx >= 1 && cx + 1 < nx) { pa0 = cx - 1; pb0 = cx + 1; }
+        else if (cx == 0) { pa0 = nx - 1; pb0 = nx - 1; pa1 = 0; pb1 = 1; }
+        else { pa0 = cx - 1; pb0 = nx - 1; pa1 = 0; pb1 = 0; }
+        for (int p = 0; p < 2; ++p) {
+          const int xa = p ? pa1 : pa0, xb = p ? pb1 : pb0;
+          if (xb < xa) break;
+          const int ca = row + xa;
+          const int jb = cstart[ca], je = cstart[row + xb + 1];
+          const TileRun R = sr[find_run_by_cell(sr, nr, ca)];
           const int base = R.lds_off + (jb - R.slot_start);
-          for (int j0 = jb; j0 < je; j0 += 4) {
-            uint4 qv[4];
+          for (int j0 = jb; j0 < je; j0 += 32) {
+            const int jn = min(32, je - j0);
+            uint32_t bits = 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) qv[u] = (j0 + u < je) ? posq[j0 + u] : qi;
+            for (int u = 0; u < 32; u += 4) {
+              uint4 q[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const int j = j0 + u;
-              if (j >= je) break;
-              if (j == i && unshifted) continue;
-              const int e = base + (j - jb);
-              const uint4 qj = qv[u];
-              // prefilter in fixed-point units: thresholds are pre-divided by the scale^2
-              const float dx = (float)(int)(qi.x - qj.x) * qsr.x;
-              const float dy = (float)(int)(qi.y - qj.y) * qsr.y;
-              const float dz = (float)(int)(qi.z - qj.z) * qsr.z;
-              const float r2 = dx * dx + dy * dy + dz * dz;
-              if (r2 > cut_hi) continue;
-              bool in = r2 < cut_lo;
-              if (!in) {
-                const double4 pi = pos[i], pj = pos[j];
-                const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
-                double ex, ey, ez;
-                if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
-                else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
-                in = (ex * ex + ey * ey + ez * ez) <= cutsq;
+              for (int v = 0; v < 4; ++v) q[v] = posq[j0 + min(u + v, jn - 1)];
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int t = u + v;
+                const float dx = (float)(int)(qi.x - q[v].x);
+                const float dy = (float)(int)(qi.y - q[v].y) * ry;
+                const float dz = (float)(int)(qi.z - q[v].z) * rz;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                bool acc = (t < jn) && (j0 + t != i) && (r2 <= cut_hi);
+                if (acc && r2 >= cut_lo) {
+                  // borderline: the reference's FP64 test in the reference's orientation
+                  const int j = j0 + t;
+                  const int cjx = cell_of_slot[j] % nx;
+                  int ox = cjx - cx, sx = 0;
+                  if (ox > 1) { ox = -1; sx = -1; } else if (ox < -1) { ox = 1; sx = 1; }
+                  const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
+                  const double4 pi = pos[i], pj = pos[j];
+                  const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
+                  double ex, ey, ez;
+                  if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
+                  else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
+                  acc = (ex * ex + ey * ey + ez * ez) <= cutsq;
+                }
+                bits |= (acc ? 1u : 0u) << t;
               }
-              if (!in) continue;
+            }
+            while (bits) {
+              const int t = __ffs(bits) - 1;
+              bits &= bits - 1u;
+              const int j = j0 + t;
               if (HAS_EXCL && is_excluded(myid, id[j], ex_start, ex_ids)) continue;
+              const uint32_t e = (uint32_t)(base + (j - jb));
               if (k & 1) {
-                if (k < cap) out[(size_t)(k >> 1) * 64] = pending | ((uint32_t)e << 16);
+                if (k < cap) out[(size_t)(k >> 1) * 64] = pending | (e << 16);
               } else {
-                pending = (uint32_t)e;
+                pending = e;
               }
               ++k;
             }

@@ -303,6 +303,8 @@ struct espp_sim {
     max_region = 0;
     plan_valid = true;
     if (n == 0) return;
+    // the tiled list build assumes at least 3 cells per dimension
+    if (g.n[0] < 3 || g.n[1] < 3 || g.n[2] < 3) return;
     const int elem = mixed() ? 16 : 32;
     const int budget = (lds_cap - 12288) / elem;
     const int sizes_mixed[3] = {256, 128, 64}, sizes_fp64[2] = {128, 64};
