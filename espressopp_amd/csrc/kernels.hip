@@ -54,20 +54,20 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 
-// Mixed-precision LJ force factor for one pair.  r^2 arrives in FP64 from exact
-// fixed-point differences; the reciprocal, its powers and the two LJ terms are
-// FP32; one FP64 residual, 1 - r2d * inv, corrects to first order both the
-// rounding of r^2 to FP32 and the error of v_rcp_f32:
-//   ff(u) = ff1 u^7 - ff2 u^4,  u = inv (1 + eps)  =>  ff += eps (7 a - 4 b).
-__device__ __forceinline__ float lj_ff_mixed(double r2d, float ff1, float ff2, float* inv_out) {
-  const float r2 = (float)r2d;
-  const float inv = __builtin_amdgcn_rcpf(r2);
-  const float r6 = inv * inv * inv;
-  const float b = ff2 * r6 * inv;
-  const float a = ff1 * r6 * r6 * inv;
-  const float eps = (float)__builtin_fma(-r2d, (double)inv, 1.0);
-  *inv_out = inv;
-  return (a - b) + eps * (7.0f * a - 4.0f * b);
+template <bool CUBIC>
+__device__ __forceinline__ double pair_r2(const uint4& qi, int ix, int iy, int iz, const Geom& g, double& ex,
+                                          double& ey, double& ez) {
+  ex = (double)ix; ey = (double)iy; ez = (double)iz;
+  if (!CUBIC) { ex *= g.qsd[0]; ey *= g.qsd[1]; ez *= g.qsd[2]; }
+  return ex * ex + ey * ey + ez * ez;
+}
+
+__device__ __forceinline__ double lj_ff_seeded(double r2, double ff1, double ff2, double& inv) {
+  double u = (double)__builtin_amdgcn_rcpf((float)r2);
+  u = u * __builtin_fma(-r2, u, 2.0);          // one Newton step: ~1e-14 relative
+  inv = u;
+  const double r6 = u * u * u;
+  return r6 * __builtin_fma(ff1, r6, -ff2) * u;
 }
 
 // ------------------------------------------------------------------ binning
@@ -442,16 +442,6 @@ __global__ __launch_bounds__(256) void k_lj_mixed(
     const int* __restrict__ nbr, const int* __restrict__ cnt, int cap, Geom g,
     const LJParam* __restrict__ prm, int ntypes, double* __restrict__ fx,
     double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ partial) {
-  __shared__ float4 tab[kMaxTypes * kMaxTypes];
-  __shared__ float2 etab[kMaxTypes * kMaxTypes];
-  if (!ONETYPE) {
-    for (int t = threadIdx.x; t < ntypes * ntypes; t += blockDim.x) {
-      const LJParam p = prm[t];
-      tab[t] = make_float4(p.ff1, p.ff2, p.set ? p.rc2 : -1.0f, p.shift);
-      etab[t] = make_float2((float)p.sig2, (float)p.eps4);
-    }
-    __syncthreads();
-  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[kNPart];
 #pragma unroll
@@ -460,49 +450,37 @@ __global__ __launch_bounds__(256) void k_lj_mixed(
     const uint4 qi = posq[i];
     const int* np = nbr + (size_t)(i >> 6) * (size_t)cap * 64 + (i & 63);
     const int c = min(cnt[i], cap);
-    float ff1 = prm[0].ff1, ff2 = prm[0].ff2, rc2 = prm[0].rc2, shift = prm[0].shift;
-    float sig2 = (float)prm[0].sig2, eps4 = (float)prm[0].eps4;
-    double drc2 = prm[0].drc2;
     const int rowi = (int)qi.w * ntypes;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     for (int k = 0; k < c; ++k) {
       const int j = np[(size_t)k * 64];
       const uint4 qj = posq[j];
-      if (!ONETYPE) {
-        const float4 t = tab[rowi + (int)qj.w];
-        ff1 = t.x; ff2 = t.y; rc2 = t.z; shift = t.w;
-        if (ENERGY) { const float2 et = etab[rowi + (int)qj.w]; sig2 = et.x; eps4 = et.y; }
-      }
-      const double ex = (double)(int)(qi.x - qj.x) * g.qsd[0];
-      const double ey = (double)(int)(qi.y - qj.y) * g.qsd[1];
-      const double ez = (double)(int)(qi.z - qj.z) * g.qsd[2];
-      const double r2d = ex * ex + ey * ey + ez * ez;
-      const float r2 = (float)r2d;
-      bool in = r2 <= rc2;
-      if (__builtin_fabsf(r2 - rc2) <= 1e-5f * rc2 && rc2 > 0.f) {
-        if (!ONETYPE) drc2 = prm[rowi + (int)qj.w].drc2;
+      const LJParam& p = prm[ONETYPE ? 0 : rowi + (int)qj.w];
+      double ex, ey, ez;
+      const double r2 = pair_r2<false>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
+      bool in = p.set && r2 <= p.drc2;
+      if (p.set && r2 >= p.dlo && r2 <= p.dhi) {
+        // fixed-point quantisation could flip the cutoff test: decide from FP64 positions
         const double4 pi = pos[i], pj = pos[j];
         double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
         hx = __builtin_fma(-g.L[0], __builtin_rint(hx * g.invL[0]), hx);
         hy = __builtin_fma(-g.L[1], __builtin_rint(hy * g.invL[1]), hy);
         hz = __builtin_fma(-g.L[2], __builtin_rint(hz * g.invL[2]), hz);
-        in = (hx * hx + hy * hy + hz * hz) <= drc2;
+        in = (hx * hx + hy * hy + hz * hz) <= p.drc2;
       }
-      float inv;
-      float ff = lj_ff_mixed(r2d, ff1, ff2, &inv);
-      ff = in ? ff : 0.0f;
-      const double fd = (double)ff;
-      Ax = __builtin_fma(ex, fd, Ax);
-      Ay = __builtin_fma(ey, fd, Ay);
-      Az = __builtin_fma(ez, fd, Az);
-      if (ENERGY) {
-        const float f2 = sig2 * inv;
-        const float f6 = f2 * f2 * f2;
-        const float en = eps4 * (f6 * f6 - f6) - shift;
-        acc[0] += in ? (double)en : 0.0;
-        acc[1] += r2d * fd;
-        acc[2] += ex * ex * fd; acc[3] += ey * ey * fd; acc[4] += ez * ez * fd;
-        acc[5] += ex * ey * fd; acc[6] += ex * ez * fd; acc[7] += ey * ez * fd;
+      double inv;
+      double ff = lj_ff_seeded(r2, p.dff1, p.dff2, inv);
+      ff = in ? ff : 0.0;
+      Ax = __builtin_fma(ex, ff, Ax);
+      Ay = __builtin_fma(ey, ff, Ay);
+      Az = __builtin_fma(ez, ff, Az);
+      if (ENERGY && in) {
+        const double f2 = p.sig2 * inv;
+        const double f6 = f2 * f2 * f2;
+        acc[0] += p.eps4 * (f6 * f6 - f6) - p.dshift;
+        acc[1] += r2 * ff;
+        acc[2] += ex * ex * ff; acc[3] += ey * ey * ff; acc[4] += ez * ez * ff;
+        acc[5] += ex * ey * ff; acc[6] += ex * ez * ff; acc[7] += ey * ez * ff;
       }
     }
     if (!ENERGY) {
@@ -868,8 +846,16 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
   if ((threadIdx.x & 63) == 0) atomicMax(&ctl->maxcount, m);
 }
 
-// Mixed-precision LJ over the tiled list: the same arithmetic as k_lj_mixed
-// with the neighbour positions read from LDS.
+// Mixed-precision LJ over the tiled list.  Neighbour positions come from the
+// 32-bit fixed-point mirror staged in LDS; their differences are exact
+// integers.  r^2 is formed exactly in FP64 from them, the reciprocal is seeded
+// by v_rcp_f32 and refined by one FP64 Newton step, and the LJ factor and the
+// force sum are FP64 (on CDNA3/4 FP64 FMA issues at the FP32 rate, only the
+// reciprocal is cheaper in FP32).  For cubic boxes everything stays in
+// fixed-point units (LJParam.u*) and the force is scaled once at the end.
+// Pairs within a tiny window of the cutoff (fixed-point quantisation could flip
+// the test) are left out of the main loop and settled from the FP64 positions
+// afterwards, so the interacting set equals the FP64 path's.
 template <bool ONETYPE, bool ENERGY, bool ACCUM, bool CUBIC>
 __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     int n, const uint4* __restrict__ posq, const double4* __restrict__ pos,
@@ -882,15 +868,6 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
   extern __shared__ uint4 lds_raw[];
   Slot* lds = reinterpret_cast<Slot*>(lds_raw);
   __shared__ TileRun sr[kMaxRuns];
-  __shared__ float4 tab[ONETYPE ? 1 : kMaxTypes * kMaxTypes];
-  __shared__ float2 etab[ONETYPE ? 1 : kMaxTypes * kMaxTypes];
-  if (!ONETYPE) {
-    for (int t = threadIdx.x; t < ntypes * ntypes; t += blockDim.x) {
-      const LJParam p = prm[t];
-      tab[t] = make_float4(p.ff1, p.ff2, p.set ? p.rc2 : -1.0f, p.shift);
-      etab[t] = make_float2((float)p.sig2, (float)p.eps4);
-    }
-  }
   const int nr = stage_tile_q<Slot>(runs, nruns, posq, lds, sr);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[kNPart];
@@ -900,10 +877,10 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     const uint4 qi = posq[i];
     const uint32_t* np = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
     const int c = nr > 0 ? min(cnt[i], 2 * capw) : 0;
-    float ff1 = prm[0].ff1, ff2 = prm[0].ff2, rc2 = prm[0].rc2, shift = prm[0].shift;
-    float sig2 = (float)prm[0].sig2, eps4 = (float)prm[0].eps4;
-    double drc2 = prm[0].drc2;
     const int rowi = (int)qi.w * ntypes;
+    const LJParam& p0 = prm[0];
+    double ff1 = CUBIC ? p0.uff1 : p0.dff1, ff2 = CUBIC ? p0.uff2 : p0.dff2;
+    double rc2 = CUBIC ? p0.urc2 : p0.drc2, lo = CUBIC ? p0.ulo : p0.dlo, hi = CUBIC ? p0.uhi : p0.dhi;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     const int nw = (c + 1) >> 1;
     uint32_t wn[4];
@@ -915,59 +892,83 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
       for (int u = 0; u < 4; ++u) w[u] = wn[u];
 #pragma unroll
       for (int u = 0; u < 4; ++u) wn[u] = (kw + 4 + u < nw) ? np[(size_t)(kw + 4 + u) * 64] : 0u;
+      uint32_t nearbits = 0;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int k = 2 * kw + u;
-        if (k >= c) break;
+        const bool valid = k < c;
         const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
         const Slot qj = lds[e];
-        const int ix = (int)(qi.x - qj.x), iy = (int)(qi.y - qj.y), iz = (int)(qi.z - qj.z);
+        if (!ONETYPE) {
+          int tj = 0;
+          if constexpr (!ONETYPE) tj = (int)qj.w;
+          const LJParam& p = prm[rowi + tj];
+          ff1 = CUBIC ? p.uff1 : p.dff1; ff2 = CUBIC ? p.uff2 : p.dff2;
+          rc2 = p.set ? (CUBIC ? p.urc2 : p.drc2) : -1.0;
+          lo = CUBIC ? p.ulo : p.dlo; hi = CUBIC ? p.uhi : p.dhi;
+        }
+        double ex, ey, ez;
+        const double r2 = pair_r2<CUBIC>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
+        const bool near = valid && r2 >= lo && r2 <= hi;
+        const bool in = valid && !near && r2 <= rc2;
+        nearbits |= (near ? 1u : 0u) << u;
+        double inv;
+        double ff = lj_ff_seeded(r2, ff1, ff2, inv);
+        ff = in ? ff : 0.0;
+        Ax = __builtin_fma(ex, ff, Ax);
+        Ay = __builtin_fma(ey, ff, Ay);
+        Az = __builtin_fma(ez, ff, Az);
+        if (ENERGY && in) {
+          int tj = 0;
+          if constexpr (!ONETYPE) tj = (int)qj.w;
+          const LJParam& p = prm[ONETYPE ? 0 : rowi + tj];
+          const double f2 = (CUBIC ? p.usig2 : p.sig2) * inv;
+          const double f6 = f2 * f2 * f2;
+          acc[0] += p.eps4 * (f6 * f6 - f6) - p.dshift;
+          acc[1] += r2 * ff;
+          acc[2] += ex * ex * ff; acc[3] += ey * ey * ff; acc[4] += ez * ez * ff;
+          acc[5] += ex * ey * ff; acc[6] += ex * ez * ff; acc[7] += ey * ez * ff;
+        }
+      }
+      // pairs at the cutoff: decide from the FP64 positions (rare)
+      while (nearbits) {
+        const int u = __ffs(nearbits) - 1;
+        nearbits &= nearbits - 1u;
+        const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
+        const Slot qj = lds[e];
         int tj = 0;
         if constexpr (!ONETYPE) tj = (int)qj.w;
-        if (!ONETYPE) {
-          const float4 t = tab[rowi + tj];
-          ff1 = t.x; ff2 = t.y; rc2 = t.z; shift = t.w;
-          if (ENERGY) { const float2 et = etab[rowi + tj]; sig2 = et.x; eps4 = et.y; }
-        }
-        // r^2 from the exact integer differences in FP64
-        double ex = (double)ix, ey = (double)iy, ez = (double)iz;
-        double r2d;
-        if (CUBIC) {
-          r2d = (ex * ex + ey * ey + ez * ez) * g.qs2;           // ex.. stay in fixed-point units
-        } else {
-          ex *= g.qsd[0]; ey *= g.qsd[1]; ez *= g.qsd[2];        // length units
-          r2d = ex * ex + ey * ey + ez * ez;
-        }
-        const float r2 = (float)r2d;
-        bool in = r2 <= rc2;
-        if (__builtin_fabsf(r2 - rc2) <= 1e-5f * rc2 && rc2 > 0.f) {
-          if (!ONETYPE) drc2 = prm[rowi + tj].drc2;
-          const int j = slot_of_lds(sr, nr, e);
-          const double4 pi = pos[i], pj = pos[j];
-          double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
-          hx = __builtin_fma(-g.L[0], __builtin_rint(hx * g.invL[0]), hx);
-          hy = __builtin_fma(-g.L[1], __builtin_rint(hy * g.invL[1]), hy);
-          hz = __builtin_fma(-g.L[2], __builtin_rint(hz * g.invL[2]), hz);
-          in = (hx * hx + hy * hy + hz * hz) <= drc2;
-        }
-        float inv;
-        float ff = lj_ff_mixed(r2d, ff1, ff2, &inv);
-        ff = in ? ff : 0.0f;
-        const double sff = CUBIC ? (double)ff * g.qsd[0] : (double)ff;   // F = d * ff
-        Ax = __builtin_fma(ex, sff, Ax);
-        Ay = __builtin_fma(ey, sff, Ay);
-        Az = __builtin_fma(ez, sff, Az);
+        const LJParam& p = prm[ONETYPE ? 0 : rowi + tj];
+        const int j = slot_of_lds(sr, nr, e);
+        const double4 pi = pos[i], pj = pos[j];
+        double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
+        hx = __builtin_fma(-g.L[0], __builtin_rint(hx * g.invL[0]), hx);
+        hy = __builtin_fma(-g.L[1], __builtin_rint(hy * g.invL[1]), hy);
+        hz = __builtin_fma(-g.L[2], __builtin_rint(hz * g.invL[2]), hz);
+        if (!(p.set && hx * hx + hy * hy + hz * hz <= p.drc2)) continue;
+        double ex, ey, ez;
+        const double r2 = pair_r2<CUBIC>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
+        double inv;
+        const double ff = lj_ff_seeded(r2, CUBIC ? p.uff1 : p.dff1, CUBIC ? p.uff2 : p.dff2, inv);
+        Ax = __builtin_fma(ex, ff, Ax);
+        Ay = __builtin_fma(ey, ff, Ay);
+        Az = __builtin_fma(ez, ff, Az);
         if (ENERGY) {
-          const float f2 = sig2 * inv;
-          const float f6 = f2 * f2 * f2;
-          const float en = eps4 * (f6 * f6 - f6) - shift;
-          acc[0] += in ? (double)en : 0.0;
-          const double wx = CUBIC ? ex * g.qsd[0] : ex, wy = CUBIC ? ey * g.qsd[0] : ey, wz = CUBIC ? ez * g.qsd[0] : ez;
-          const double fd = (double)ff;
-          acc[1] += r2d * fd;
-          acc[2] += wx * wx * fd; acc[3] += wy * wy * fd; acc[4] += wz * wz * fd;
-          acc[5] += wx * wy * fd; acc[6] += wx * wz * fd; acc[7] += wy * wz * fd;
+          const double f2 = (CUBIC ? p.usig2 : p.sig2) * inv;
+          const double f6 = f2 * f2 * f2;
+          acc[0] += p.eps4 * (f6 * f6 - f6) - p.dshift;
+          acc[1] += r2 * ff;
+          acc[2] += ex * ex * ff; acc[3] += ey * ey * ff; acc[4] += ez * ez * ff;
+          acc[5] += ex * ey * ff; acc[6] += ex * ez * ff; acc[7] += ey * ez * ff;
         }
+      }
+    }
+    if (CUBIC) {
+      // back from fixed-point units: F = d_units * s * ff_units; w = r2_units * s^2 * ff_units
+      Ax *= g.qsd[0]; Ay *= g.qsd[0]; Az *= g.qsd[0];
+      if (ENERGY) {
+#pragma unroll
+        for (int q = 1; q < kNPart; ++q) acc[q] *= g.qs2;
       }
     }
     if (!ENERGY) {

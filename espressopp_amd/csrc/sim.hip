@@ -448,8 +448,26 @@ struct espp_sim {
     if (!a.dirty && a.dtab_nt == nt) return;
     sync();
     std::vector<LJParam> t((size_t)nt * nt);
+    // fixed-point units: r^2 unit q2 = (L / 2^(32+k))^2; ff1 ~ u^7, ff2 ~ u^4, u = 1/r^2
+    const double q2 = g.qs2;
+    const double qmax = std::max(g.qsd[0], std::max(g.qsd[1], g.qsd[2]));
     for (int i = 0; i < nt; ++i)
-      for (int j = 0; j < nt; ++j) t[(size_t)i * nt + j] = a.tab[(size_t)i * kMaxTypes + j];
+      for (int j = 0; j < nt; ++j) {
+        LJParam p = a.tab[(size_t)i * kMaxTypes + j];
+        // window of r^2 where fixed-point quantisation (<= sqrt(3) LSB per difference) could
+        // flip the cutoff test
+        const double rc = std::sqrt(p.drc2);
+        const double win = std::isfinite(rc) ? 4.0 * std::sqrt(3.0) * qmax * rc + 1e-300 : 0.0;
+        p.dlo = p.drc2 - win;
+        p.dhi = p.drc2 + win;
+        p.uff1 = p.dff1 / std::pow(q2, 7);
+        p.uff2 = p.dff2 / std::pow(q2, 4);
+        p.urc2 = p.drc2 / q2;
+        p.ulo = p.dlo / q2;
+        p.uhi = p.dhi / q2;
+        p.usig2 = p.sig2 / q2;
+        t[(size_t)i * nt + j] = p;
+      }
     a.dtab.resize(t.size());
     hip_check(hipMemcpy(a.dtab.p, t.data(), t.size() * sizeof(LJParam), hipMemcpyHostToDevice), "lj table upload");
     a.dtab_nt = nt;
