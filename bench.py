@@ -112,6 +112,20 @@ def cpu_baseline(pos, vel, box, cells, rc, skin, steps, threads):
     return n * steps / dt, dt
 
 
+def calibrate_copy(mib):
+    """A plain elementwise copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel read
+    against its mib MiB in and out give the scale of those counters on this GPU."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,14 +134,22 @@ def main():
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--prof-steps", type=int, default=100,
+                    help="NVE steps after the timed run with per-kernel HIP events")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the short-range MD path here runs on a single MI355X "
               "(--gpus %d requested). Multi-GPU domain decomposition is not built." % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import espressopp_amd as espressopp
 
@@ -178,12 +200,15 @@ def main():
         del f32, f64
 
     e0 = total_energy(system, lj, n)
-    h.call("espp_prof_reset")
-    h.call("espp_prof_enable", 1)
     dt_run = timed_run(espressopp, system, integ, args.steps)
-    h.call("espp_prof_enable", 0)
     e1 = total_energy(system, lj, n)
     value = n * args.steps / dt_run
+    # per-kernel durations from HIP events on the library's stream, in a separate NVE run
+    # (events are kept out of the timed run)
+    h.call("espp_prof_reset")
+    h.call("espp_prof_enable", 1)
+    integ.run(args.prof_steps)
+    h.call("espp_prof_enable", 0)
     launches, pair_ms = kernel_stats(h, "pair")
     entries, cap, maxcount = vl_entries(h, vl)
     nbar = entries / n
@@ -203,9 +228,10 @@ def main():
 
     if not args.no_fp64:
         k64 = args.fp64_steps or args.steps
+        dt64 = timed_run(espressopp, sys64, integ64, k64)
         sys64.storage._h.call("espp_prof_reset")
         sys64.storage._h.call("espp_prof_enable", 1)
-        dt64 = timed_run(espressopp, sys64, integ64, k64)
+        integ64.run(min(args.prof_steps, k64))
         sys64.storage._h.call("espp_prof_enable", 0)
         l64, ms64 = kernel_stats(sys64.storage._h, "pair")
         fp64 = dict(value=n * k64 / dt64, ms_per_step=dt64 / k64 * 1e3, pair_kernel_ms=ms64 / max(l64, 1),
@@ -236,14 +262,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-pair/fp64-integrate",
+        "dtype": "int32-fixed/fp64",
         "data": "synthetic: jittered simple-cubic lattice, Maxwell-Boltzmann velocities, Langevin warm-up",
         "config": {"workload": "LJ fluid N=%d rho=0.8442 rc=2.5 skin=0.3 dt=0.005 NVE after %d-step "
                                "Langevin warm-up at T=0.72" % (n, args.warmup),
                    "n": n, "box": box[0], "cells": cells, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_lj_mixed", "bytes_per_particle": b_force, "nbar": nbar,
+                     "kernel": "k_lj_tiled_mixed", "bytes_per_particle": b_force, "nbar": nbar,
                      "avg_launch_ms": avg_s * 1e3, "launches": launches},
         "fp64_path": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,

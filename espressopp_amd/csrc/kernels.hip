@@ -1,13 +1,15 @@
 // HIP kernels of the MI355X short-range MD step (gfx950, wave64).
 //
 //   binning          k_fold_cell, k_scan_cells, k_scatter, k_sort_cells, k_permute
-//   neighbour build  k_nbr_build   (full list, wavefront-padded ragged SoA)
+//   neighbour build  k_nbr_build / k_nbr_build_tiled (full list, wavefront-padded
+//                                   ragged SoA; 16-bit LDS offsets when tiled)
 //   pair forces      k_lj_tiled_mixed / k_lj_mixed (32-bit fixed-point positions,
-//                                   FP32 reciprocal and LJ terms with an FP64
-//                                   residual correction, FP64 force accumulation)
+//                                   exact FP64 r^2, FP32-seeded reciprocal,
+//                                   FP64 LJ factor and force accumulation)
 //                    k_lj_fp64     (all-FP64 comparison path)
 //   bonds            k_fene        (per-particle bond adjacency, no atomics)
-//   integrator       k_integrate1, k_thermo_integrate2
+//   integrator       k_step_fused (thermostat + integrate2 + integrate1),
+//                    k_integrate1, k_thermo_integrate2, k_finish_max
 //   reductions       k_kinetic, k_reduce_partials
 //
 // Launch wrappers at the bottom are the only entry points used by the host
@@ -429,12 +431,13 @@ __device__ void block_reduce_store(double (&v)[NV], double* __restrict__ partial
   }
 }
 
-// Mixed-precision LJ over the full list.  Pair distances come from exact
-// integer differences of the 32-bit fixed-point mirror (minimum image for free,
-// one rounding when converted to FP32), pair math is FP32, and each particle's
-// force is summed in FP32 chunks of kChunk neighbours carried into FP64.  Pairs
-// whose FP32 r^2 lies within 1e-5 relative of rc^2 re-decide the cutoff test in
-// FP64 so the set of interacting pairs equals the FP64 path's.
+// Mixed-precision LJ over the full list (grids with fewer than 3 cells in a
+// dimension).  Pair distances come from exact integer differences of the 32-bit
+// fixed-point mirror (minimum image for free), r^2 is formed in FP64, the
+// reciprocal is seeded by v_rcp_f32 and refined by one FP64 Newton step, and the
+// force sum is FP64.  Pairs whose r^2 lies in a small window around rc^2
+// re-decide the cutoff test from the FP64 positions so the set of interacting
+// pairs equals the FP64 path's.
 // Potential: src/interaction/LennardJones.hpp:129-136 (force), :109-114 (energy).
 template <bool ONETYPE, bool ENERGY, bool ACCUM>
 __global__ __launch_bounds__(256) void k_lj_mixed(

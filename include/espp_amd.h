@@ -33,8 +33,10 @@ typedef struct espp_sim espp_sim;
 
 /* precision of the pair path */
 #define ESPP_FP64 0                /* all-FP64 pair forces */
-#define ESPP_MIXED 1               /* FP32 pair math on 32-bit fixed-point positions,
-                                      FP64 integration, list build and reductions */
+#define ESPP_MIXED 1               /* pair forces from 32-bit fixed-point positions:
+                                      exact FP64 r^2, FP32-seeded reciprocal, FP64
+                                      LJ factor and accumulation; FP64 integration,
+                                      list test and reductions */
 
 /* particle fields for espp_get_particles / espp_set_particles */
 #define ESPP_FIELD_POS 0

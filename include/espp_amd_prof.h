@@ -19,8 +19,9 @@ extern "C" {
 /* kernel ids for espp_prof_stats */
 #define ESPP_K_PAIR 0        /* Lennard-Jones pair-force kernel */
 #define ESPP_K_NBR_BUILD 1   /* Verlet-list build */
-#define ESPP_K_INTEGRATE1 2  /* half kick + drift + max displacement */
-#define ESPP_K_INTEGRATE2 3  /* thermostat + half kick */
+#define ESPP_K_INTEGRATE1 2  /* per step: thermostat + second half kick of the previous
+                                step + half kick + drift + max displacement (fused) */
+#define ESPP_K_INTEGRATE2 3  /* thermostat + half kick after the last step of a run */
 #define ESPP_K_BINNING 4     /* fold, counting sort and permutation (all kernels of one binning) */
 #define ESPP_K_BOND 5        /* FENE bond-force kernel */
 

@@ -24,6 +24,9 @@
  *   - Lennard-Jones: src/interaction/LennardJones.hpp:79-160,
  *       src/interaction/Potential.hpp:190-235,360-430 (cut test distSqr>cutoffSqr)
  *   - FENE: src/interaction/FENE.hpp:91-112
+ *   - Tabulated: src/interaction/Tabulated.hpp:43-115 with its interpolation
+ *       tables InterpolationLinear.cpp:35-150 (.hpp:58-75), InterpolationAkima.cpp:136-214
+ *       (.hpp:86-120) and InterpolationCubic.cpp:58-215 (.hpp:89-120)
  *   - pair loops with Newton-3, energy, virial, virial tensor:
  *       src/interaction/VerletListInteractionTemplate.hpp:116-331
  *       src/interaction/FixedPairListInteractionTemplate.hpp:112-262,400-462
@@ -46,6 +49,10 @@
  *     differ only by rounding.
  *   - type pairs without a potential are skipped (the reference evaluates a
  *     default LennardJones(eps=0, sigma=0) that yields zero force).
+ *   - table index: clamped to the last interval [N-2] where the reference
+ *     clamps to N-1 and then reads past its coefficient arrays (distances at or
+ *     beyond the last grid point); the Akima table keeps the reference's
+ *     integer truncation of z*z.
  */
 #include <math.h>
 #include <stdint.h>
@@ -84,11 +91,25 @@ typedef struct {
     int32_t *a, *b;          /* bond ids, stored as given (a, b) */
 } or_fpl;
 
+/* Interpolation table of interaction.Tabulated (src/interaction/Tabulated.hpp:43-115).
+ * itype 1 linear (src/interaction/InterpolationLinear.cpp:35-150), 2 Akima
+ * (src/interaction/InterpolationAkima.cpp, .hpp:86-125), 3 cubic spline
+ * (src/interaction/InterpolationCubic.cpp, .hpp:89-120). */
 typedef struct {
-    int kind;                /* 0: VerletList LJ, 1: FixedPairList FENE */
+    int set, itype, N;
+    double inner, outer, delta, invdelta, deltasq6;
+    double cutoff, cutoffSqr, shift;
+    double *radius;
+    double *ce[4], *cf[4];   /* per-interval coefficients of energy and force */
+} or_tab;
+
+typedef struct {
+    int kind;                /* 0: VerletList LJ, 1: FixedPairList FENE,
+                                2: VerletList Tabulated, 3: FixedPairList Tabulated */
     int list;
     or_lj lj[OR_MAXT * OR_MAXT];
     double K, r0, rMax, rMaxSqr, cutoff, cutoffSqr, shift;
+    or_tab *tab[OR_MAXT * OR_MAXT];   /* kind 2: per type pair; kind 3: tab[0] */
 } or_ia;
 
 typedef struct or_sim {
@@ -201,6 +222,25 @@ or_sim *or_create(const double *L, const int *cells, double skin, uint64_t seed)
     return s;
 }
 
+static void tab_free(or_tab *t) {
+    if (!t) return;
+    free(t->radius);
+    for (int k = 0; k < 4; ++k) { free(t->ce[k]); free(t->cf[k]); }
+    free(t);
+}
+
+static void ia_free(or_ia *a) {
+    if (!a) return;
+    for (int k = 0; k < OR_MAXT * OR_MAXT; ++k) {
+        or_tab *t = a->tab[k];
+        if (!t) continue;
+        /* a table is shared by (t1, t2) and (t2, t1): clear every alias before freeing */
+        for (int q = k; q < OR_MAXT * OR_MAXT; ++q) if (a->tab[q] == t) a->tab[q] = NULL;
+        tab_free(t);
+    }
+    free(a);
+}
+
 static void vl_free(or_vl *v) {
     free(v->p1); free(v->p2); free(v->sh); free(v->excl); free(v->cell_pstart);
     memset(v, 0, sizeof(*v));
@@ -212,7 +252,7 @@ void or_destroy(or_sim *s) {
     free(s->loc_of_id); free(s->cell_of); free(s->cstart); free(s->clist);
     for (int i = 0; i < s->nvl; ++i) vl_free(&s->vl[i]);
     for (int i = 0; i < s->nfpl; ++i) { free(s->fpl[i].a); free(s->fpl[i].b); }
-    for (int i = 0; i < s->nia; ++i) free(s->ia[i]);
+    for (int i = 0; i < s->nia; ++i) ia_free(s->ia[i]);
     free(s->lang_excl);
     free(s);
 }
@@ -618,6 +658,243 @@ void or_fene_force(double K, double r0, double rMax, const double *d, double *F)
     fene_force(&a, d, F);
 }
 
+/* ---------------------------------------------------------------- Tabulated */
+/* Coefficients of the linear table: a r + b on interval i
+ * (InterpolationLinear::spline, src/interaction/InterpolationLinear.cpp:139-150). */
+static void tab_linear(const double *x, const double *y, int N, double *a, double *b) {
+    for (int i = 0; i < N - 1; i++) {
+        a[i] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+        b[i] = y[i] - a[i] * x[i];
+    }
+    a[N - 1] = a[N - 2 >= 0 ? N - 2 : 0];
+    b[N - 1] = b[N - 2 >= 0 ? N - 2 : 0];
+}
+
+/* Second derivatives of the clamped cubic spline: the tridiagonal sweep of
+ * InterpolationCubic::spline (src/interaction/InterpolationCubic.cpp:170-215);
+ * an end slope above 0.99e30 selects the natural end. */
+static void tab_cubic(const double *x, const double *y, int n, double yp1, double ypn, double *y2) {
+    double *u = (double *)malloc((size_t)n * sizeof(double));
+    if (yp1 > 0.99e30) { y2[0] = 0.0; u[0] = 0.0; }
+    else {
+        y2[0] = -0.5;
+        u[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - yp1);
+    }
+    for (int i = 1; i < n - 1; i++) {
+        double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+        u[i] = (6.0 * u[i] / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    double qn, un;
+    if (ypn > 0.99e30) { qn = 0.0; un = 0.0; }
+    else {
+        qn = 0.5;
+        un = (3.0 / (x[n - 1] - x[n - 2])) * (ypn - (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]));
+    }
+    y2[n - 1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+    for (int k = n - 2; k >= 0; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+    free(u);
+}
+
+/* Akima slope from four neighbouring secants (InterpolationAkima::getSlope,
+ * src/interaction/InterpolationAkima.hpp:108-120). */
+static double akima_slope(double m1, double m2, double m3, double m4) {
+    if ((m1 == m2) && (m3 == m4)) return (m2 + m3) / 2.0;
+    return (fabs(m4 - m3) * m2 + fabs(m2 - m1) * m3) / (fabs(m4 - m3) + fabs(m2 - m1));
+}
+
+/* Akima coefficients p0..p3 per interval, with two points extrapolated on each
+ * side by a parabola (InterpolationAkima::spline, src/interaction/InterpolationAkima.cpp:136-214).
+ * Needs N >= 4. */
+static void tab_akima(const double *x, const double *y, int N, double *p0, double *p1, double *p2, double *p3) {
+    double m1, m2, m3, m4, m5, temp, g0, g1, g2, x1, x2, y1, y2, x4, x5, y4, y5;
+    double *t = (double *)malloc((size_t)N * sizeof(double));
+    temp = (x[1] - x[0]) / (x[2] - x[0]);
+    temp = temp * temp;
+    g0 = y[0];
+    g1 = ((y[1] - y[0]) - temp * (y[2] - y[0])) / ((x[1] - x[0]) - temp * (x[2] - x[0]));
+    g2 = ((y[2] - y[0]) - g1 * (x[2] - x[0])) / ((x[2] - x[0]) * (x[2] - x[0]));
+    x1 = x[0] - (x[2] - x[0]);
+    x2 = x[1] - (x[2] - x[0]);
+    y1 = g0 + g1 * (x1 - x[0]) + g2 * (x1 - x[0]) * (x1 - x[0]);
+    y2 = g0 + g1 * (x2 - x[0]) + g2 * (x2 - x[0]) * (x2 - x[0]);
+    m1 = (y2 - y1) / (x2 - x1);
+    m2 = (y[0] - y2) / (x[0] - x2);
+    m3 = (y[1] - y[0]) / (x[1] - x[0]);
+    m4 = (y[2] - y[1]) / (x[2] - x[1]);
+    t[0] = akima_slope(m1, m2, m3, m4);
+    m5 = (y[3] - y[2]) / (x[3] - x[2]);
+    t[1] = akima_slope(m2, m3, m4, m5);
+    temp = (x[N - 2] - x[N - 1]) / (x[N - 3] - x[N - 1]);
+    temp = temp * temp;
+    g0 = y[N - 1];
+    g1 = ((y[N - 2] - y[N - 1]) - temp * (y[N - 3] - y[N - 1])) /
+         ((x[N - 2] - x[N - 1]) - temp * (x[N - 3] - x[N - 1]));
+    g2 = ((y[N - 3] - y[N - 1]) - g1 * (x[N - 3] - x[N - 1])) /
+         ((x[N - 3] - x[N - 1]) * (x[N - 3] - x[N - 1]));
+    x4 = x[N - 2] + (x[N - 1] - x[N - 3]);
+    x5 = x[N - 1] + (x[N - 1] - x[N - 3]);
+    y4 = g0 + g1 * (x4 - x[N - 1]) + g2 * (x4 - x[N - 1]) * (x4 - x[N - 1]);
+    y5 = g0 + g1 * (x5 - x[N - 1]) + g2 * (x5 - x[N - 1]) * (x5 - x[N - 1]);
+    m1 = (y[N - 3] - y[N - 4]) / (x[N - 3] - x[N - 4]);
+    m2 = (y[N - 2] - y[N - 3]) / (x[N - 2] - x[N - 3]);
+    m3 = (y[N - 1] - y[N - 2]) / (x[N - 1] - x[N - 2]);
+    m4 = (y4 - y[N - 1]) / (x4 - x[N - 1]);
+    m5 = (y5 - y4) / (x5 - x4);
+    t[N - 2] = akima_slope(m1, m2, m3, m4);
+    t[N - 1] = akima_slope(m2, m3, m4, m5);
+    for (int i = 2; i < N - 2; i++) {
+        m1 = (y[i - 1] - y[i - 2]) / (x[i - 1] - x[i - 2]);
+        m2 = (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+        m3 = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+        m4 = (y[i + 2] - y[i + 1]) / (x[i + 2] - x[i + 1]);
+        t[i] = akima_slope(m1, m2, m3, m4);
+    }
+    for (int i = 0; i < N - 1; i++) {
+        p0[i] = y[i];
+        p1[i] = t[i];
+        p2[i] = (3.0 * (y[i + 1] - y[i]) / (x[i + 1] - x[i]) - 2.0 * t[i] - t[i + 1]) / (x[i + 1] - x[i]);
+        p3[i] = (t[i] + t[i + 1] - 2.0 * (y[i + 1] - y[i]) / (x[i + 1] - x[i])) /
+                ((x[i + 1] - x[i]) * (x[i + 1] - x[i]));
+    }
+    p0[N - 1] = p0[N - 2]; p1[N - 1] = p1[N - 2]; p2[N - 1] = p2[N - 2]; p3[N - 1] = p3[N - 2];
+    free(t);
+}
+
+/* Table from rows (r, e, f) on a uniform grid.  The grid spacing is the mean
+ * spacing (outer - inner) / (N - 1) as in the reference's readRaw. */
+static or_tab *tab_create(int itype, int N, const double *r, const double *e, const double *f,
+                          double cutoff) {
+    if (itype < 1 || itype > 3 || N < 2 || (itype == 2 && N < 4)) return NULL;
+    or_tab *t = (or_tab *)calloc(1, sizeof(or_tab));
+    t->set = 1; t->itype = itype; t->N = N;
+    t->radius = (double *)malloc((size_t)N * sizeof(double));
+    memcpy(t->radius, r, (size_t)N * sizeof(double));
+    for (int k = 0; k < 4; ++k) {
+        t->ce[k] = (double *)calloc((size_t)N, sizeof(double));
+        t->cf[k] = (double *)calloc((size_t)N, sizeof(double));
+    }
+    t->inner = r[0];
+    t->outer = r[N - 1];
+    t->delta = (t->outer - t->inner) / (N - 1);
+    t->invdelta = 1.0 / t->delta;
+    t->deltasq6 = t->delta * t->delta / 6.0;
+    t->cutoff = cutoff; t->cutoffSqr = cutoff * cutoff; t->shift = 0.0;
+    if (itype == 1) {
+        tab_linear(r, e, N, t->ce[0], t->ce[1]);
+        tab_linear(r, f, N, t->cf[0], t->cf[1]);
+    } else if (itype == 2) {
+        tab_akima(r, e, N, t->ce[0], t->ce[1], t->ce[2], t->ce[3]);
+        tab_akima(r, f, N, t->cf[0], t->cf[1], t->cf[2], t->cf[3]);
+    } else {
+        memcpy(t->ce[0], e, (size_t)N * sizeof(double));
+        memcpy(t->cf[0], f, (size_t)N * sizeof(double));
+        /* energy end slopes are minus the tabulated forces; force end slopes are secants */
+        tab_cubic(r, e, N, -f[0], -f[N - 1], t->ce[1]);
+        tab_cubic(r, f, N, (f[1] - f[0]) / (r[1] - r[0]), (f[N - 1] - f[N - 2]) / (r[N - 1] - r[N - 2]), t->cf[1]);
+    }
+    return t;
+}
+
+/* Interval index of r: truncation of (r - inner)/delta, clamped to the table's
+ * intervals [0, N-2].  The reference clamps only to [0, N-1] and then reads
+ * past its coefficient arrays for r at or beyond the last grid point; the last
+ * interval is extended instead. */
+static inline int tab_index(const or_tab *t, double r) {
+    int index = (int)((r - t->inner) * t->invdelta);
+    if (index < 0) index = 0;
+    if (index > t->N - 2) index = t->N - 2;
+    return index;
+}
+
+static double tab_eval(const or_tab *t, double *const c[4], double r) {
+    int i = tab_index(t, r);
+    if (t->itype == 1) return c[0][i] * r + c[1][i];
+    if (t->itype == 2) {
+        double z = r - t->radius[i];
+        /* the reference stores z*z in an int (src/interaction/InterpolationAkima.hpp:105);
+           kept so the values are the reference's */
+        int zz2 = (int)(z * z);
+        return c[0][i] + c[1][i] * z + c[2][i] * zz2 + c[3][i] * zz2 * z;
+    }
+    double b = (r - t->radius[i]) * t->invdelta;
+    double a = 1.0 - b;
+    return a * c[0][i] + b * c[0][i + 1] +
+           ((a * a * a - a) * c[1][i] + (b * b * b - b) * c[1][i + 1]) * t->deltasq6;
+}
+
+double or_tab_energy_at(const or_tab *t, double r) { return tab_eval(t, (double *const *)t->ce, r); }
+double or_tab_force_at(const or_tab *t, double r) { return tab_eval(t, (double *const *)t->cf, r); }
+
+/* Tabulated::_computeForceRaw (src/interaction/Tabulated.hpp:100-115) behind
+ * PotentialTemplate's cut test (distSqr > cutoffSqr -> no force). */
+static inline int tab_force(const or_tab *t, const double d[3], double F[3]) {
+    if (!t || !t->set) return 0;
+    double distSqr = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (distSqr > t->cutoffSqr) return 0;
+    double distrt = sqrt(distSqr);
+    double ffactor = or_tab_force_at(t, distrt);
+    ffactor /= distrt;
+    F[0] = d[0] * ffactor; F[1] = d[1] * ffactor; F[2] = d[2] * ffactor;
+    return 1;
+}
+
+static inline double tab_energy(const or_tab *t, double distSqr) {
+    if (!t || !t->set || distSqr > t->cutoffSqr) return 0.0;
+    return or_tab_energy_at(t, sqrt(distSqr)) - t->shift;
+}
+
+/* stand-alone table for known-answer checks (handle freed with or_tab_destroy) */
+or_tab *or_tab_new(int itype, int N, const double *r, const double *e, const double *f, double cutoff) {
+    return tab_create(itype, N, r, e, f, cutoff);
+}
+void or_tab_destroy(or_tab *t) { tab_free(t); }
+void or_tab_eval(const or_tab *t, int n, const double *r, double *e, double *f) {
+    for (int k = 0; k < n; ++k) { e[k] = or_tab_energy_at(t, r[k]); f[k] = or_tab_force_at(t, r[k]); }
+}
+
+int or_ia_tab_create(or_sim *s, int vl) {
+    if (s->nia >= OR_MAXIA) { or_seterr(s, "too many interactions"); return -1; }
+    or_ia *a = (or_ia *)calloc(1, sizeof(or_ia));
+    a->kind = 2; a->list = vl;
+    s->ia[s->nia] = a;
+    return s->nia++;
+}
+
+/* VerletListTabulated::setPotential(type1, type2, Tabulated(itype, file, cutoff)) */
+int or_ia_tab_set(or_sim *s, int ii, int t1, int t2, int itype, int N, const double *r, const double *e,
+                  const double *f, double cutoff) {
+    if (t1 < 0 || t2 < 0 || t1 >= OR_MAXT || t2 >= OR_MAXT) { or_seterr(s, "type out of range"); return -1; }
+    or_tab *t = tab_create(itype, N, r, e, f, cutoff);
+    if (!t) { or_seterr(s, "illegal table"); return -1; }
+    or_ia *a = s->ia[ii];
+    or_tab *old1 = a->tab[t1 * OR_MAXT + t2], *old2 = a->tab[t2 * OR_MAXT + t1];
+    a->tab[t1 * OR_MAXT + t2] = t;
+    a->tab[t2 * OR_MAXT + t1] = t;
+    int used = 0;
+    for (int k = 0; k < OR_MAXT * OR_MAXT; ++k) used |= (a->tab[k] == old1);
+    if (old1 && !used) tab_free(old1);
+    used = 0;
+    for (int k = 0; k < OR_MAXT * OR_MAXT; ++k) used |= (a->tab[k] == old2);
+    if (old2 && old2 != old1 && !used) tab_free(old2);
+    return 0;
+}
+
+/* FixedPairListTabulated(system, fpl, Tabulated(itype, file, cutoff)) */
+int or_ia_tabbond_create(or_sim *s, int fpl, int itype, int N, const double *r, const double *e,
+                         const double *f, double cutoff) {
+    if (s->nia >= OR_MAXIA) { or_seterr(s, "too many interactions"); return -1; }
+    or_tab *t = tab_create(itype, N, r, e, f, cutoff);
+    if (!t) { or_seterr(s, "illegal table"); return -1; }
+    or_ia *a = (or_ia *)calloc(1, sizeof(or_ia));
+    a->kind = 3; a->list = fpl;
+    a->tab[0] = t;
+    s->ia[s->nia] = a;
+    return s->nia++;
+}
+
 /* OrthorhombicBC::getMinimumImageVectorBox */
 static inline void min_image_box(const or_sim *s, const double *p1, const double *p2, double d[3]) {
     for (int k = 0; k < 3; ++k) {
@@ -633,7 +910,37 @@ static inline void pair_dist(const or_sim *s, const or_vl *v, int64_t k, double 
     for (int c = 0; c < 3; ++c) d[c] = x1[c] - (x2[c] + sh[c] * s->L[c]);
 }
 
-/* LJ over the half list with Newton-3.  Cells are processed in z-layer phases
+/* Pair force of a VerletList interaction: LennardJones (kind 0) or Tabulated (kind 2). */
+static inline int vl_pair_force(const or_sim *s, const or_ia *a, int32_t i, int32_t j, const double d[3],
+                                double F[3]) {
+    const int tp = s->type[i] * OR_MAXT + s->type[j];
+    if (a->kind == 0) {
+        const or_lj *p = &a->lj[tp];
+        return p->set ? lj_force(p, d, F) : 0;
+    }
+    return tab_force(a->tab[tp], d, F);
+}
+
+static inline double vl_pair_energy(const or_sim *s, const or_ia *a, int32_t i, int32_t j, double d2) {
+    const int tp = s->type[i] * OR_MAXT + s->type[j];
+    if (a->kind == 0) {
+        const or_lj *p = &a->lj[tp];
+        if (!p->set) return 0.0;
+        return (d2 > p->rc2) ? 0.0 : lj_energy_raw(p, d2) - p->shift;
+    }
+    return tab_energy(a->tab[tp], d2);
+}
+
+static inline int bond_force(const or_ia *a, const double d[3], double F[3]) {
+    return a->kind == 1 ? fene_force(a, d, F) : tab_force(a->tab[0], d, F);
+}
+
+static inline double bond_energy(const or_ia *a, double d2) {
+    if (a->kind == 1) return (d2 > a->cutoffSqr) ? 0.0 : fene_energy_raw(a, d2) - a->shift;
+    return tab_energy(a->tab[0], d2);
+}
+
+/* Pair interactions over the half list with Newton-3.  Cells are processed in z-layer phases
  * (even layers, odd layers, then a lone last layer when nz is odd) so that
  * threads never write the same particle; the per-particle summation order is
  * the same for every thread count. */
@@ -651,11 +958,9 @@ static void lj_add_forces(or_sim *s, const or_ia *a) {
             for (int64_t c = z * per_layer; c < (z + 1) * per_layer; ++c) {
                 for (int64_t k = v->cell_pstart[c]; k < v->cell_pstart[c + 1]; ++k) {
                     int32_t i = v->p1[k], j = v->p2[k];
-                    const or_lj *p = &a->lj[s->type[i] * OR_MAXT + s->type[j]];
-                    if (!p->set) continue;
                     double d[3], F[3];
                     pair_dist(s, v, k, d);
-                    if (lj_force(p, d, F)) {
+                    if (vl_pair_force(s, a, i, j, d, F)) {
                         for (int c3 = 0; c3 < 3; ++c3) { s->f[3 * i + c3] += F[c3]; s->f[3 * j + c3] -= F[c3]; }
                     }
                 }
@@ -677,7 +982,7 @@ static void fene_add_forces(or_sim *s, const or_ia *a) {
     for (int64_t k = 0; k < p->n; ++k) {
         int32_t i, j; double d[3], F[3];
         if (!fene_bond_dist(s, p, k, &i, &j, d)) continue;
-        if (fene_force(a, d, F))
+        if (bond_force(a, d, F))
             for (int c = 0; c < 3; ++c) { s->f[3 * i + c] += F[c]; s->f[3 * j + c] -= F[c]; }
     }
 }
@@ -706,7 +1011,7 @@ static void thermalize(or_sim *s) {
 static void update_forces(or_sim *s) {
     memset(s->f, 0, (size_t)s->np * 3 * sizeof(double));
     for (int k = 0; k < s->nia; ++k) {
-        if (s->ia[k]->kind == 0) lj_add_forces(s, s->ia[k]);
+        if (s->ia[k]->kind == 0 || s->ia[k]->kind == 2) lj_add_forces(s, s->ia[k]);
         else fene_add_forces(s, s->ia[k]);
     }
     if (s->lang_on) thermalize(s);
@@ -716,7 +1021,7 @@ static void update_forces(or_sim *s) {
 int or_calc_forces(or_sim *s) {
     memset(s->f, 0, (size_t)s->np * 3 * sizeof(double));
     for (int k = 0; k < s->nia; ++k) {
-        if (s->ia[k]->kind == 0) lj_add_forces(s, s->ia[k]);
+        if (s->ia[k]->kind == 0 || s->ia[k]->kind == 2) lj_add_forces(s, s->ia[k]);
         else fene_add_forces(s, s->ia[k]);
     }
     return 0;
@@ -818,14 +1123,12 @@ int or_run(or_sim *s, int nsteps) {
 double or_ia_energy(or_sim *s, int ii) {
     const or_ia *a = s->ia[ii];
     double es = 0.0;
-    if (a->kind == 0) {
+    if (a->kind == 0 || a->kind == 2) {
         const or_vl *v = &s->vl[a->list];
         for (int64_t k = 0; k < v->npairs; ++k) {
-            const or_lj *p = &a->lj[s->type[v->p1[k]] * OR_MAXT + s->type[v->p2[k]]];
-            if (!p->set) continue;
             double d[3]; pair_dist(s, v, k, d);
             double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            es += (d2 > p->rc2) ? 0.0 : lj_energy_raw(p, d2) - p->shift;
+            es += vl_pair_energy(s, a, v->p1[k], v->p2[k], d2);
         }
     } else {
         const or_fpl *p = &s->fpl[a->list];
@@ -833,7 +1136,7 @@ double or_ia_energy(or_sim *s, int ii) {
             int32_t i, j; double d[3];
             if (!fene_bond_dist(s, p, k, &i, &j, d)) continue;
             double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            es += (d2 > a->cutoffSqr) ? 0.0 : fene_energy_raw(a, d2) - a->shift;
+            es += bond_energy(a, d2);
         }
     }
     return es;
@@ -843,13 +1146,11 @@ double or_ia_energy(or_sim *s, int ii) {
 double or_ia_virial(or_sim *s, int ii, double *w6) {
     const or_ia *a = s->ia[ii];
     double w = 0.0, t[6] = {0, 0, 0, 0, 0, 0};
-    if (a->kind == 0) {
+    if (a->kind == 0 || a->kind == 2) {
         const or_vl *v = &s->vl[a->list];
         for (int64_t k = 0; k < v->npairs; ++k) {
-            const or_lj *p = &a->lj[s->type[v->p1[k]] * OR_MAXT + s->type[v->p2[k]]];
-            if (!p->set) continue;
             double d[3], F[3]; pair_dist(s, v, k, d);
-            if (lj_force(p, d, F)) {
+            if (vl_pair_force(s, a, v->p1[k], v->p2[k], d, F)) {
                 w = w + (d[0] * F[0] + d[1] * F[1] + d[2] * F[2]);
                 t[0] += d[0] * F[0]; t[1] += d[1] * F[1]; t[2] += d[2] * F[2];
                 t[3] += d[0] * F[1]; t[4] += d[0] * F[2]; t[5] += d[1] * F[2];
@@ -860,7 +1161,7 @@ double or_ia_virial(or_sim *s, int ii, double *w6) {
         for (int64_t k = 0; k < p->n; ++k) {
             int32_t i, j; double d[3], F[3];
             if (!fene_bond_dist(s, p, k, &i, &j, d)) continue;
-            if (fene_force(a, d, F)) {
+            if (bond_force(a, d, F)) {
                 w = w + (d[0] * F[0] + d[1] * F[1] + d[2] * F[2]);
                 t[0] += d[0] * F[0]; t[1] += d[1] * F[1]; t[2] += d[2] * F[2];
                 t[3] += d[0] * F[1]; t[4] += d[0] * F[2]; t[5] += d[1] * F[2];

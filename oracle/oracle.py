@@ -71,6 +71,12 @@ def lib():
             "or_lj_force": (None, [dbl, dbl, dbl, vp, vp]),
             "or_fene_energy": (dbl, [dbl, dbl, dbl, dbl]),
             "or_fene_force": (None, [dbl, dbl, dbl, vp, vp]),
+            "or_tab_new": (vp, [i32, i32, vp, vp, vp, dbl]),
+            "or_tab_destroy": (None, [vp]),
+            "or_tab_eval": (None, [vp, i32, vp, vp, vp]),
+            "or_ia_tab_create": (i32, [vp, i32]),
+            "or_ia_tab_set": (i32, [vp, i32, i32, i32, i32, i32, vp, vp, vp, dbl]),
+            "or_ia_tabbond_create": (i32, [vp, i32, i32, i32, vp, vp, vp, dbl]),
             "or_philox4x32_10": (None, [vp, vp, vp]),
             "or_langevin_uniforms": (None, [u64, u64, i32, vp]),
         }
@@ -120,6 +126,30 @@ def fene_force(K, r0, rMax, d):
     F = np.zeros(3)
     lib().or_fene_force(K, r0, rMax, _ptr(d), _ptr(F))
     return F
+
+
+def _table(r, e, f):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    if not (len(r) == len(e) == len(f)):
+        raise ValueError("table columns differ in length")
+    return r, e, f
+
+
+def tab_eval(itype, r, e, f, at, cutoff=float("inf")):
+    """Energy and force of a Tabulated table (rows r, e, f) at distances `at`."""
+    r, e, f = _table(r, e, f)
+    h = lib().or_tab_new(int(itype), len(r), _ptr(r), _ptr(e), _ptr(f), float(cutoff))
+    if not h:
+        raise ValueError("illegal table")
+    try:
+        at = np.ascontiguousarray(at, dtype=np.float64)
+        eo, fo = np.zeros(len(at)), np.zeros(len(at))
+        lib().or_tab_eval(h, len(at), _ptr(at), _ptr(eo), _ptr(fo))
+    finally:
+        lib().or_tab_destroy(h)
+    return eo, fo
 
 
 class OracleSim:
@@ -200,6 +230,19 @@ class OracleSim:
 
     def fene_create(self, fpl, K=30.0, r0=0.0, rMax=1.5, cutoff=float("inf"), shift=0.0):
         return self._check(lib().or_ia_fene_create(self._h, fpl, K, r0, rMax, cutoff, shift))
+
+    def tab_create(self, vl):
+        return self._check(lib().or_ia_tab_create(self._h, vl))
+
+    def tab_set(self, ia, t1, t2, itype, r, e, f, cutoff=float("inf")):
+        r, e, f = _table(r, e, f)
+        self._check(lib().or_ia_tab_set(self._h, ia, t1, t2, int(itype), len(r), _ptr(r), _ptr(e), _ptr(f),
+                                        float(cutoff)))
+
+    def tabbond_create(self, fpl, itype, r, e, f, cutoff=float("inf")):
+        r, e, f = _table(r, e, f)
+        return self._check(lib().or_ia_tabbond_create(self._h, fpl, int(itype), len(r), _ptr(r), _ptr(e), _ptr(f),
+                                                      float(cutoff)))
 
     def energy(self, ia):
         return lib().or_ia_energy(self._h, ia)

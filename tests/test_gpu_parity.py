@@ -304,3 +304,73 @@ def test_empty_system_and_errors(gpu):
     with pytest.raises(ValueError):
         gpu_system((10.0, 10.0, 10.0), (5, 5, 5), 0.3)
         espressopp.VerletList(gpu_system((10.0, 10.0, 10.0), (5, 5, 5), 0.3), 2.5)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_free_flight_gpu(gpu, precision):
+    # testsuite/integrator/PTestVelocityVerlet.cpp:222-256: free particles cross the box and return
+    pos, box = jittered_lattice(125, 0.5, seed=3, jitter=0.0)
+    L = box[0]
+    system = gpu_system(box, (2, 2, 2), 0.3, precision)
+    system.storage.addParticlesArray(np.arange(125), pos, vel=np.tile([1.0, -0.5, 0.25], (125, 1)))
+    system.storage.decompose()
+    integ = espressopp.integrator.VelocityVerlet(system)
+    integ.dt = L / 400.0
+    integ.run(1600)                          # x moves 4 L, y -2 L, z 1 L
+    a = system.storage.getParticleArrays()
+    p = a["pos"] + a["image"] * np.asarray(box)
+    assert np.allclose(p, pos + np.array([4 * L, -2 * L, L]), atol=1e-6)
+    assert np.all((a["pos"] >= -1e-9) & (a["pos"] <= L + 1e-9))
+
+
+def test_decompose_folds_and_conserves(gpu):
+    # testsuite/storage/PTestDomainDecomposition.cpp:117-238 pattern: every particle is kept,
+    # folded into the box with its image count, and fields survive reordering into cells
+    rng = np.random.default_rng(21)
+    box = (9.0, 11.0, 13.0)
+    n = 5000
+    ids = rng.permutation(np.arange(3, 3 * n + 3, 3)).astype(np.int32)   # sparse, unordered ids
+    pos = rng.uniform(-2.5, 3.5, (n, 3)) * np.asarray(box)              # up to a few boxes away
+    vel = rng.normal(size=(n, 3))
+    mass = rng.uniform(0.5, 2.0, n)
+    types = rng.integers(0, 4, n).astype(np.int32)
+    system = gpu_system(box, (3, 3, 4), 0.3)
+    system.storage.addParticlesArray(ids, pos, types, vel, mass)
+    system.storage.decompose()
+    ref = OracleSim(box, (3, 3, 4), 0.3)
+    ref.add_particles(ids, pos, types=types, vel=vel, mass=mass)
+    ref.decompose()
+    a = system.storage.getParticleArrays()
+    order = np.argsort(ids)
+    assert np.array_equal(a["id"], ids[order])
+    assert np.array_equal(a["id"], ref.ids())
+    assert np.array_equal(a["pos"], ref.get("pos"))           # the fold is FP64 and bit-exact
+    assert np.array_equal(a["image"], ref.get("image"))
+    assert np.array_equal(a["v"], vel[order])
+    assert np.array_equal(a["mass"], mass[order])
+    assert np.array_equal(a["type"], types[order])
+    assert np.all((a["pos"] >= -1e-9) & (a["pos"] <= np.asarray(box) + 1e-9))
+    # set/get by id after decompose
+    pick = ids[order][::7]
+    newv = rng.normal(size=(len(pick), 3))
+    system.storage.setParticleArrays("v", pick, newv)
+    assert np.array_equal(system.storage.getParticleArrays()["v"][::7], newv)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pairs_and_forces_262k(gpu, precision):
+    # 64^3 particles at the benchmark density: the full pair set and the forces against the oracle
+    n = 64 ** 3
+    pos, box = jittered_lattice(n, 0.8442, seed=22)
+    rc, skin = 2.5, 0.3
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells_for(box, rc, skin), rc, skin, precision)
+    got = np.sort(vl.pairs_array().reshape(-1, 2).astype(np.int64), axis=1)
+    want = np.sort(ref.vl_pairs(rvl).reshape(-1, 2).astype(np.int64), axis=1)
+    assert len(got) == len(want) == vl.totalSize()
+    kg, kw = np.sort(got[:, 0] * n + got[:, 1]), np.sort(want[:, 0] * n + want[:, 1])
+    assert np.array_equal(kg, kw)
+    espressopp.integrator.VelocityVerlet(system).run(0)
+    ref.calc_forces()
+    err = force_rel_err(system.storage.getParticleArrays()["f"], ref.get("f"), 0.1)
+    assert err.max() <= (1e-12 if precision == "fp64" else 1e-5)
+    assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=1e-12 if precision == "fp64" else 1e-6)

@@ -186,3 +186,113 @@ def test_thread_count_does_not_change_results():
         s.run(20, dt=0.005)
         out.append(s.get("pos"))
     assert np.array_equal(out[0], out[1])
+
+
+# ---------------------------------------------------------------- Tabulated tables
+def test_tab_linear_reproduces_linear_functions():
+    # InterpolationLinear: a r + b per interval is exact for a linear table
+    r = np.linspace(0.3, 2.7, 33)
+    e, f = 2.0 - 0.75 * r, 0.5 + 1.25 * r
+    at = np.concatenate([np.linspace(0.3, 2.69999, 97), [0.1, 2.7, 2.9]])   # both ends extrapolate
+    ev, fv = oracle.tab_eval(1, r, e, f, at)
+    assert np.allclose(ev, 2.0 - 0.75 * at, rtol=0, atol=1e-13)
+    assert np.allclose(fv, 0.5 + 1.25 * at, rtol=0, atol=1e-13)
+
+
+def test_tab_cubic_exact_for_cubic_energy_and_linear_force():
+    # clamped cubic spline (end slopes -f) reproduces a cubic energy whose force is -dE/dr;
+    # a linear force table (secant end slopes) is reproduced exactly as well
+    r = np.linspace(0.5, 2.5, 41)
+    E = lambda x: 3 * x ** 3 - 2 * x ** 2 + x - 5  # noqa: E731
+    Fc = lambda x: -(9 * x ** 2 - 4 * x + 1)       # noqa: E731
+    at = np.linspace(0.5, 2.4999, 301)
+    ev, _ = oracle.tab_eval(3, r, E(r), Fc(r), at)
+    assert np.allclose(ev, E(at), rtol=0, atol=1e-11)
+    _, fv = oracle.tab_eval(3, r, E(r), 4.0 - 2.0 * r, at)
+    assert np.allclose(fv, 4.0 - 2.0 * at, rtol=0, atol=1e-12)
+    # grid points are interpolated exactly for any data
+    rng = np.random.default_rng(3)
+    y = rng.normal(size=len(r))
+    ev, fv = oracle.tab_eval(3, r, y, y, r[:-1])
+    assert np.allclose(ev, y[:-1], atol=1e-12) and np.allclose(fv, y[:-1], atol=1e-12)
+
+
+@pytest.mark.parametrize("delta", [0.05, 1.5])
+def test_tab_akima_quadratic_with_integer_square(delta):
+    # Akima slopes are exact for quadratic data (equal weights, parabolic end extension), so
+    # on interval i: y = y_i + y'(x_i) z + c * int(z*z) + 0 * ..., the reference keeping
+    # z*z in an int (src/interaction/InterpolationAkima.hpp:105)
+    c, b0, a0 = 0.7, -1.1, 2.0
+    r = 0.4 + delta * np.arange(20)
+    y = c * r ** 2 + b0 * r + a0
+    i = np.arange(len(r) - 1)
+    z = delta * np.array([0.1, 0.45, 0.8, 0.97])
+    at = (r[i][:, None] + z[None, :]).ravel()
+    xi = np.repeat(r[i], len(z))
+    zz = np.tile(z, len(i))
+    want = (c * xi ** 2 + b0 * xi + a0) + (2 * c * xi + b0) * zz + c * np.floor(zz * zz)
+    ev, fv = oracle.tab_eval(2, r, y, y, at)
+    assert np.allclose(ev, want, rtol=1e-12, atol=1e-12)
+    assert np.allclose(fv, want, rtol=1e-12, atol=1e-12)
+
+
+def _lj_table(N, lo, hi, eps=1.0, sig=1.0, rc=2.5):
+    r = np.linspace(lo, hi, N)
+    e = np.array([oracle.lj_energy(eps, sig, rc, 0.0, x) for x in r])
+    f = np.array([oracle.lj_force(eps, sig, float("inf"), [x, 0.0, 0.0])[0] for x in r])
+    return r, e, f
+
+
+def test_tabulated_pair_interaction_matches_lennard_jones():
+    # testsuite/tabulated/polymer_melt_tabulated.py pattern: a fine LJ table run through
+    # VerletListTabulated gives the analytic LJ forces and energy to interpolation accuracy
+    pid, pos, _, box = read_fixture()
+    rc, skin = 2.5, 0.3
+    r, e, f = _lj_table(4001, 0.7, rc)
+    out = {}
+    for kind in ("lj", "tab"):
+        s = OracleSim(box, cells_for(box, rc, skin), skin)
+        s.add_particles(pid, pos)
+        s.decompose()
+        vl = s.vl_create(rc)
+        if kind == "lj":
+            ia = s.lj_create(vl)
+            s.lj_set(ia, 0, 0, 1.0, 1.0, rc, 0.0)
+        else:
+            ia = s.tab_create(vl)
+            s.tab_set(ia, 0, 0, 3, r, e, f, cutoff=rc)
+        s.calc_forces()
+        out[kind] = (s.get("f"), s.energy(ia), s.virial(ia)[0])
+    fl, el, wl = out["lj"]
+    ft, et, wt = out["tab"]
+    scale = np.sqrt(np.mean(np.sum(fl ** 2, axis=1)))
+    assert np.abs(ft - fl).max() < 1e-6 * scale
+    assert et == pytest.approx(el, rel=1e-8)
+    assert wt == pytest.approx(wl, rel=1e-7)
+
+
+def test_tabulated_bond_matches_fene():
+    box = (20.0, 20.0, 20.0)
+    rng = np.random.default_rng(9)
+    n = 200
+    pos = rng.uniform(0, 20, (n, 3))
+    bonds = []
+    for k in range(0, n - 1, 2):   # place partners at random bond lengths 0.6-1.3
+        u = rng.normal(size=3)
+        pos[k + 1] = pos[k] + rng.uniform(0.6, 1.3) * u / np.linalg.norm(u)
+        bonds.append((k, k + 1))
+    r = np.linspace(0.0001, 1.49, 2981)
+    e = np.array([oracle.fene_energy(30.0, 0.0, 1.5, x) for x in r])
+    f = np.array([oracle.fene_force(30.0, 0.0, 1.5, [x, 0.0, 0.0])[0] for x in r])
+    out = {}
+    for kind in ("fene", "tab"):
+        s = OracleSim(box, (3, 3, 3), 0.3)
+        s.add_particles(np.arange(n), pos)
+        s.decompose()
+        fpl = s.fpl_create()
+        s.fpl_add(fpl, bonds)
+        ia = s.fene_create(fpl) if kind == "fene" else s.tabbond_create(fpl, 3, r, e, f)
+        s.calc_forces()
+        out[kind] = (s.get("f"), s.energy(ia))
+    assert np.abs(out["tab"][0] - out["fene"][0]).max() < 1e-5
+    assert out["tab"][1] == pytest.approx(out["fene"][1], rel=1e-8)
