@@ -46,6 +46,9 @@ SIGNATURES = {
     "espp_lj_create": [_vp, _c_int, _P(_c_int)],
     "espp_lj_set": [_vp, _c_int, _c_int, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl],
     "espp_fene_create": [_vp, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _P(_c_int)],
+    "espp_tab_create": [_vp, _c_int, _P(_c_int)],
+    "espp_tab_set": [_vp, _c_int, _c_int, _c_int, _c_int, _c_i64, _vp, _vp, _vp, _c_dbl],
+    "espp_tabbond_create": [_vp, _c_int, _c_int, _c_i64, _vp, _vp, _vp, _c_dbl, _P(_c_int)],
     "espp_add_interaction": [_vp, _c_int],
     "espp_ia_energy": [_vp, _c_int, _P(_c_dbl)],
     "espp_ia_virial": [_vp, _c_int, _P(_c_dbl), _vp],

@@ -41,6 +41,19 @@ struct FENEParam {
   double K, r0, rMax, rMaxSqr, cutoffSqr, shift;
 };
 
+// One interaction.Tabulated table (src/interaction/Tabulated.hpp:43-115).  Its
+// rows live in a coefficient pool at `off`: N force rows then N energy rows,
+// each row kTabRow doubles {r_i, c0, c1, c2, c3}:
+//   itype 1 (linear)  c0 r + c1
+//   itype 2 (Akima)   c0 + c1 z + c2 int(z z) + c3 int(z z) z,  z = r - r_i
+//   itype 3 (cubic)   c0 = y_i, c1 = y''_i, c2 = y_{i+1}, c3 = y''_{i+1}
+struct TabParam {
+  double inner, invdelta, deltasq6, rc2, shift;
+  double dlo, dhi;                 // cutoff window settled from FP64 positions (mixed path)
+  int itype, N, off, set;
+};
+constexpr int kTabRow = 5;
+
 // One contiguous range of cells of a tile's staged region: the particles of
 // cells [cell_a, cell_b] are slots [slot_start, slot_start + count) and sit at
 // LDS offset lds_off of the tile's staging buffer.

@@ -48,6 +48,16 @@ void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, cons
                  const uint4* posq, const int* id, const int* slot_of_id, int nid,
                  const int* bstart, const int* bpartner, const Geom& g, const FENEParam& fp,
                  double* fx, double* fy, double* fz, double* partial);
+void launch_tab_tiled(hipStream_t s, bool mixed, bool energy, bool accum, int n, int T, int lds_bytes,
+                      const uint4* posq, const double4* pos, const uint32_t* nbr, const int* cnt, int capw,
+                      const Geom& g, const TileRun* runs, const int* nruns, const TabParam* prm, const double* pool,
+                      int ntypes, double* fx, double* fy, double* fz, double* partial);
+void launch_tab(hipStream_t s, bool energy, bool accum, int n, const double4* pos, const int* nbr, const int* cnt,
+                int cap, const Geom& g, const TabParam* prm, const double* pool, int ntypes, double* fx, double* fy,
+                double* fz, double* partial);
+void launch_bond_tab(hipStream_t s, bool energy, bool accum, int n, const double4* pos, const int* id,
+                     const int* slot_of_id, int nid, const int* bstart, const int* bpartner, const Geom& g,
+                     const TabParam* prm, const double* pool, double* fx, double* fy, double* fz, double* partial);
 void launch_zero3(hipStream_t s, int n, double* a, double* b, double* c);
 void launch_thermo_integrate2(hipStream_t s, int n, const int* id, double4* vel, double* fx,
                               double* fy, double* fz, int lang, double pref1, double pref2,

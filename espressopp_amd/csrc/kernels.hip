@@ -1066,6 +1066,223 @@ __global__ __launch_bounds__(256) void k_lj_tiled_fp64(
   }
 }
 
+// ------------------------------------------------------------------ Tabulated
+//
+// interaction.Tabulated over a Verlet list or a fixed pair list
+// (src/interaction/Tabulated.hpp:100-115: F = d * table.getForce(r) / r,
+// E = table.getEnergy(r) - shift, cut test distSqr > cutoffSqr).  Distances and
+// table evaluation are FP64 in both precisions; the mixed path forms d from the
+// exact fixed-point differences and settles pairs in the cutoff window from the
+// FP64 positions.
+
+// Interpolation of one table row at r (InterpolationLinear.hpp:58-75,
+// InterpolationAkima.hpp:86-106, InterpolationCubic.hpp:89-120), without FMA
+// contraction, as the reference evaluates it.
+__device__ __forceinline__ double tab_row_eval(const double* __restrict__ row, int itype, double r,
+                                               double invdelta, double deltasq6) {
+#pragma clang fp contract(off)
+  if (itype == 1) return row[1] * r + row[2];
+  if (itype == 2) {
+    const double z = r - row[0];
+    const int zz2 = (int)(z * z);   // integer square, as the reference stores it
+    return row[1] + row[2] * z + row[3] * zz2 + row[4] * zz2 * z;
+  }
+  const double b = (r - row[0]) * invdelta;
+  const double a = 1.0 - b;
+  return a * row[1] + b * row[3] + ((a * a * a - a) * row[2] + (b * b * b - b) * row[4]) * deltasq6;
+}
+
+// row index: truncation of (r - inner) / delta, kept inside the table's intervals
+__device__ __forceinline__ int tab_index(const TabParam& t, double r) {
+  int index = (int)((r - t.inner) * t.invdelta);
+  index = index < 0 ? 0 : index;
+  return index > t.N - 2 ? t.N - 2 : index;
+}
+
+// force factor F(r)/r of a pair at distance^2 r2 inside the cutoff
+__device__ __forceinline__ double tab_ffactor(const TabParam& t, const double* __restrict__ pool, double r2) {
+#pragma clang fp contract(off)
+  const double r = sqrt(r2);
+  const int idx = tab_index(t, r);
+  double ff = tab_row_eval(pool + (size_t)t.off + (size_t)idx * kTabRow, t.itype, r, t.invdelta, t.deltasq6);
+  return ff / r;
+}
+
+__device__ __forceinline__ double tab_energy(const TabParam& t, const double* __restrict__ pool, double r2) {
+#pragma clang fp contract(off)
+  const double r = sqrt(r2);
+  const int idx = tab_index(t, r);
+  return tab_row_eval(pool + (size_t)t.off + (size_t)(t.N + idx) * kTabRow, t.itype, r, t.invdelta, t.deltasq6) -
+         t.shift;
+}
+
+// Adds the contribution of one pair (distance vector d, r2 = |d|^2) to the
+// particle's force or to its energy/virial partials.
+template <bool ENERGY>
+__device__ __forceinline__ void tab_pair(const TabParam& t, const double* __restrict__ pool, double dx, double dy,
+                                         double dz, double r2, double& Ax, double& Ay, double& Az,
+                                         double (&acc)[kNPart]) {
+#pragma clang fp contract(off)
+  if (!t.set || r2 > t.rc2) return;
+  const double ff = tab_ffactor(t, pool, r2);
+  const double Fx = dx * ff, Fy = dy * ff, Fz = dz * ff;
+  if (!ENERGY) {
+    Ax += Fx; Ay += Fy; Az += Fz;
+  } else {
+    acc[0] += tab_energy(t, pool, r2);
+    acc[1] += dx * Fx + dy * Fy + dz * Fz;
+    acc[2] += dx * Fx; acc[3] += dy * Fy; acc[4] += dz * Fz;
+    acc[5] += dx * Fy; acc[6] += dx * Fz; acc[7] += dy * Fz;
+  }
+}
+
+// FP64 minimum-image distance of slots i and j from the FP64 positions
+__device__ __forceinline__ double min_image_fp64(const double4& pi, const double4& pj, const Geom& g, double& dx,
+                                                 double& dy, double& dz) {
+  dx = pi.x - pj.x; dy = pi.y - pj.y; dz = pi.z - pj.z;
+  dx = __builtin_fma(-g.L[0], __builtin_rint(dx * g.invL[0]), dx);
+  dy = __builtin_fma(-g.L[1], __builtin_rint(dy * g.invL[1]), dy);
+  dz = __builtin_fma(-g.L[2], __builtin_rint(dz * g.invL[2]), dz);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Tabulated over the tiled list.  MIXED stages the fixed-point mirror (with
+// type), FP64 stages the FP64 positions (type in w).
+template <bool MIXED, bool ENERGY, bool ACCUM>
+__global__ __launch_bounds__(256) void k_tab_tiled(
+    int n, const uint4* __restrict__ posq, const double4* __restrict__ pos, const uint32_t* __restrict__ nbr,
+    const int* __restrict__ cnt, int capw, Geom g, const TileRun* __restrict__ runs,
+    const int* __restrict__ nruns, const TabParam* __restrict__ prm, const double* __restrict__ pool, int ntypes,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ partial) {
+  extern __shared__ uint4 lds_raw[];
+  __shared__ TileRun sr[kMaxRuns];
+  int nr;
+  if (MIXED) nr = stage_tile_q<uint4>(runs, nruns, posq, lds_raw, sr);
+  else nr = stage_tile<double4>(runs, nruns, pos, reinterpret_cast<double4*>(lds_raw), sr);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kNPart];
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
+  if (i < n) {
+    const double4 pi = pos[i];
+    const uint4 qi = MIXED ? posq[i] : make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t* np = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
+    const int c = nr > 0 ? min(cnt[i], 2 * capw) : 0;
+    const int rowi = (int)pi.w * ntypes;
+    double Ax = 0.0, Ay = 0.0, Az = 0.0;
+    for (int k = 0; k < c; ++k) {
+      const uint32_t w = np[(size_t)(k >> 1) * 64];
+      const int e = (k & 1) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
+      double dx, dy, dz, r2;
+      int tj;
+      if (MIXED) {
+        const uint4 qj = lds_raw[e];
+        tj = (int)qj.w;
+        dx = (double)(int)(qi.x - qj.x) * g.qsd[0];
+        dy = (double)(int)(qi.y - qj.y) * g.qsd[1];
+        dz = (double)(int)(qi.z - qj.z) * g.qsd[2];
+        r2 = dx * dx + dy * dy + dz * dz;
+        const TabParam& t = prm[rowi + tj];
+        if (r2 >= t.dlo && r2 <= t.dhi) {
+          // fixed-point quantisation could flip the cut test: the pair from the FP64 positions
+          r2 = min_image_fp64(pi, pos[slot_of_lds(sr, nr, e)], g, dx, dy, dz);
+        }
+      } else {
+        const double4 pj = reinterpret_cast<const double4*>(lds_raw)[e];
+        tj = (int)pj.w;
+        r2 = min_image_fp64(pi, pj, g, dx, dy, dz);
+      }
+      tab_pair<ENERGY>(prm[rowi + tj], pool, dx, dy, dz, r2, Ax, Ay, Az, acc);
+    }
+    if (!ENERGY) {
+      if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
+      else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
+    }
+  }
+  if (ENERGY) {
+#pragma unroll
+    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
+    block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
+// Tabulated over the untiled full list (neighbour entries are slots).
+template <bool ENERGY, bool ACCUM>
+__global__ __launch_bounds__(256) void k_tab(
+    int n, const double4* __restrict__ pos, const int* __restrict__ nbr, const int* __restrict__ cnt, int cap,
+    Geom g, const TabParam* __restrict__ prm, const double* __restrict__ pool, int ntypes, double* __restrict__ fx,
+    double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ partial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kNPart];
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
+  if (i < n) {
+    const double4 pi = pos[i];
+    const int* np = nbr + (size_t)(i >> 6) * (size_t)cap * 64 + (i & 63);
+    const int c = min(cnt[i], cap);
+    const int rowi = (int)pi.w * ntypes;
+    double Ax = 0.0, Ay = 0.0, Az = 0.0;
+    for (int k = 0; k < c; ++k) {
+      const double4 pj = pos[np[(size_t)k * 64]];
+      double dx, dy, dz;
+      const double r2 = min_image_fp64(pi, pj, g, dx, dy, dz);
+      tab_pair<ENERGY>(prm[rowi + (int)pj.w], pool, dx, dy, dz, r2, Ax, Ay, Az, acc);
+    }
+    if (!ENERGY) {
+      if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
+      else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
+    }
+  }
+  if (ENERGY) {
+#pragma unroll
+    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
+    block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
+// FixedPairListTabulated (src/interaction/FixedPairListInteractionTemplate.hpp:112-200
+// with Tabulated): per-particle bond adjacency, FP64 minimum image of
+// OrthorhombicBC::getMinimumImageVectorBox (src/bc/OrthorhombicBC.cpp:86-105).
+template <bool ENERGY, bool ACCUM>
+__global__ __launch_bounds__(256) void k_bond_tab(
+    int n, const double4* __restrict__ pos, const int* __restrict__ id, const int* __restrict__ slot_of_id, int nid,
+    const int* __restrict__ bstart, const int* __restrict__ bpartner, Geom g, const TabParam* __restrict__ prm,
+    const double* __restrict__ pool, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    double* __restrict__ partial) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kNPart];
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
+  if (i < n) {
+    const int pid = id[i];
+    const double4 pi = pos[i];
+    const TabParam t = prm[0];
+    double Ax = 0.0, Ay = 0.0, Az = 0.0;
+    const int b0 = bstart[pid], b1 = bstart[pid + 1];
+    for (int b = b0; b < b1; ++b) {
+      const int q = bpartner[b];
+      const int j = (q >= 0 && q < nid) ? slot_of_id[q] : -1;
+      if (j < 0) continue;
+      const double4 pj = pos[j];
+      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      if (dx < -g.halfL[0]) dx += g.L[0]; else if (dx > g.halfL[0]) dx -= g.L[0];
+      if (dy < -g.halfL[1]) dy += g.L[1]; else if (dy > g.halfL[1]) dy -= g.L[1];
+      if (dz < -g.halfL[2]) dz += g.L[2]; else if (dz > g.halfL[2]) dz -= g.L[2];
+      tab_pair<ENERGY>(t, pool, dx, dy, dz, dx * dx + dy * dy + dz * dz, Ax, Ay, Az, acc);
+    }
+    if (!ENERGY) {
+      if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
+      else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
+    }
+  }
+  if (ENERGY) {
+#pragma unroll
+    for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
+    block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
 // fixed-point mirror of the FP64 positions (refresh before a tiled build of an FP64 system)
 __global__ void k_posq(int n, const double4* __restrict__ pos, uint4* __restrict__ posq, Geom g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1534,6 +1751,41 @@ void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, cons
     if (energy) FN(false, true, false); else if (accum) FN(false, false, true); else FN(false, false, false);
   }
 #undef FN
+}
+
+void launch_tab_tiled(hipStream_t s, bool mixed, bool energy, bool accum, int n, int T, int lds_bytes,
+                      const uint4* posq, const double4* pos, const uint32_t* nbr, const int* cnt, int capw,
+                      const Geom& g, const TileRun* runs, const int* nruns, const TabParam* prm, const double* pool,
+                      int ntypes, double* fx, double* fy, double* fz, double* partial) {
+  if (n <= 0) return;
+  dim3 grid(nblocks(n, T)), block(T);
+#define TT(MX, EN, AC) hipLaunchKernelGGL((k_tab_tiled<MX, EN, AC>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, pool, ntypes, fx, fy, fz, partial)
+  if (mixed) {
+    if (energy) TT(true, true, false); else if (accum) TT(true, false, true); else TT(true, false, false);
+  } else {
+    if (energy) TT(false, true, false); else if (accum) TT(false, false, true); else TT(false, false, false);
+  }
+#undef TT
+}
+
+void launch_tab(hipStream_t s, bool energy, bool accum, int n, const double4* pos, const int* nbr, const int* cnt,
+                int cap, const Geom& g, const TabParam* prm, const double* pool, int ntypes, double* fx, double* fy,
+                double* fz, double* partial) {
+  if (n <= 0) return;
+  dim3 grid(nblocks(n, 256)), block(256);
+#define TB(EN, AC) hipLaunchKernelGGL((k_tab<EN, AC>), grid, block, 0, s, n, pos, nbr, cnt, cap, g, prm, pool, ntypes, fx, fy, fz, partial)
+  if (energy) TB(true, false); else if (accum) TB(false, true); else TB(false, false);
+#undef TB
+}
+
+void launch_bond_tab(hipStream_t s, bool energy, bool accum, int n, const double4* pos, const int* id,
+                     const int* slot_of_id, int nid, const int* bstart, const int* bpartner, const Geom& g,
+                     const TabParam* prm, const double* pool, double* fx, double* fy, double* fz, double* partial) {
+  if (n <= 0) return;
+  dim3 grid(nblocks(n, 256)), block(256);
+#define BT(EN, AC) hipLaunchKernelGGL((k_bond_tab<EN, AC>), grid, block, 0, s, n, pos, id, slot_of_id, nid, bstart, bpartner, g, prm, pool, fx, fy, fz, partial)
+  if (energy) BT(true, false); else if (accum) BT(false, true); else BT(false, false);
+#undef BT
 }
 
 void launch_zero3(hipStream_t s, int n, double* a, double* b, double* c) {

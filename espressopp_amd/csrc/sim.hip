@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <numeric>
 #include <string>
@@ -104,8 +105,15 @@ struct FixedPairListState {
   bool dirty = true;
 };
 
+// Rows (r, e, f) of one interaction.Tabulated potential and its cutoff.
+struct HostTable {
+  int itype = 0;
+  std::vector<double> r, e, f;
+  double cutoff = 0.0;
+};
+
 struct InteractionState {
-  int kind = 0;  // 0: VerletList LJ, 1: FixedPairList FENE
+  int kind = 0;  // 0: VerletList LJ, 1: FixedPairList FENE, 2: VerletList Tabulated, 3: FixedPairList Tabulated
   int list = 0;
   std::vector<LJParam> tab = std::vector<LJParam>(kMaxTypes * kMaxTypes);
   int maxset = -1;       // largest type with a potential
@@ -113,7 +121,156 @@ struct InteractionState {
   int dtab_nt = 0;
   bool dirty = true;
   FENEParam fp{};
+  // Tabulated: tables per type pair (kind 2) or tables[0] (kind 3)
+  std::vector<std::shared_ptr<HostTable>> tables = std::vector<std::shared_ptr<HostTable>>(kMaxTypes * kMaxTypes);
+  DBuf<TabParam> ttab;
+  DBuf<double> tpool;
 };
+
+// ------------------------------------------------------------------ tables
+// Per-interval coefficients of interaction.Tabulated's interpolation tables,
+// computed once on the host when a potential is set.
+
+// InterpolationLinear::spline (src/interaction/InterpolationLinear.cpp:139-150)
+void table_linear(const std::vector<double>& x, const std::vector<double>& y, std::vector<double>& a,
+                  std::vector<double>& b) {
+  const int N = (int)x.size();
+  a.assign((size_t)N, 0.0);
+  b.assign((size_t)N, 0.0);
+  for (int i = 0; i < N - 1; i++) {
+    a[(size_t)i] = (y[(size_t)i + 1] - y[(size_t)i]) / (x[(size_t)i + 1] - x[(size_t)i]);
+    b[(size_t)i] = y[(size_t)i] - a[(size_t)i] * x[(size_t)i];
+  }
+}
+
+// Clamped cubic spline second derivatives, tridiagonal sweep of
+// InterpolationCubic::spline (src/interaction/InterpolationCubic.cpp:170-215);
+// an end slope above 0.99e30 selects the natural end.
+void table_cubic(const std::vector<double>& x, const std::vector<double>& y, double yp1, double ypn,
+                 std::vector<double>& y2) {
+  const int n = (int)x.size();
+  std::vector<double> u((size_t)n, 0.0);
+  y2.assign((size_t)n, 0.0);
+  if (yp1 > 0.99e30) {
+    y2[0] = 0.0;
+    u[0] = 0.0;
+  } else {
+    y2[0] = -0.5;
+    u[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - yp1);
+  }
+  for (int i = 1; i < n - 1; i++) {
+    const size_t k = (size_t)i;
+    const double sig = (x[k] - x[k - 1]) / (x[k + 1] - x[k - 1]);
+    const double p = sig * y2[k - 1] + 2.0;
+    y2[k] = (sig - 1.0) / p;
+    u[k] = (y[k + 1] - y[k]) / (x[k + 1] - x[k]) - (y[k] - y[k - 1]) / (x[k] - x[k - 1]);
+    u[k] = (6.0 * u[k] / (x[k + 1] - x[k - 1]) - sig * u[k - 1]) / p;
+  }
+  double qn, un;
+  const size_t m = (size_t)n;
+  if (ypn > 0.99e30) {
+    qn = 0.0;
+    un = 0.0;
+  } else {
+    qn = 0.5;
+    un = (3.0 / (x[m - 1] - x[m - 2])) * (ypn - (y[m - 1] - y[m - 2]) / (x[m - 1] - x[m - 2]));
+  }
+  y2[m - 1] = (un - qn * u[m - 2]) / (qn * y2[m - 2] + 1.0);
+  for (int k = n - 2; k >= 0; k--) y2[(size_t)k] = y2[(size_t)k] * y2[(size_t)k + 1] + u[(size_t)k];
+}
+
+// InterpolationAkima::getSlope (src/interaction/InterpolationAkima.hpp:108-120)
+double akima_slope(double m1, double m2, double m3, double m4) {
+  if ((m1 == m2) && (m3 == m4)) return (m2 + m3) / 2.0;
+  return (std::fabs(m4 - m3) * m2 + std::fabs(m2 - m1) * m3) / (std::fabs(m4 - m3) + std::fabs(m2 - m1));
+}
+
+// Akima coefficients with two parabola-extrapolated points on each side
+// (InterpolationAkima::spline, src/interaction/InterpolationAkima.cpp:136-214); N >= 4.
+void table_akima(const std::vector<double>& x, const std::vector<double>& y, std::vector<double>& p0,
+                 std::vector<double>& p1, std::vector<double>& p2, std::vector<double>& p3) {
+  const size_t N = x.size();
+  std::vector<double> t(N, 0.0);
+  double temp = (x[1] - x[0]) / (x[2] - x[0]);
+  temp = temp * temp;
+  double g0 = y[0];
+  double g1 = ((y[1] - y[0]) - temp * (y[2] - y[0])) / ((x[1] - x[0]) - temp * (x[2] - x[0]));
+  double g2 = ((y[2] - y[0]) - g1 * (x[2] - x[0])) / ((x[2] - x[0]) * (x[2] - x[0]));
+  const double x1 = x[0] - (x[2] - x[0]);
+  const double x2 = x[1] - (x[2] - x[0]);
+  const double y1 = g0 + g1 * (x1 - x[0]) + g2 * (x1 - x[0]) * (x1 - x[0]);
+  const double y2 = g0 + g1 * (x2 - x[0]) + g2 * (x2 - x[0]) * (x2 - x[0]);
+  double m1 = (y2 - y1) / (x2 - x1);
+  double m2 = (y[0] - y2) / (x[0] - x2);
+  double m3 = (y[1] - y[0]) / (x[1] - x[0]);
+  double m4 = (y[2] - y[1]) / (x[2] - x[1]);
+  t[0] = akima_slope(m1, m2, m3, m4);
+  double m5 = (y[3] - y[2]) / (x[3] - x[2]);
+  t[1] = akima_slope(m2, m3, m4, m5);
+  temp = (x[N - 2] - x[N - 1]) / (x[N - 3] - x[N - 1]);
+  temp = temp * temp;
+  g0 = y[N - 1];
+  g1 = ((y[N - 2] - y[N - 1]) - temp * (y[N - 3] - y[N - 1])) / ((x[N - 2] - x[N - 1]) - temp * (x[N - 3] - x[N - 1]));
+  g2 = ((y[N - 3] - y[N - 1]) - g1 * (x[N - 3] - x[N - 1])) / ((x[N - 3] - x[N - 1]) * (x[N - 3] - x[N - 1]));
+  const double x4 = x[N - 2] + (x[N - 1] - x[N - 3]);
+  const double x5 = x[N - 1] + (x[N - 1] - x[N - 3]);
+  const double y4 = g0 + g1 * (x4 - x[N - 1]) + g2 * (x4 - x[N - 1]) * (x4 - x[N - 1]);
+  const double y5 = g0 + g1 * (x5 - x[N - 1]) + g2 * (x5 - x[N - 1]) * (x5 - x[N - 1]);
+  m1 = (y[N - 3] - y[N - 4]) / (x[N - 3] - x[N - 4]);
+  m2 = (y[N - 2] - y[N - 3]) / (x[N - 2] - x[N - 3]);
+  m3 = (y[N - 1] - y[N - 2]) / (x[N - 1] - x[N - 2]);
+  m4 = (y4 - y[N - 1]) / (x4 - x[N - 1]);
+  m5 = (y5 - y4) / (x5 - x4);
+  t[N - 2] = akima_slope(m1, m2, m3, m4);
+  t[N - 1] = akima_slope(m2, m3, m4, m5);
+  for (size_t i = 2; i + 2 < N; i++) {
+    m1 = (y[i - 1] - y[i - 2]) / (x[i - 1] - x[i - 2]);
+    m2 = (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+    m3 = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+    m4 = (y[i + 2] - y[i + 1]) / (x[i + 2] - x[i + 1]);
+    t[i] = akima_slope(m1, m2, m3, m4);
+  }
+  p0.assign(N, 0.0); p1.assign(N, 0.0); p2.assign(N, 0.0); p3.assign(N, 0.0);
+  for (size_t i = 0; i + 1 < N; i++) {
+    const double h = x[i + 1] - x[i];
+    p0[i] = y[i];
+    p1[i] = t[i];
+    p2[i] = (3.0 * (y[i + 1] - y[i]) / h - 2.0 * t[i] - t[i + 1]) / h;
+    p3[i] = (t[i] + t[i + 1] - 2.0 * (y[i + 1] - y[i]) / h) / (h * h);
+  }
+}
+
+// Pool rows {r_i, c0, c1, c2, c3} of one table: N force rows, then N energy rows.
+void table_rows(const HostTable& t, std::vector<double>& pool) {
+  const size_t N = t.r.size();
+  auto emit = [&](const std::vector<double>& y, const std::vector<double>* dy) {
+    std::vector<double> c0, c1, c2, c3;
+    if (t.itype == 1) {
+      table_linear(t.r, y, c0, c1);
+    } else if (t.itype == 2) {
+      table_akima(t.r, y, c0, c1, c2, c3);
+    } else {
+      // energy: end slopes are minus the tabulated forces; force: secant end slopes
+      std::vector<double> y2;
+      if (dy) table_cubic(t.r, y, -(*dy)[0], -(*dy)[N - 1], y2);
+      else table_cubic(t.r, y, (y[1] - y[0]) / (t.r[1] - t.r[0]), (y[N - 1] - y[N - 2]) / (t.r[N - 1] - t.r[N - 2]), y2);
+      c0 = y;
+      c1 = y2;
+    }
+    for (size_t i = 0; i < N; ++i) {
+      pool.push_back(t.r[i]);
+      if (t.itype == 3) {
+        const size_t j = std::min(i + 1, N - 1);
+        pool.push_back(c0[i]); pool.push_back(c1[i]); pool.push_back(c0[j]); pool.push_back(c1[j]);
+      } else {
+        pool.push_back(c0[i]); pool.push_back(c1[i]);
+        pool.push_back(c2.empty() ? 0.0 : c2[i]); pool.push_back(c3.empty() ? 0.0 : c3[i]);
+      }
+    }
+  };
+  emit(t.f, nullptr);
+  emit(t.e, &t.f);
+}
 
 }  // namespace
 
@@ -474,7 +631,80 @@ struct espp_sim {
     a.dirty = false;
   }
 
+  // Device tables of a Tabulated interaction: TabParam per type pair (kind 2) or
+  // one (kind 3), rows in a coefficient pool.
+  void upload_tab(InteractionState& a) {
+    const int nt = a.kind == 2 ? lj_ntypes(a) : 1;
+    if (nt > kMaxTypes) fail(ESPP_ERR_ARG, "at most " + std::to_string(kMaxTypes) + " particle types are supported");
+    if (!a.dirty && a.dtab_nt == nt) return;
+    sync();
+    std::vector<TabParam> prm((size_t)nt * nt);
+    std::vector<double> pool;
+    std::vector<std::pair<const HostTable*, int>> done;   // a table shared by (t1,t2) and (t2,t1) is stored once
+    const double qmax = std::max(g.qsd[0], std::max(g.qsd[1], g.qsd[2]));
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nt; ++j) {
+        TabParam p{};
+        const HostTable* t = a.tables[a.kind == 2 ? (size_t)i * kMaxTypes + j : 0].get();
+        if (t) {
+          int off = -1;
+          for (auto& d : done) if (d.first == t) off = d.second;
+          if (off < 0) {
+            off = (int)pool.size();
+            table_rows(*t, pool);
+            done.push_back({t, off});
+          }
+          const int N = (int)t->r.size();
+          p.itype = t->itype;
+          p.N = N;
+          p.off = off;
+          p.set = 1;
+          p.inner = t->r[0];
+          const double delta = (t->r[(size_t)N - 1] - t->r[0]) / (N - 1);
+          p.invdelta = 1.0 / delta;
+          p.deltasq6 = delta * delta / 6.0;
+          p.rc2 = t->cutoff * t->cutoff;
+          p.shift = 0.0;
+          const double win = std::isfinite(t->cutoff) ? 4.0 * std::sqrt(3.0) * qmax * t->cutoff + 1e-300 : 0.0;
+          p.dlo = std::isfinite(t->cutoff) ? p.rc2 - win : INFINITY;
+          p.dhi = std::isfinite(t->cutoff) ? p.rc2 + win : INFINITY;
+        }
+        prm[(size_t)i * nt + j] = p;
+      }
+    if (pool.empty()) pool.push_back(0.0);
+    a.ttab.resize(prm.size());
+    a.tpool.resize(pool.size());
+    hip_check(hipMemcpy(a.ttab.p, prm.data(), prm.size() * sizeof(TabParam), hipMemcpyHostToDevice), "table upload");
+    hip_check(hipMemcpy(a.tpool.p, pool.data(), pool.size() * sizeof(double), hipMemcpyHostToDevice), "table upload");
+    a.dtab_nt = nt;
+    a.dirty = false;
+  }
+
   void run_interaction(InteractionState& a, bool energy, bool accum, double* partial_out) {
+    if (a.kind == 2) {
+      VerletListState& v = *vls[(size_t)a.list];
+      upload_tab(a);
+      hipEvent_t pe = energy ? nullptr : prof_begin();
+      if (v.tiled)
+        launch_tab_tiled(stream, mixed(), energy, accum, (int)n, T, max_region * (mixed() ? 16 : 32), posq.p, pos.p,
+                         (const uint32_t*)v.nbr.p, v.cnt.p, v.cap / 2, g, runs.p, nruns.p, a.ttab.p, a.tpool.p,
+                         a.dtab_nt, fx.p, fy.p, fz.p, partial_out);
+      else
+        launch_tab(stream, energy, accum, (int)n, pos.p, v.nbr.p, v.cnt.p, v.cap, g, a.ttab.p, a.tpool.p, a.dtab_nt,
+                   fx.p, fy.p, fz.p, partial_out);
+      prof_end(0, pe);
+      return;
+    }
+    if (a.kind == 3) {
+      FixedPairListState& f = *fpls[(size_t)a.list];
+      if (f.dirty || f.built_nid != nid) upload_bonds(f);
+      upload_tab(a);
+      hipEvent_t pe = energy ? nullptr : prof_begin();
+      launch_bond_tab(stream, energy, accum, (int)n, pos.p, id.p, slot_of_id.p, nid, f.bstart.p, f.bpartner.p, g,
+                      a.ttab.p, a.tpool.p, fx.p, fy.p, fz.p, partial_out);
+      prof_end(5, pe);
+      return;
+    }
     if (a.kind == 0) {
       VerletListState& v = *vls[(size_t)a.list];
       upload_lj(a);
@@ -530,7 +760,7 @@ struct espp_sim {
 
   void interaction_sums(InteractionState& a, double out8[8]) {
     ensure_lists();
-    bool tiled = a.kind == 0 && vls[(size_t)a.list]->tiled;
+    bool tiled = (a.kind == 0 || a.kind == 2) && vls[(size_t)a.list]->tiled;
     int nblk = std::max(tiled ? lj_tiled_blocks((int)n, T) : lj_blocks((int)n), 1);
     partial.resize((size_t)nblk * 8);
     if (n == 0) { for (int q = 0; q < 8; ++q) out8[q] = 0.0; return; }
@@ -1142,6 +1372,63 @@ int espp_fene_create(espp_sim* s, int32_t fpl, double K, double r0, double rMax,
     a->fp.rMaxSqr = rMax * rMax;
     a->fp.cutoffSqr = cutoff * cutoff;
     a->fp.shift = shift;
+    s->ias.push_back(a);
+    *ia = (int32_t)s->ias.size() - 1;
+  });
+}
+
+int espp_tab_create(espp_sim* s, int32_t vl, int32_t* ia) {
+  return guarded([&] {
+    get_vl(s, vl);
+    need(ia != nullptr, "NULL argument");
+    auto* a = new InteractionState();
+    a->kind = 2;
+    a->list = vl;
+    s->ias.push_back(a);
+    *ia = (int32_t)s->ias.size() - 1;
+  });
+}
+
+static std::shared_ptr<HostTable> make_table(int32_t itype, int64_t n, const double* r, const double* e,
+                                             const double* f, double cutoff) {
+  need(itype >= 1 && itype <= 3, "interpolation type must be 1 (linear), 2 (Akima) or 3 (cubic)");
+  need(r && e && f, "NULL argument");
+  need(n >= 2 && (itype != 2 || n >= 4), "illegal file for tabulated potential");
+  need(n < (int64_t)1 << 24, "table too long");
+  need(r[n - 1] > r[0], "illegal distance for entries");
+  auto t = std::make_shared<HostTable>();
+  t->itype = itype;
+  t->r.assign(r, r + n);
+  t->e.assign(e, e + n);
+  t->f.assign(f, f + n);
+  t->cutoff = cutoff;
+  return t;
+}
+
+int espp_tab_set(espp_sim* s, int32_t ia, int32_t t1, int32_t t2, int32_t itype, int64_t n, const double* r,
+                 const double* e, const double* f, double cutoff) {
+  return guarded([&] {
+    InteractionState& a = get_ia(s, ia);
+    need(a.kind == 2, "not a VerletList Tabulated interaction");
+    need(t1 >= 0 && t2 >= 0 && t1 < kMaxTypes && t2 < kMaxTypes, "particle type out of range");
+    auto t = make_table(itype, n, r, e, f, cutoff);
+    a.tables[(size_t)t1 * kMaxTypes + t2] = t;
+    a.tables[(size_t)t2 * kMaxTypes + t1] = t;
+    a.maxset = std::max(a.maxset, std::max(t1, t2));
+    a.dirty = true;
+  });
+}
+
+int espp_tabbond_create(espp_sim* s, int32_t fpl, int32_t itype, int64_t n, const double* r, const double* e,
+                        const double* f, double cutoff, int32_t* ia) {
+  return guarded([&] {
+    need(s && ia, "NULL argument");
+    need(fpl >= 0 && fpl < (int32_t)s->fpls.size(), "no such fixed pair list");
+    auto t = make_table(itype, n, r, e, f, cutoff);
+    auto* a = new InteractionState();
+    a->kind = 3;
+    a->list = fpl;
+    a->tables[0] = t;
     s->ias.push_back(a);
     *ia = (int32_t)s->ias.size() - 1;
   });

@@ -1,4 +1,4 @@
-"""espressopp.interaction: LennardJones, FENE and their list interactions.
+"""espressopp.interaction: LennardJones, FENE, Tabulated and their list interactions.
 
 Reference: src/interaction/LennardJones.py:721-750 (LennardJones(epsilon=1,
 sigma=1, cutoff=inf, shift='auto'), VerletListLennardJones),
@@ -12,6 +12,7 @@ energies and virials of a system are always computed by the GPU kernels.
 """
 import ctypes
 import math
+import re
 
 import numpy as np
 
@@ -259,6 +260,144 @@ class FixedPairListFENE(Interaction):
 
     def setFixedPairList(self, fpl):
         raise NotImplementedError("the bond list is fixed at construction on the GPU storage")
+
+    def getMaxCutoff(self):
+        return self._potential.cutoff if not math.isinf(self._potential.cutoff) else 0.0
+
+
+# ---------------------------------------------------------------- Tabulated
+_NUM = re.compile(r"[ \t\n\r\f\v]*([+-]?(?:inf(?:inity)?|nan|(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?))",
+                  re.IGNORECASE)
+
+
+def read_table(filename):
+    """Rows (r, e, f) of a table file, read as the reference reads it
+    (InterpolationTable::readFile, src/interaction/InterpolationTable.cpp:106-140):
+    a line counts when its first three whitespace-separated fields parse as
+    numbers (sscanf "%lg %lg %lg" == 3); every other line is skipped."""
+    rows = []
+    with open(filename) as fh:
+        for line in fh:
+            vals, pos = [], 0
+            for _ in range(3):
+                m = _NUM.match(line, pos)
+                if not m:
+                    break
+                vals.append(float(m.group(1)))
+                pos = m.end()
+            if len(vals) == 3:
+                rows.append(vals)
+    a = np.asarray(rows, dtype=np.float64).reshape(-1, 3)
+    return np.ascontiguousarray(a[:, 0]), np.ascontiguousarray(a[:, 1]), np.ascontiguousarray(a[:, 2])
+
+
+class Tabulated(Potential):
+    """interaction.Tabulated(itype, filename, cutoff) (src/interaction/Tabulated.hpp:43-115,
+    src/interaction/Tabulated.py).  itype 1 linear, 2 Akima, 3 cubic spline
+    interpolation of the file's (r, e, f) rows; shift 0.  The table is read when
+    the potential is created; energies and forces of a system are computed by
+    the GPU kernels."""
+
+    def __init__(self, itype, filename, cutoff=infinity):
+        if int(itype) not in (1, 2, 3):
+            raise ValueError("itype must be 1 (linear), 2 (Akima) or 3 (cubic)")
+        self._itype = int(itype)
+        self._filename = str(filename)
+        self._r, self._e, self._f = read_table(self._filename)
+        if len(self._r) < 2 or (self._itype == 2 and len(self._r) < 4):
+            raise RuntimeError("illegal file for tabulated potential")
+        Potential.__init__(self, cutoff, 0.0)
+
+    @property
+    def filename(self):
+        return self._filename
+
+    @property
+    def itype(self):
+        return self._itype
+
+    def getInterpolationType(self):
+        return self._itype
+
+    def _update(self):
+        for o in self._owners:
+            o._potential_changed(self)
+
+    def table(self):
+        return self._r, self._e, self._f
+
+    def _energy_raw(self, d2):
+        raise NotImplementedError("single-distance evaluation of a Tabulated potential is not provided; "
+                                  "use the interaction's computeEnergy")
+
+    _force_factor = _energy_raw
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class VerletListTabulated(Interaction):
+    """VerletListInteractionTemplate<Tabulated> (src/interaction/Tabulated.cpp:73,100-103)."""
+
+    _bond_type = 1   # Nonbonded
+
+    def __init__(self, vl):
+        self._vl = vl
+        self._h = vl._h
+        self._id = self._h.out_int("espp_tab_create", vl._id)
+        self._potentials = {}
+
+    def setPotential(self, type1, type2, potential):
+        if not isinstance(potential, Tabulated):
+            raise TypeError("VerletListTabulated needs a Tabulated potential")
+        self._potentials[(type1, type2)] = potential
+        self._potentials[(type2, type1)] = potential
+        if self not in potential._owners:
+            potential._owners.append(self)
+        self._upload(type1, type2, potential)
+
+    def _upload(self, t1, t2, p):
+        r, e, f = p.table()
+        self._h.call("espp_tab_set", self._id, int(t1), int(t2), p.itype, len(r), _ptr(r), _ptr(e), _ptr(f),
+                     p.cutoff)
+
+    def _potential_changed(self, p):
+        for (t1, t2), q in self._potentials.items():
+            if q is p and t1 <= t2:
+                self._upload(t1, t2, p)
+
+    def getPotential(self, type1, type2):
+        return self._potentials.get((type1, type2))
+
+    def getVerletList(self):
+        return self._vl
+
+    def getMaxCutoff(self):
+        cs = [p.cutoff for p in self._potentials.values()]
+        return max(cs) if cs else 0.0
+
+
+class FixedPairListTabulated(Interaction):
+    """FixedPairListInteractionTemplate<Tabulated> (src/interaction/Tabulated.cpp:85,182-190)."""
+
+    _bond_type = 2   # Pair
+
+    def __init__(self, system, fpl, potential):
+        if not isinstance(potential, Tabulated):
+            raise TypeError("FixedPairListTabulated needs a Tabulated potential")
+        self._fpl = fpl
+        self._h = fpl._h
+        self._potential = potential
+        r, e, f = potential.table()
+        self._id = self._h.out_int("espp_tabbond_create", fpl._id, potential.itype, len(r), _ptr(r), _ptr(e),
+                                   _ptr(f), potential.cutoff)
+
+    def getPotential(self):
+        return self._potential
+
+    def getFixedPairList(self):
+        return self._fpl
 
     def getMaxCutoff(self):
         return self._potential.cutoff if not math.isinf(self._potential.cutoff) else 0.0

@@ -144,6 +144,22 @@ int espp_lj_set(espp_sim *sim, int32_t ia, int32_t type1, int32_t type2, double 
 int espp_fene_create(espp_sim *sim, int32_t fpl, double K, double r0, double rMax,
                      double cutoff, double shift, int32_t *ia);
 
+/* interaction.VerletListTabulated(vl) (src/interaction/Tabulated.cpp:73,100-103,
+ * VerletListInteractionTemplate<Tabulated>). */
+int espp_tab_create(espp_sim *sim, int32_t vl, int32_t *ia);
+
+/* VerletListTabulated::setPotential(type1, type2, Tabulated(itype, filename, cutoff))
+ * (src/interaction/Tabulated.hpp:56-115, Tabulated.cpp:60-80).  The table is given as
+ * its n rows (r, e, f) on a uniform grid, as the reference reads them from the file
+ * (InterpolationLinear.cpp:89-125); itype 1 linear, 2 Akima, 3 cubic spline. */
+int espp_tab_set(espp_sim *sim, int32_t ia, int32_t type1, int32_t type2, int32_t itype, int64_t n,
+                 const double *r, const double *e, const double *f, double cutoff);
+
+/* interaction.FixedPairListTabulated(system, fpl, Tabulated(itype, filename, cutoff))
+ * (src/interaction/Tabulated.cpp:85,182-190, FixedPairListInteractionTemplate<Tabulated>). */
+int espp_tabbond_create(espp_sim *sim, int32_t fpl, int32_t itype, int64_t n, const double *r,
+                        const double *e, const double *f, double cutoff, int32_t *ia);
+
 /* System::addInteraction (src/System.cpp:75-85, bound at :221): appends the
  * interaction to the short-range list whose order is the force summation order. */
 int espp_add_interaction(espp_sim *sim, int32_t ia);

@@ -374,3 +374,108 @@ def test_pairs_and_forces_262k(gpu, precision):
     err = force_rel_err(system.storage.getParticleArrays()["f"], ref.get("f"), 0.1)
     assert err.max() <= (1e-12 if precision == "fp64" else 1e-5)
     assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=1e-12 if precision == "fp64" else 1e-6)
+
+
+def _write_table(path, r, e, f, junk=True):
+    with open(path, "w") as fh:
+        if junk:
+            fh.write("# r energy force\n\n")
+        for a, b, c in zip(r, e, f):
+            fh.write("%.17g %.17g %.17g\n" % (a, b, c))
+
+
+def _lj_rows(N, lo, hi, eps, sig, rc):
+    from oracle import oracle
+    r = np.linspace(lo, hi, N)
+    e = np.array([oracle.lj_energy(eps, sig, rc, 0.0, x) for x in r])
+    f = np.array([oracle.lj_force(eps, sig, float("inf"), [x, 0.0, 0.0])[0] for x in r])
+    return r, e, f
+
+
+@pytest.mark.parametrize("precision,itype,cells", [("fp64", 1, None), ("fp64", 2, None), ("fp64", 3, None),
+                                                   ("mixed", 3, None), ("mixed", 1, None), ("fp64", 3, (2, 2, 2)),
+                                                   ("mixed", 2, (2, 2, 2))])
+def test_verlet_list_tabulated(gpu, tmp_path, precision, itype, cells):
+    # VerletListTabulated against the oracle's Tabulated on the same table files, two types
+    pid, pos, _, box = read_fixture()
+    rc, skin = 2.5, 0.3
+    cells = cells or cells_for(box, rc, skin)
+    types = (pid % 2).astype(np.int32)
+    tabs = {}
+    for (t1, t2, eps, sig) in ((0, 0, 1.0, 1.0), (0, 1, 0.8, 0.9), (1, 1, 1.2, 1.1)):
+        r, e, f = _lj_rows(513, 0.6, rc, eps, sig, rc)
+        fn = str(tmp_path / ("t%d%d.tab" % (t1, t2)))
+        _write_table(fn, r, e, f)
+        tabs[(t1, t2)] = (fn, r, e, f)
+    system = gpu_system(box, cells, skin, precision)
+    system.storage.addParticlesArray(pid, pos, types)
+    system.storage.decompose()
+    vl = espressopp.VerletList(system, cutoff=rc)
+    ia = espressopp.interaction.VerletListTabulated(vl)
+    for (t1, t2), (fn, *_r) in tabs.items():
+        ia.setPotential(t1, t2, espressopp.interaction.Tabulated(itype, fn, rc))
+    system.addInteraction(ia)
+    espressopp.integrator.VelocityVerlet(system).run(0)
+    ref = OracleSim(box, cells, skin)
+    ref.add_particles(pid, pos, types=types)
+    ref.decompose()
+    ria = ref.tab_create(ref.vl_create(rc))
+    for (t1, t2), (fn, r, e, f) in tabs.items():
+        ref.tab_set(ria, t1, t2, itype, r, e, f, cutoff=rc)
+    ref.calc_forces()
+    f = system.storage.getParticleArrays()["f"]
+    # mixed: distances from the 32-bit fixed-point mirror (quantisation ~L/2^32), the LJ bars
+    tol, etol = (1e-12, 1e-11) if precision == "fp64" else (1e-5, 1e-6)
+    assert force_rel_err(f, ref.get("f"), 0.1).max() <= tol
+    assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=etol)
+    w, t = ref.virial(ria)
+    assert ia.computeVirial() == pytest.approx(w, rel=10 * etol)
+    if precision == "fp64":
+        assert np.allclose(ia.computeVirialTensor(), t, rtol=1e-9, atol=1e-8)
+
+
+@pytest.mark.parametrize("itype", [1, 3])
+def test_fixed_pair_list_tabulated(gpu, tmp_path, itype):
+    # FixedPairListTabulated (a FENE table) plus a WCA table on a chain melt, against the oracle
+    from oracle import oracle
+    pos, box, bonds = random_walk_chains(20, 40, 0.85, seed=31)
+    n = len(pos)
+    rc, skin = 2 ** (1 / 6), 0.3
+    cells = cells_for(box, rc, skin)
+    rb = np.linspace(0.0001, 1.49, 1491)
+    eb = np.array([oracle.fene_energy(30.0, 0.0, 1.5, x) for x in rb])
+    fb = np.array([oracle.fene_force(30.0, 0.0, 1.5, [x, 0.0, 0.0])[0] for x in rb])
+    rw, ew, fw = _lj_rows(257, 0.5, rc, 1.0, 1.0, rc)
+    ew = ew + 1.0                                     # WCA: LJ shifted by +eps at 2^(1/6)
+    fnb, fnw = str(tmp_path / "fene.tab"), str(tmp_path / "wca.tab")
+    _write_table(fnb, rb, eb, fb)
+    _write_table(fnw, rw, ew, fw, junk=False)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        system = gpu_system(box, cells, skin, precision)
+        system.storage.addParticlesArray(np.arange(n), pos)
+        system.storage.decompose()
+        vl = espressopp.VerletList(system, cutoff=rc)
+        wca = espressopp.interaction.VerletListTabulated(vl)
+        wca.setPotential(0, 0, espressopp.interaction.Tabulated(itype, fnw, rc))
+        system.addInteraction(wca)
+        fpl = espressopp.FixedPairList(system.storage)
+        fpl.addBonds(bonds.tolist())
+        bond = espressopp.interaction.FixedPairListTabulated(system, fpl, espressopp.interaction.Tabulated(itype, fnb))
+        system.addInteraction(bond)
+        espressopp.integrator.VelocityVerlet(system).run(0)
+        out[precision] = (system.storage.getParticleArrays()["f"], wca.computeEnergy(), bond.computeEnergy())
+    ref = OracleSim(box, cells, skin)
+    ref.add_particles(np.arange(n), pos)
+    ref.decompose()
+    rw_ia = ref.tab_create(ref.vl_create(rc))
+    ref.tab_set(rw_ia, 0, 0, itype, rw, ew, fw, cutoff=rc)
+    rfpl = ref.fpl_create()
+    ref.fpl_add(rfpl, bonds)
+    rb_ia = ref.tabbond_create(rfpl, itype, rb, eb, fb)
+    ref.calc_forces()
+    for precision, tol, etol in (("fp64", 1e-12, 1e-11), ("mixed", 1e-5, 1e-6)):
+        f, ew_gpu, eb_gpu = out[precision]
+        assert force_rel_err(f, ref.get("f"), 0.1).max() <= tol, precision
+        assert ew_gpu == pytest.approx(ref.energy(rw_ia), rel=etol)
+        assert eb_gpu == pytest.approx(ref.energy(rb_ia), rel=etol)

@@ -71,3 +71,27 @@ def test_run_requires_integer_steps():
             self._dt = 0.005
     with pytest.raises(ValueError):
         Dummy().run(1.5)
+
+
+def test_read_table_follows_sscanf_rule(tmp_path):
+    # InterpolationTable::readFile: a line counts when sscanf("%lg %lg %lg") reads three numbers
+    from espressopp_amd.interaction import read_table
+    fn = tmp_path / "t.tab"
+    fn.write_text("# header\n\n0.5 1.0 2.0\n1.0e0 -2 +3.5 trailing words\n1.5 2.0\n2.0 x 3.0\n"
+                  "  2.5\t-1e-3   4E2\n3.0 1.0 2.0abc\n")
+    r, e, f = read_table(str(fn))
+    assert list(r) == [0.5, 1.0, 2.5, 3.0]
+    assert list(e) == [1.0, -2.0, -1e-3, 1.0]
+    assert list(f) == [2.0, 3.5, 400.0, 2.0]
+
+
+def test_tabulated_potential_checks_its_table(tmp_path):
+    import espressopp_amd as espressopp
+    fn = tmp_path / "short.tab"
+    fn.write_text("0.5 1.0 2.0\n1.0 0.5 1.0\n1.5 0.2 0.4\n")
+    espressopp.interaction.Tabulated(1, str(fn), 2.0)
+    espressopp.interaction.Tabulated(3, str(fn), 2.0)
+    with pytest.raises(RuntimeError):
+        espressopp.interaction.Tabulated(2, str(fn), 2.0)     # Akima needs four rows
+    with pytest.raises(ValueError):
+        espressopp.interaction.Tabulated(4, str(fn), 2.0)
