@@ -5,16 +5,19 @@ espressopp_amd as espressopp`` and the classes System, storage.DomainDecompositi
 VerletList, FixedPairList, interaction.LennardJones / VerletListLennardJones /
 FENE / FixedPairListFENE, integrator.VelocityVerlet / LangevinThermostat and
 analysis.Temperature / Pressure run the step on one GPU through the C ABI of
-libespp_amd.so (include/espp_amd.h).
+libespp_amd.so (include/espp_amd.h); interaction.Tabulated and
+ParallelTempering cover the tabulated potentials and replica exchange.
 """
 from . import _lib
 from ._vec3 import Int3D, Real3D, toInt3DFromVector, toReal3DFromVector
 from .system import MPI, FixedPairList, System, VerletList
 from . import analysis, bc, esutil, integrator, interaction, storage, tools
+from .parallel_tempering import MultiSystem, ParallelTempering
 
 infinity = float("inf")
 nan = float("nan")
 
-__all__ = ["System", "VerletList", "FixedPairList", "Real3D", "Int3D", "MPI", "analysis", "bc",
+__all__ = ["System", "VerletList", "FixedPairList", "ParallelTempering", "MultiSystem", "Real3D", "Int3D", "MPI",
+           "analysis", "bc",
            "esutil", "integrator", "interaction", "storage", "tools", "infinity",
            "toReal3DFromVector", "toInt3DFromVector"]

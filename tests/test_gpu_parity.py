@@ -479,3 +479,55 @@ def test_fixed_pair_list_tabulated(gpu, tmp_path, itype):
         assert force_rel_err(f, ref.get("f"), 0.1).max() <= tol, precision
         assert ew_gpu == pytest.approx(ref.energy(rw_ia), rel=etol)
         assert eb_gpu == pytest.approx(ref.energy(rb_ia), rel=etol)
+
+
+def test_parallel_tempering_replicas(gpu):
+    # replicas of uneven size advance concurrently, each on its own stream; every replica ends
+    # bit-identical to the same replica advanced alone, and exchange() follows the rule on
+    # the replicas' own observables
+    sizes, temps = [600, 1100, 1700, 2300], [0.7, 0.9, 1.1, 1.3]
+    LJ = espressopp.interaction.LennardJones
+
+    def make(i):
+        pos, box = jittered_lattice(sizes[i], 0.8442, seed=40 + i)
+        system = gpu_system(box, cells_for(box, 2.5, 0.3), 0.3, "mixed", seed=100 + i)
+        system.storage.addParticlesArray(np.arange(sizes[i]), pos)
+        system.storage.decompose()
+        vl = espressopp.VerletList(system, cutoff=2.5)
+        lj = espressopp.interaction.VerletListLennardJones(vl)
+        lj.setPotential(0, 0, LJ(1.0, 1.0, 2.5, "auto"))
+        system.addInteraction(lj)
+        integ = espressopp.integrator.VelocityVerlet(system)
+        integ.dt = 0.005
+        lang = espressopp.integrator.LangevinThermostat(system)
+        lang.gamma = 1.0
+        lang.temperature = temps[i]
+        integ.addExtension(lang)
+        return system, lj, integ, lang
+
+    pt = espressopp.ParallelTempering(NumberOfSystems=4, RNG=np.random.default_rng(7))
+    reps = []
+    for i in range(4):
+        pt.startDefiningSystem(i)
+        system, lj, integ, lang = make(i)
+        pt.setIntegrator(integ, lang)
+        pt.setAnalysisE(lj)
+        pt.setAnalysisT(espressopp.analysis.Temperature(system))
+        pt.setAnalysisNPart(espressopp.analysis.NPart(system))
+        pt.endDefiningSystem(i)
+        reps.append((system, lj, lang))
+    pt.run(40)
+    for i in range(4):
+        alone, _, integ, _ = make(i)
+        integ.run(40)
+        assert np.array_equal(alone.storage.getParticleArrays()["pos"], reps[i][0].storage.getParticleArrays()["pos"])
+    E = [r[1].computeEnergy() for r in reps]
+    T = [espressopp.analysis.Temperature(r[0]).compute() for r in reps]
+    swaps = pt.exchange()
+    for m, n, d, u, yes in pt.log:
+        assert d == pytest.approx((1 / T[n] - 1 / T[m]) * (E[m] / sizes[m] - E[n] / sizes[n]), rel=1e-12)
+        assert yes == (d <= 0 or math.exp(-d) > u)
+        if yes:
+            assert reps[n][2].temperature == pytest.approx(T[m]) and reps[m][2].temperature == pytest.approx(T[n])
+    assert [(m, n) for m, n, _, _, y in pt.log if y] == swaps
+    pt.run(10)
