@@ -35,12 +35,12 @@ void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum,
 void launch_tile_plan(hipStream_t s, int n, int T, int ntiles, const int* cell_of_slot, const int* cstart,
                       const Geom& g, int budget, TileRun* runs, int* nruns, Ctl* ctl);
 void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Geom& g);
-void launch_nbr_build_tiled(hipStream_t s, int n, int T, int lds_bytes, const uint4* posq, const double4* pos,
+void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
-                            const int* ex_ids, uint32_t* nbr, int* cnt, int capw, Ctl* ctl);
+                            const int* ex_ids, uint4* nbr, int* cnt, int capg, Ctl* ctl);
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
-                     const uint4* posq, const double4* pos, const uint32_t* nbr, const int* cnt, int capw,
+                     const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                      const Geom& g, const TileRun* runs, const int* nruns, const LJParam* prm, int ntypes,
                      double* fx, double* fy, double* fz, double* partial);
 int lj_tiled_blocks(int n, int T);
@@ -49,7 +49,7 @@ void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, cons
                  const int* bstart, const int* bpartner, const Geom& g, const FENEParam& fp,
                  double* fx, double* fy, double* fz, double* partial);
 void launch_tab_tiled(hipStream_t s, bool mixed, bool energy, bool accum, int n, int T, int lds_bytes,
-                      const uint4* posq, const double4* pos, const uint32_t* nbr, const int* cnt, int capw,
+                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                       const Geom& g, const TileRun* runs, const int* nruns, const TabParam* prm, const double* pool,
                       int ntypes, double* fx, double* fy, double* fz, double* partial);
 void launch_tab(hipStream_t s, bool energy, bool accum, int n, const double4* pos, const int* nbr, const int* cnt,

@@ -662,8 +662,7 @@ __device__ __forceinline__ int slot_of_lds(const TileRun* sr, int nr, int e) {
 
 template <typename V>
 __device__ __forceinline__ int stage_tile(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
-                                          const V* __restrict__ src, V* lds, TileRun* sr) {
-  const int t = blockIdx.x;
+                                          const V* __restrict__ src, V* lds, TileRun* sr, int t) {
   const int nr = nruns[t];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)t * kMaxRuns + threadIdx.x];
   __syncthreads();
@@ -704,8 +703,7 @@ __device__ __forceinline__ uint3 narrow_slot<uint3>(const uint4& v) { return mak
 // stage_tile for the fixed-point mirror, storing S (uint3 or uint4) per particle
 template <typename S>
 __device__ __forceinline__ int stage_tile_q(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
-                                            const uint4* __restrict__ src, S* lds, TileRun* sr) {
-  const int t = blockIdx.x;
+                                            const uint4* __restrict__ src, S* lds, TileRun* sr, int t) {
   const int nr = nruns[t];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)t * kMaxRuns + threadIdx.x];
   __syncthreads();
@@ -735,28 +733,55 @@ __device__ __forceinline__ int stage_tile_q(const TileRun* __restrict__ runs, co
   return nr;
 }
 
+// d.sqr() <= cutsq of Real3D (src/Real3D.hpp), evaluated left to right without contraction
+__device__ __forceinline__ bool exact_sqr_le(double ex, double ey, double ez, double cutsq) {
+#pragma clang fp contract(off)
+  return ex * ex + ey * ey + ez * ez <= cutsq;
+}
+
+// Workgroup -> tile map that keeps consecutive tiles on one XCD: workgroups are
+// dispatched round-robin over the 8 XCDs, so XCD x receives workgroups x, x+8,
+// ...; those take tiles [x*q + min(x, r) ...) in order (ntiles = 8q + r).
+// Neighbouring tiles stage overlapping regions, which then hit the same L2.
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+  const int q = ntiles / kXcd, r = ntiles % kXcd;
+  const int x = b % kXcd, k = b / kXcd;
+  return x * q + min(x, r) + k;
+}
+
+// Tiled list layout: entry q of particle i is 16-bit half (q & 1) of word
+// ((q >> 1) & 3) of the uint4 at ((i >> 6) * capg + (q >> 3)) * 64 + (i & 63);
+// a lane reads or writes eight entries with one 16-byte access and the 64 lanes
+// of a wavefront touch 1 KiB of consecutive memory.
+__device__ __forceinline__ size_t tiled_group(int i, int capg, int g) {
+  return ((size_t)(i >> 6) * (size_t)capg + (size_t)g) * 64 + (size_t)(i & 63);
+}
+
 // Tiled Verlet-list build (cell grids with at least 3 cells per dimension).
 // For each of its 9 neighbour x-rows a particle scans the candidates of the
 // three cells (cx-1, cx, cx+1), one contiguous slot range or two when the row
 // wraps, read from the fixed-point mirror (lanes of one cell read the same
-// candidates, so the loads broadcast).  Acceptance bits of 32 candidates are
-// collected first and appended afterwards.  Pairs whose FP32 r^2 is within a
-// small window of (rc+skin)^2 are settled by the reference's FP64 test in the
-// reference's orientation (see k_nbr_build); all others by the FP32 value.
-// Entries are the candidate's offset in the tile's staged region.
-template <bool HAS_EXCL>
+// candidates, so the loads broadcast; the mirror is padded so reads past a
+// range stay inside it).  For 32 candidates at a time two bit masks are formed
+// without branches: FP32 r^2 (fixed-point units) below the window (surely in)
+// and below its top (in or undecided).  Undecided pairs, within a small window
+// of (rc+skin)^2, are settled by the reference's FP64 test in the reference's
+// orientation (see k_nbr_build).  Accepted candidates are appended as their
+// offset in the tile's staged region, eight entries per 16-byte store.
+template <bool HAS_EXCL, bool CUBIC>
 __global__ __launch_bounds__(256) void k_nbr_build_tiled(
-    int n, const uint4* __restrict__ posq, const double4* __restrict__ pos, const int* __restrict__ id,
+    int n, int ntiles, const uint4* __restrict__ posq, const double4* __restrict__ pos, const int* __restrict__ id,
     const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g, double cutsq, float cut_lo,
     float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,  // cut_lo/hi in units of qs[0]^2
-    const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint32_t* __restrict__ nbr,
-    int* __restrict__ cnt, int capw, Ctl* ctl) {
-#pragma clang fp contract(off)
+    const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint4* __restrict__ nbr,
+    int* __restrict__ cnt, int capg, Ctl* ctl) {
   __shared__ TileRun sr[kMaxRuns];
-  const int nr = nruns[blockIdx.x];
-  if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)blockIdx.x * kMaxRuns + threadIdx.x];
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int nr = nruns[tile];
+  if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)tile * kMaxRuns + threadIdx.x];
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = tile * blockDim.x + threadIdx.x;
   int k = 0;
   if (i < n && nr > 0) {
     const uint4 qi = posq[i];
@@ -764,9 +789,10 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
     const int myid = HAS_EXCL ? id[i] : 0;
-    uint32_t* out = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
-    uint32_t pending = 0;
-    const int cap = 2 * capw;
+    uint4* out = nbr + tiled_group(i, capg, 0);
+    const int cap = 8 * capg;
+    // entries not yet stored: w0..w3 hold the current group, half-filled word in `pend`
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, pend = 0;
     const float ry = (float)(g.qsd[1] / g.qsd[0]), rz = (float)(g.qsd[2] / g.qsd[0]);
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
@@ -788,48 +814,60 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
           const TileRun R = sr[find_run_by_cell(sr, nr, ca)];
           const int base = R.lds_off + (jb - R.slot_start);
           for (int j0 = jb; j0 < je; j0 += 32) {
-            const int jn = min(32, je - j0);
-            uint32_t bits = 0;
+            const int jn = je - j0;
+            const uint4* cand = posq + j0;
+            uint32_t in_bits = 0, maybe_bits = 0;
+            // candidates 31 .. 0, so that candidate t ends at bit t of the masks
 #pragma unroll
-            for (int u = 0; u < 32; u += 4) {
+            for (int u = 28; u >= 0; u -= 4) {
               uint4 q[4];
 #pragma unroll
-              for (int v = 0; v < 4; ++v) q[v] = posq[j0 + min(u + v, jn - 1)];
+              for (int v = 0; v < 4; ++v) q[v] = cand[u + v];
 #pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                const int t = u + v;
+              for (int v = 3; v >= 0; --v) {
                 const float dx = (float)(int)(qi.x - q[v].x);
-                const float dy = (float)(int)(qi.y - q[v].y) * ry;
-                const float dz = (float)(int)(qi.z - q[v].z) * rz;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                bool acc = (t < jn) && (j0 + t != i) && (r2 <= cut_hi);
-                if (acc && r2 >= cut_lo) {
-                  // borderline: the reference's FP64 test in the reference's orientation
-                  const int j = j0 + t;
-                  const int cjx = cell_of_slot[j] % nx;
-                  int ox = cjx - cx, sx = 0;
-                  if (ox > 1) { ox = -1; sx = -1; } else if (ox < -1) { ox = 1; sx = 1; }
-                  const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
-                  const double4 pi = pos[i], pj = pos[j];
-                  const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
-                  double ex, ey, ez;
-                  if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
-                  else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
-                  acc = (ex * ex + ey * ey + ez * ez) <= cutsq;
-                }
-                bits |= (acc ? 1u : 0u) << t;
+                float dy = (float)(int)(qi.y - q[v].y);
+                float dz = (float)(int)(qi.z - q[v].z);
+                if (!CUBIC) { dy *= ry; dz *= rz; }
+                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                in_bits = (in_bits << 1) + (r2 < cut_lo ? 1u : 0u);
+                maybe_bits = (maybe_bits << 1) + (r2 <= cut_hi ? 1u : 0u);
               }
             }
-            while (bits) {
-              const int t = __ffs(bits) - 1;
-              bits &= bits - 1u;
+            uint32_t valid = jn >= 32 ? 0xFFFFFFFFu : ((1u << jn) - 1u);
+            const int self = i - j0;
+            if (self >= 0 && self < 32) valid &= ~(1u << self);
+            in_bits &= valid;
+            uint32_t border = maybe_bits & valid & ~in_bits;
+            while (border) {
+              // the reference's FP64 test in the reference's orientation (rare)
+              const int t = __ffs(border) - 1;
+              border &= border - 1u;
+              const int j = j0 + t;
+              const int cjx = cell_of_slot[j] % nx;
+              int ox = cjx - cx, sx = 0;
+              if (ox > 1) { ox = -1; sx = -1; } else if (ox < -1) { ox = 1; sx = 1; }
+              const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
+              const double4 pi = pos[i], pj = pos[j];
+              const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
+              double ex, ey, ez;
+              if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
+              else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
+              if (exact_sqr_le(ex, ey, ez, cutsq)) in_bits |= 1u << t;
+            }
+            while (in_bits) {
+              const int t = __ffs(in_bits) - 1;
+              in_bits &= in_bits - 1u;
               const int j = j0 + t;
               if (HAS_EXCL && is_excluded(myid, id[j], ex_start, ex_ids)) continue;
               const uint32_t e = (uint32_t)(base + (j - jb));
               if (k & 1) {
-                if (k < cap) out[(size_t)(k >> 1) * 64] = pending | (e << 16);
+                const uint32_t w = pend | (e << 16);
+                const int wi = (k >> 1) & 3;
+                w0 = wi == 0 ? w : w0; w1 = wi == 1 ? w : w1; w2 = wi == 2 ? w : w2; w3 = wi == 3 ? w : w3;
+                if (wi == 3 && k < cap) out[(size_t)(k >> 3) * 64] = make_uint4(w0, w1, w2, w3);
               } else {
-                pending = e;
+                pend = e;
               }
               ++k;
             }
@@ -837,7 +875,12 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
         }
       }
     }
-    if ((k & 1) && k < cap) out[(size_t)(k >> 1) * 64] = pending;
+    // last partial group; entries past the count are never read
+    if ((k & 7) != 0 && k < cap) {
+      const int wi = (k >> 1) & 3;
+      if (k & 1) { w0 = wi == 0 ? pend : w0; w1 = wi == 1 ? pend : w1; w2 = wi == 2 ? pend : w2; w3 = wi == 3 ? pend : w3; }
+      out[(size_t)(k >> 3) * 64] = make_uint4(w0, w1, w2, w3);
+    }
     cnt[i] = k;
     if (k > cap) ctl->overflow = 1;
   } else if (i < n) {
@@ -861,8 +904,8 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
 // afterwards, so the interacting set equals the FP64 path's.
 template <bool ONETYPE, bool ENERGY, bool ACCUM, bool CUBIC>
 __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
-    int n, const uint4* __restrict__ posq, const double4* __restrict__ pos,
-    const uint32_t* __restrict__ nbr, const int* __restrict__ cnt, int capw, Geom g,
+    int n, int ntiles, const uint4* __restrict__ posq, const double4* __restrict__ pos,
+    const uint4* __restrict__ nbr, const int* __restrict__ cnt, int capg, Geom g,
     const TileRun* __restrict__ runs, const int* __restrict__ nruns, const LJParam* __restrict__ prm,
     int ntypes, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
     double* __restrict__ partial) {
@@ -871,34 +914,30 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
   extern __shared__ uint4 lds_raw[];
   Slot* lds = reinterpret_cast<Slot*>(lds_raw);
   __shared__ TileRun sr[kMaxRuns];
-  const int nr = stage_tile_q<Slot>(runs, nruns, posq, lds, sr);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int nr = stage_tile_q<Slot>(runs, nruns, posq, lds, sr, tile);
+  const int i = tile * blockDim.x + threadIdx.x;
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
   if (i < n) {
     const uint4 qi = posq[i];
-    const uint32_t* np = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
-    const int c = nr > 0 ? min(cnt[i], 2 * capw) : 0;
+    const uint4* np = nbr + tiled_group(i, capg, 0);
+    const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
     const int rowi = (int)qi.w * ntypes;
     const LJParam& p0 = prm[0];
     double ff1 = CUBIC ? p0.uff1 : p0.dff1, ff2 = CUBIC ? p0.uff2 : p0.dff2;
     double rc2 = CUBIC ? p0.urc2 : p0.drc2, lo = CUBIC ? p0.ulo : p0.dlo, hi = CUBIC ? p0.uhi : p0.dhi;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
-    const int nw = (c + 1) >> 1;
-    uint32_t wn[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) wn[u] = (u < nw) ? np[(size_t)u * 64] : 0u;
-    for (int kw = 0; kw < nw; kw += 4) {
-      uint32_t w[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) w[u] = wn[u];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) wn[u] = (kw + 4 + u < nw) ? np[(size_t)(kw + 4 + u) * 64] : 0u;
+    const int ng = (c + 7) >> 3;
+    uint4 wn = ng > 0 ? np[0] : make_uint4(0u, 0u, 0u, 0u);
+    for (int kg = 0; kg < ng; ++kg) {
+      const uint32_t w[4] = {wn.x, wn.y, wn.z, wn.w};
+      wn = (kg + 1 < ng) ? np[(size_t)(kg + 1) * 64] : make_uint4(0u, 0u, 0u, 0u);
       uint32_t nearbits = 0;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int k = 2 * kw + u;
+        const int k = 8 * kg + u;
         const bool valid = k < c;
         const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
         const Slot qj = lds[e];
@@ -989,40 +1028,36 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
 // All-FP64 LJ over the tiled list; LDS holds FP64 positions with the type in w.
 template <bool ONETYPE, bool ENERGY, bool ACCUM>
 __global__ __launch_bounds__(256) void k_lj_tiled_fp64(
-    int n, const double4* __restrict__ pos, const uint32_t* __restrict__ nbr,
-    const int* __restrict__ cnt, int capw, Geom g, const TileRun* __restrict__ runs,
+    int n, int ntiles, const double4* __restrict__ pos, const uint4* __restrict__ nbr,
+    const int* __restrict__ cnt, int capg, Geom g, const TileRun* __restrict__ runs,
     const int* __restrict__ nruns, const LJParam* __restrict__ prm, int ntypes,
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
     double* __restrict__ partial) {
   extern __shared__ double4 ldsd[];
   __shared__ TileRun sr[kMaxRuns];
-  const int nr = stage_tile<double4>(runs, nruns, pos, ldsd, sr);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int nr = stage_tile<double4>(runs, nruns, pos, ldsd, sr, tile);
+  const int i = tile * blockDim.x + threadIdx.x;
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
   if (i < n) {
     const double4 pi = pos[i];
-    const uint32_t* np = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
-    const int c = nr > 0 ? min(cnt[i], 2 * capw) : 0;
+    const uint4* np = nbr + tiled_group(i, capg, 0);
+    const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
     const int rowi = (int)pi.w * ntypes;
     double ff1 = prm[0].dff1, ff2 = prm[0].dff2, rc2 = prm[0].drc2, shift = prm[0].dshift;
     double sig2 = prm[0].sig2, eps4 = prm[0].eps4;
     int set = prm[0].set;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
-    const int nw = (c + 1) >> 1;
-    uint32_t wn[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) wn[u] = (u < nw) ? np[(size_t)u * 64] : 0u;
-    for (int kw = 0; kw < nw; kw += 4) {
-      uint32_t w[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) w[u] = wn[u];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) wn[u] = (kw + 4 + u < nw) ? np[(size_t)(kw + 4 + u) * 64] : 0u;
+    const int ng = (c + 7) >> 3;
+    uint4 wn = ng > 0 ? np[0] : make_uint4(0u, 0u, 0u, 0u);
+    for (int kg = 0; kg < ng; ++kg) {
+      const uint32_t w[4] = {wn.x, wn.y, wn.z, wn.w};
+      wn = (kg + 1 < ng) ? np[(size_t)(kg + 1) * 64] : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int k = 2 * kw + u;
+        const int k = 8 * kg + u;
         if (k >= c) break;
         const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
         const double4 pj = ldsd[e];
@@ -1150,28 +1185,29 @@ __device__ __forceinline__ double min_image_fp64(const double4& pi, const double
 // type), FP64 stages the FP64 positions (type in w).
 template <bool MIXED, bool ENERGY, bool ACCUM>
 __global__ __launch_bounds__(256) void k_tab_tiled(
-    int n, const uint4* __restrict__ posq, const double4* __restrict__ pos, const uint32_t* __restrict__ nbr,
-    const int* __restrict__ cnt, int capw, Geom g, const TileRun* __restrict__ runs,
+    int n, int ntiles, const uint4* __restrict__ posq, const double4* __restrict__ pos, const uint4* __restrict__ nbr,
+    const int* __restrict__ cnt, int capg, Geom g, const TileRun* __restrict__ runs,
     const int* __restrict__ nruns, const TabParam* __restrict__ prm, const double* __restrict__ pool, int ntypes,
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ partial) {
   extern __shared__ uint4 lds_raw[];
   __shared__ TileRun sr[kMaxRuns];
+  const int tile = xcd_tile(blockIdx.x, ntiles);
   int nr;
-  if (MIXED) nr = stage_tile_q<uint4>(runs, nruns, posq, lds_raw, sr);
-  else nr = stage_tile<double4>(runs, nruns, pos, reinterpret_cast<double4*>(lds_raw), sr);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (MIXED) nr = stage_tile_q<uint4>(runs, nruns, posq, lds_raw, sr, tile);
+  else nr = stage_tile<double4>(runs, nruns, pos, reinterpret_cast<double4*>(lds_raw), sr, tile);
+  const int i = tile * blockDim.x + threadIdx.x;
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
   if (i < n) {
     const double4 pi = pos[i];
     const uint4 qi = MIXED ? posq[i] : make_uint4(0u, 0u, 0u, 0u);
-    const uint32_t* np = nbr + (size_t)(i >> 6) * (size_t)capw * 64 + (i & 63);
-    const int c = nr > 0 ? min(cnt[i], 2 * capw) : 0;
+    const uint32_t* np = reinterpret_cast<const uint32_t*>(nbr + tiled_group(i, capg, 0));
+    const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
     const int rowi = (int)pi.w * ntypes;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     for (int k = 0; k < c; ++k) {
-      const uint32_t w = np[(size_t)(k >> 1) * 64];
+      const uint32_t w = np[(size_t)(k >> 3) * 256 + ((k >> 1) & 3)];
       const int e = (k & 1) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
       double dx, dy, dz, r2;
       int tj;
@@ -1699,30 +1735,33 @@ void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Ge
   if (n > 0) hipLaunchKernelGGL(k_posq, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, posq, g);
 }
 
-void launch_nbr_build_tiled(hipStream_t s, int n, int T, int lds_bytes, const uint4* posq, const double4* pos,
+static bool cubic_box(const Geom& g) { return g.qsd[0] == g.qsd[1] && g.qsd[1] == g.qsd[2]; }
+
+void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
-                            const int* ex_ids, uint32_t* nbr, int* cnt, int capw, Ctl* ctl) {
+                            const int* ex_ids, uint4* nbr, int* cnt, int capg, Ctl* ctl) {
   if (n <= 0) return;
-  dim3 grid(nblocks(n, T)), block(T);
-  if (ex_start)
-    hipLaunchKernelGGL(k_nbr_build_tiled<true>, grid, block, 0, s, n, posq, pos, id, cell_of_slot, cstart, g,
-                       cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capw, ctl);
-  else
-    hipLaunchKernelGGL(k_nbr_build_tiled<false>, grid, block, 0, s, n, posq, pos, id, cell_of_slot, cstart, g,
-                       cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capw, ctl);
+  const int ntiles = nblocks(n, T);
+  dim3 grid(ntiles), block(T);
+#define NB(EX, CU) hipLaunchKernelGGL((k_nbr_build_tiled<EX, CU>), grid, block, 0, s, n, ntiles, posq, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, ctl)
+  const bool cu = cubic_box(g);
+  if (ex_start) { if (cu) NB(true, true); else NB(true, false); }
+  else { if (cu) NB(false, true); else NB(false, false); }
+#undef NB
 }
 
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
-                     const uint4* posq, const double4* pos, const uint32_t* nbr, const int* cnt, int capw,
+                     const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                      const Geom& g, const TileRun* runs, const int* nruns, const LJParam* prm, int ntypes,
                      double* fx, double* fy, double* fz, double* partial) {
   if (n <= 0) return;
-  dim3 grid(nblocks(n, T)), block(T);
-  const bool cubic = g.qsd[0] == g.qsd[1] && g.qsd[1] == g.qsd[2];
-#define LJM(OT, EN, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, true>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); \
-                             else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, false>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); } while (0)
-#define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64<OT, EN, AC>), grid, block, lds_bytes, s, n, pos, nbr, cnt, capw, g, runs, nruns, prm, ntypes, fx, fy, fz, partial)
+  const int ntiles = nblocks(n, T);
+  dim3 grid(ntiles), block(T);
+  const bool cubic = cubic_box(g);
+#define LJM(OT, EN, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, true>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); \
+                             else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, false>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); } while (0)
+#define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64<OT, EN, AC>), grid, block, lds_bytes, s, n, ntiles, pos, nbr, cnt, capg, g, runs, nruns, prm, ntypes, fx, fy, fz, partial)
   if (mixed) {
     if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }
     else if (onetype) { if (accum) LJM(true, false, true); else LJM(true, false, false); }
@@ -1754,12 +1793,13 @@ void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, cons
 }
 
 void launch_tab_tiled(hipStream_t s, bool mixed, bool energy, bool accum, int n, int T, int lds_bytes,
-                      const uint4* posq, const double4* pos, const uint32_t* nbr, const int* cnt, int capw,
+                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                       const Geom& g, const TileRun* runs, const int* nruns, const TabParam* prm, const double* pool,
                       int ntypes, double* fx, double* fy, double* fz, double* partial) {
   if (n <= 0) return;
-  dim3 grid(nblocks(n, T)), block(T);
-#define TT(MX, EN, AC) hipLaunchKernelGGL((k_tab_tiled<MX, EN, AC>), grid, block, lds_bytes, s, n, posq, pos, nbr, cnt, capw, g, runs, nruns, prm, pool, ntypes, fx, fy, fz, partial)
+  const int ntiles = nblocks(n, T);
+  dim3 grid(ntiles), block(T);
+#define TT(MX, EN, AC) hipLaunchKernelGGL((k_tab_tiled<MX, EN, AC>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, prm, pool, ntypes, fx, fy, fz, partial)
   if (mixed) {
     if (energy) TT(true, true, false); else if (accum) TT(true, false, true); else TT(true, false, false);
   } else {

@@ -40,6 +40,8 @@ void hip_check(hipError_t e, const char* what) {
 }
 
 // Device buffer; resize() discards contents, grow_keep() preserves them.
+constexpr size_t kPosqPad = 64;   // spare fixed-point mirror entries after the last particle
+
 template <class T>
 struct DBuf {
   T* p = nullptr;
@@ -404,7 +406,8 @@ struct espp_sim {
     pos2.resize(m); vel2.resize(m); fx2.resize(m); fy2.resize(m); fz2.resize(m);
     id2.resize(m); img2.resize(m);
     cell_of.resize(m); rank.resize(m); order.resize(m); cell_of_slot.grow_keep(m, keep, stream);
-    posq.grow_keep(m, keep, stream);
+    // the tiled list build reads up to 31 entries past a candidate range
+    posq.grow_keep(m + kPosqPad, keep, stream);
   }
 
   int64_t ncell() const { return (int64_t)g.n[0] * g.n[1] * g.n[2]; }
@@ -523,11 +526,11 @@ struct espp_sim {
       launch_reset_ctl(stream, ctl.p);
       hipEvent_t pe = prof_begin();
       if (v.tiled) {
-        const int capw = v.cap / 2;
-        v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)capw * 64);
-        launch_nbr_build_tiled(stream, (int)n, T, max_region * 16, posq.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
+        // 16-bit entries, eight per 16-byte group: cap / 2 ints per particle
+        v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)(v.cap / 2) * 64);
+        launch_nbr_build_tiled(stream, (int)n, T, posq.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
                                cutsq, cut_lo_q, cut_hi_q, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
-                               v.has_excl ? v.ex_ids.p : nullptr, (uint32_t*)v.nbr.p, v.cnt.p, capw, ctl.p);
+                               v.has_excl ? v.ex_ids.p : nullptr, (uint4*)v.nbr.p, v.cnt.p, v.cap / 8, ctl.p);
       } else {
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
         launch_nbr_build(stream, (int)n, pos.p, id.p, cell_of_slot.p, cstart.p, g, cutsq,
@@ -687,7 +690,7 @@ struct espp_sim {
       hipEvent_t pe = energy ? nullptr : prof_begin();
       if (v.tiled)
         launch_tab_tiled(stream, mixed(), energy, accum, (int)n, T, max_region * (mixed() ? 16 : 32), posq.p, pos.p,
-                         (const uint32_t*)v.nbr.p, v.cnt.p, v.cap / 2, g, runs.p, nruns.p, a.ttab.p, a.tpool.p,
+                         (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, a.ttab.p, a.tpool.p,
                          a.dtab_nt, fx.p, fy.p, fz.p, partial_out);
       else
         launch_tab(stream, energy, accum, (int)n, pos.p, v.nbr.p, v.cnt.p, v.cap, g, a.ttab.p, a.tpool.p, a.dtab_nt,
@@ -713,7 +716,7 @@ struct espp_sim {
       if (v.tiled)
         launch_lj_tiled(stream, mixed(), onetype, energy, accum, (int)n, T,
                         max_region * (mixed() ? (onetype ? 12 : 16) : 32), posq.p,
-                        pos.p, (const uint32_t*)v.nbr.p, v.cnt.p, v.cap / 2, g, runs.p, nruns.p, a.dtab.p, a.dtab_nt,
+                        pos.p, (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, a.dtab.p, a.dtab_nt,
                         fx.p, fy.p, fz.p, partial_out);
       else
         launch_lj(stream, mixed(), onetype, energy, accum, (int)n, posq.p, pos.p, v.nbr.p, v.cnt.p, v.cap, g,
@@ -1241,9 +1244,10 @@ int espp_vl_pairs(espp_sim* s, int32_t vl, int32_t* out, int64_t cap, int64_t* t
       nruns = download(s->nruns, ntiles);
     }
     auto entry_slot = [&](size_t i, int q) -> int {
-      const int* row = nb.data() + (i / 64) * rowlen * 64 + (i % 64);
-      if (!v.tiled) return row[(size_t)q * 64];
-      uint32_t w = (uint32_t)row[(size_t)(q / 2) * 64];
+      if (!v.tiled) return nb[(i / 64) * rowlen * 64 + (size_t)q * 64 + (i % 64)];
+      // 16-byte groups of eight entries, interleaved over the 64 particles of a wavefront
+      const size_t grp = ((i / 64) * (size_t)(v.cap / 8) + (size_t)(q / 8)) * 64 + (i % 64);
+      uint32_t w = (uint32_t)nb[grp * 4 + (size_t)((q / 2) % 4)];
       int e = (q & 1) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
       size_t t = i / (size_t)s->T;
       const TileRun* sr = runs.data() + t * kMaxRuns;
