@@ -60,6 +60,7 @@ SIGNATURES = {
     "espp_num_resorts": [_vp, _P(_c_i64)],
     "espp_timers": [_vp, _vp],
     "espp_kinetic": [_vp, _P(_c_dbl)],
+    "espp_rdf": [_vp, _c_int, _vp],
     "espp_kinetic_tensor": [_vp, _c_u64, _vp, _vp, _vp],
 }
 

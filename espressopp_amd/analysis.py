@@ -1,4 +1,4 @@
-"""espressopp.analysis: Temperature, Pressure, PressureTensor, NPart.
+"""espressopp.analysis: Temperature, Pressure, PressureTensor, NPart, RadialDistrF.
 
 Reference: src/analysis/Temperature.hpp:60-125 (T = sum m v^2 / (3 N), no
 centre-of-mass correction), src/analysis/Pressure.cpp:41-121
@@ -80,3 +80,18 @@ class PressureTensor(Observable):
 class NPart(Observable):
     def compute(self):
         return self._h().n
+
+
+class RadialDistrF(Observable):
+    """g(r) over all pairs up to half the box (src/analysis/RadialDistrF.cpp:47-140,
+    src/analysis/RadialDistrF.py): compute(rdfN) returns rdfN values for bins of
+    width (L_y / 2) / rdfN.  The pair histogram is a GPU kernel."""
+
+    def __init__(self, system):
+        Observable.__init__(self, system)
+        self.print_progress = False
+
+    def compute(self, rdfN):
+        out = np.zeros(int(rdfN))
+        self._h().call("espp_rdf", int(rdfN), out.ctypes.data_as(ctypes.c_void_p))
+        return out.tolist()

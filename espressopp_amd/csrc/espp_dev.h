@@ -2,7 +2,8 @@
 //
 // Particle storage on the device is cell-sorted ("slot" order): after every
 // decompose the particles of inner cell 0 come first, then cell 1, ... and
-// inside a cell ascending particle id.  Cells use the reference's inner-cell
+// inside a cell in the order of their previous slots (insertion order at the
+// first decompose).  Cells use the reference's inner-cell
 // numbering x + nx*(y + ny*z) (src/esutil/Grid.hpp:68-71).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -31,8 +32,9 @@ struct LJParam {
   double dff1, dff2, drc2, dshift; // FP64, length units
   double sig2, eps4;               // energy: 4 eps ((sig2/r2)^6 - (sig2/r2)^3)
   double dlo, dhi;                 // cutoff window (length units) settled from FP64 positions
+  double dmid, dhw;                // the window as centre and half width
   // the same in fixed-point units of a cubic box (r^2 unit = (L / 2^(32+k))^2)
-  double uff1, uff2, urc2, ulo, uhi, usig2;
+  double uff1, uff2, urc2, ulo, uhi, usig2, umid, uhw;
   int set;
   int pad;
 };

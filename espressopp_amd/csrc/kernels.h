@@ -12,10 +12,11 @@ int scan_blocks(int ncell);
 void launch_step_fused(hipStream_t s, int n, const int* id, double4* pos, double4* vel, double* fx, double* fy,
                        double* fz, int lang, double pref1, double pref2, unsigned long long seed,
                        unsigned long long call, const unsigned char* excl, int nid, int do_int2, double dt,
-                       uint4* posq, const Geom& g, double* blockmax, Ctl* ctl);
+                       uint4* posq, const Geom& g, double* blockmax, Ctl* ctl, unsigned long long* host_out);
+void launch_permute_int(hipStream_t s, int n, const int* order, const int* src, int* dst);
 void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start,
                     int* order);
-void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const int* id);
+void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const double4* pos);
 void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, const double4* pos,
                     const double4* vel, const double* fx, const double* fy, const double* fz,
                     const int* id, const int4* img, double4* pos2, double4* vel2, double* fx2,
@@ -38,11 +39,11 @@ void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Ge
 void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
-                            const int* ex_ids, uint4* nbr, int* cnt, int capg, Ctl* ctl);
+                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot, Ctl* ctl);
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
-                     const Geom& g, const TileRun* runs, const int* nruns, const LJParam* prm, int ntypes,
-                     double* fx, double* fy, double* fz, double* partial);
+                     const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
+                     int ntypes, double* fx, double* fy, double* fz, double* partial);
 int lj_tiled_blocks(int n, int T);
 void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, const double4* pos,
                  const uint4* posq, const int* id, const int* slot_of_id, int nid,
@@ -50,14 +51,17 @@ void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, cons
                  double* fx, double* fy, double* fz, double* partial);
 void launch_tab_tiled(hipStream_t s, bool mixed, bool energy, bool accum, int n, int T, int lds_bytes,
                       const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
-                      const Geom& g, const TileRun* runs, const int* nruns, const TabParam* prm, const double* pool,
-                      int ntypes, double* fx, double* fy, double* fz, double* partial);
+                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const TabParam* prm,
+                      const double* pool, int ntypes, double* fx, double* fy, double* fz, double* partial);
 void launch_tab(hipStream_t s, bool energy, bool accum, int n, const double4* pos, const int* nbr, const int* cnt,
                 int cap, const Geom& g, const TabParam* prm, const double* pool, int ntypes, double* fx, double* fy,
                 double* fz, double* partial);
 void launch_bond_tab(hipStream_t s, bool energy, bool accum, int n, const double4* pos, const int* id,
                      const int* slot_of_id, int nid, const int* bstart, const int* bpartner, const Geom& g,
                      const TabParam* prm, const double* pool, double* fx, double* fy, double* fz, double* partial);
+constexpr int kRdfLdsBins = 12288;   // histograms up to this many bins are kept in LDS
+void launch_rdf(hipStream_t s, int n, const double4* pos, const Geom& g, double dr, int nbins,
+                unsigned long long* counts);
 void launch_zero3(hipStream_t s, int n, double* a, double* b, double* c);
 void launch_thermo_integrate2(hipStream_t s, int n, const int* id, double4* vel, double* fx,
                               double* fy, double* fz, int lang, double pref1, double pref2,

@@ -248,6 +248,13 @@ __global__ __launch_bounds__(1024) void k_scan_down(int ncell, const int* __rest
 }
 
 
+// dst[s] = src[order[s]]: a per-particle int array into the new slot order
+__global__ void k_permute_int(int n, const int* __restrict__ order, const int* __restrict__ src,
+                              int* __restrict__ dst) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) dst[s] = src[order[s]];
+}
+
 __global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
                           const int* __restrict__ start, int* __restrict__ order) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -255,19 +262,55 @@ __global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __r
   order[start[cell_of[i]] + rank[i]] = i;
 }
 
-// Inside each cell, order members by particle id (insertion sort; cells hold
-// a few tens of particles at liquid densities).
-__global__ void k_sort_cells(int ncell, const int* __restrict__ start, int* __restrict__ order,
-                             const int* __restrict__ id) {
+// Inside each cell, order members by x, ties by previous slot (insertion sort;
+// cells hold a few tens of particles at liquid densities).  The order is
+// deterministic, and a row of consecutive cells is sorted by x, which lets the
+// list build clip a row to the candidates within reach in x.
+constexpr int kSortThreads = 64, kSortLds = 48;   // cells with up to kSortLds members sort in LDS
+__global__ __launch_bounds__(kSortThreads) void k_sort_cells(int ncell, const int* __restrict__ start,
+                                                             int* __restrict__ order, const double4* __restrict__ pos) {
+  __shared__ double sx[kSortLds * kSortThreads];
+  __shared__ int so[kSortLds * kSortThreads];
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
-  int b = start[c], e = start[c + 1];
+  const int b = start[c], e = start[c + 1];
+  // key (x, previous slot); members were x-sorted at the last binning and move
+  // little, so the insertion sort does few moves
+  if (e - b <= kSortLds) {
+    // keys gathered once (independent loads), sorted in LDS (entries interleaved by thread)
+    const int m0 = e - b, t = threadIdx.x;
+    for (int k = 0; k < m0; ++k) {
+      const int o = order[b + k];
+      so[k * kSortThreads + t] = o;
+      sx[k * kSortThreads + t] = pos[o].x;
+    }
+    for (int k = 1; k < m0; ++k) {
+      const int o = so[k * kSortThreads + t];
+      const double xo = sx[k * kSortThreads + t];
+      int m = k - 1;
+      while (m >= 0) {
+        const double xm = sx[m * kSortThreads + t];
+        const int om = so[m * kSortThreads + t];
+        if (!(xm > xo || (xm == xo && om > o))) break;
+        sx[(m + 1) * kSortThreads + t] = xm;
+        so[(m + 1) * kSortThreads + t] = om;
+        --m;
+      }
+      sx[(m + 1) * kSortThreads + t] = xo;
+      so[(m + 1) * kSortThreads + t] = o;
+    }
+    for (int k = 0; k < m0; ++k) order[b + k] = so[k * kSortThreads + t];
+    return;
+  }
   for (int k = b + 1; k < e; ++k) {
-    int o = order[k];
-    int key = id[o];
+    const int o = order[k];
+    const double xo = pos[o].x;
     int m = k - 1;
-    while (m >= b && id[order[m]] > key) {
-      order[m + 1] = order[m];
+    while (m >= b) {
+      const int om = order[m];
+      const double xm = pos[om].x;
+      if (!(xm > xo || (xm == xo && om > o))) break;
+      order[m + 1] = om;
       --m;
     }
     order[m + 1] = o;
@@ -571,17 +614,25 @@ __global__ __launch_bounds__(256) void k_lj_fp64(
 
 __device__ __forceinline__ int wrap_dim(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-// One thread per tile.  Runs are sorted by cell index and never overlap.
-__global__ void k_tile_plan(int n, int T, int ntiles, const int* __restrict__ cell_of_slot,
+// One thread per tile, kPlanThreads tiles per workgroup.  Each thread keeps its
+// row pieces in LDS (interleaved by thread, so the threads of a wavefront hit
+// distinct banks).  Runs are sorted by cell index and never overlap.
+constexpr int kPlanThreads = 32;
+__global__ __launch_bounds__(kPlanThreads) void k_tile_plan(int n, int T, int ntiles, const int* __restrict__ cell_of_slot,
                             const int* __restrict__ cstart, Geom g, int budget,
                             TileRun* __restrict__ runs, int* __restrict__ nruns, Ctl* ctl) {
+  __shared__ int spa[kMaxPieces * kPlanThreads], spb[kMaxPieces * kPlanThreads];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int s0 = t * T, s1 = min(s0 + T, n) - 1;
   const int c0 = cell_of_slot[s0], c1 = cell_of_slot[s1];
   const int r0 = c0 / nx, r1 = c1 / nx;
-  int pa[kMaxPieces], pb[kMaxPieces];
+  struct Strided {
+    int* base;
+    __device__ int& operator[](int k) const { return base[k * kPlanThreads]; }
+  };
+  const Strided pa{spa + threadIdx.x}, pb{spb + threadIdx.x};
   int np = 0;
   bool bad = (r1 - r0 + 1) * 18 > kMaxPieces;
   for (int r = r0; r <= r1 && !bad; ++r) {
@@ -668,19 +719,17 @@ __device__ __forceinline__ int stage_tile(const TileRun* __restrict__ runs, cons
   __syncthreads();
   const int total = nr > 0 ? sr[nr - 1].lds_off + sr[nr - 1].count : 0;
   const int bs = blockDim.x;
-  // eight independent loads in flight per thread, then eight LDS stores
+  // eight independent loads in flight per thread, then eight LDS stores; a
+  // thread's entries rise by bs, so its run index only ever moves forward
+  int r = 0;
   for (int base = 0; base < total; base += 8 * bs) {
     V v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int e = base + u * bs + (int)threadIdx.x;
       if (e < total) {
-        int lo = 0, hi = nr - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (sr[mid].lds_off <= e) lo = mid; else hi = mid - 1;
-        }
-        v[u] = src[sr[lo].slot_start + (e - sr[lo].lds_off)];
+        while (r + 1 < nr && sr[r + 1].lds_off <= e) ++r;
+        v[u] = src[sr[r].slot_start + (e - sr[r].lds_off)];
       }
     }
 #pragma unroll
@@ -709,18 +758,15 @@ __device__ __forceinline__ int stage_tile_q(const TileRun* __restrict__ runs, co
   __syncthreads();
   const int total = nr > 0 ? sr[nr - 1].lds_off + sr[nr - 1].count : 0;
   const int bs = blockDim.x;
+  int r = 0;   // run of the thread's current entry (entries rise by bs)
   for (int base = 0; base < total; base += 8 * bs) {
     uint4 v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int e = base + u * bs + (int)threadIdx.x;
       if (e < total) {
-        int lo = 0, hi = nr - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (sr[mid].lds_off <= e) lo = mid; else hi = mid - 1;
-        }
-        v[u] = src[sr[lo].slot_start + (e - sr[lo].lds_off)];
+        while (r + 1 < nr && sr[r + 1].lds_off <= e) ++r;
+        v[u] = src[sr[r].slot_start + (e - sr[r].lds_off)];
       }
     }
 #pragma unroll
@@ -737,6 +783,24 @@ __device__ __forceinline__ int stage_tile_q(const TileRun* __restrict__ runs, co
 __device__ __forceinline__ bool exact_sqr_le(double ex, double ey, double ez, double cutsq) {
 #pragma clang fp contract(off)
   return ex * ex + ey * ey + ez * ez <= cutsq;
+}
+
+// Ascending bitonic sort of the block's (key, value) pairs in LDS, one pair per
+// thread, blockDim.x a power of two; equal keys keep value order.
+__device__ void block_sort_pairs(int* skey, int* sval) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int size = 2; size <= nt; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      const int o = t ^ stride;
+      if (o > t) {
+        const int ka = skey[t], kb = skey[o], va = sval[t], vb = sval[o];
+        const bool gt = ka > kb || (ka == kb && va > vb);
+        if (gt == ((t & size) == 0)) { skey[t] = kb; skey[o] = ka; sval[t] = vb; sval[o] = va; }
+      }
+    }
+  }
+  __syncthreads();
 }
 
 // Workgroup -> tile map that keeps consecutive tiles on one XCD: workgroups are
@@ -758,12 +822,17 @@ __device__ __forceinline__ size_t tiled_group(int i, int capg, int g) {
   return ((size_t)(i >> 6) * (size_t)capg + (size_t)g) * 64 + (size_t)(i & 63);
 }
 
+// candidates tested per block of the list build (bit masks of this width); a
+// short block wastes less of a row's clipped range at its end
+constexpr int kCandChunk = 16;
+constexpr uint32_t kCandAll = (1u << kCandChunk) - 1u;
+
 // Tiled Verlet-list build (cell grids with at least 3 cells per dimension).
 // For each of its 9 neighbour x-rows a particle scans the candidates of the
 // three cells (cx-1, cx, cx+1), one contiguous slot range or two when the row
 // wraps, read from the fixed-point mirror (lanes of one cell read the same
 // candidates, so the loads broadcast; the mirror is padded so reads past a
-// range stay inside it).  For 32 candidates at a time two bit masks are formed
+// range stay inside it).  For kCandChunk candidates at a time two bit masks are formed
 // without branches: FP32 r^2 (fixed-point units) below the window (surely in)
 // and below its top (in or undecided).  Undecided pairs, within a small window
 // of (rc+skin)^2, are settled by the reference's FP64 test in the reference's
@@ -775,13 +844,25 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g, double cutsq, float cut_lo,
     float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,  // cut_lo/hi in units of qs[0]^2
     const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint4* __restrict__ nbr,
-    int* __restrict__ cnt, int capg, Ctl* ctl) {
+    int* __restrict__ cnt, int capg, int* __restrict__ pred, int* __restrict__ lane_slot, Ctl* ctl) {
   __shared__ TileRun sr[kMaxRuns];
+  __shared__ int skey[256], sval[256];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = nruns[tile];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)tile * kMaxRuns + threadIdx.x];
+  // Lanes take the tile's particles in order of their list length at the last
+  // build (carried with the particles through binning), so the lanes of a
+  // wavefront have lists of similar length in every kernel that walks them.
+  const int s0 = tile * blockDim.x;
+  skey[threadIdx.x] = (s0 + (int)threadIdx.x < n) ? pred[s0 + threadIdx.x] : 0x7FFFFFFF;
+  sval[threadIdx.x] = threadIdx.x;
+  block_sort_pairs(skey, sval);
+  skey[sval[threadIdx.x]] = threadIdx.x;  // rank of each particle (sval is no longer needed)
   __syncthreads();
-  const int i = tile * blockDim.x + threadIdx.x;
+  // threads stay in slot order (lanes of one cell read the same candidates);
+  // each particle's list goes to the lane of its rank
+  const int i = s0 + threadIdx.x;        // particle slot
+  const int L = s0 + skey[threadIdx.x];  // list lane (slots past n rank last, with lanes past n)
   int k = 0;
   if (i < n && nr > 0) {
     const uint4 qi = posq[i];
@@ -789,11 +870,13 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
     const int myid = HAS_EXCL ? id[i] : 0;
-    uint4* out = nbr + tiled_group(i, capg, 0);
+    uint4* out = nbr + tiled_group(L, capg, 0);
     const int cap = 8 * capg;
     // entries not yet stored: w0..w3 hold the current group, half-filled word in `pend`
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, pend = 0;
     const float ry = (float)(g.qsd[1] / g.qsd[0]), rz = (float)(g.qsd[2] / g.qsd[0]);
+    // reach in x, fixed-point units: the prefilter's top plus two units of rounding
+    const int reach_x = (int)fminf(sqrtf(cut_hi) * 1.0001f + 2.0f, 2.0e9f);
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nz; sz = -1; } else if (qz >= nz) { qz -= nz; sz = 1; }
@@ -810,16 +893,33 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
           const int xa = p ? pa1 : pa0, xb = p ? pb1 : pb0;
           if (xb < xa) break;
           const int ca = row + xa;
-          const int jb = cstart[ca], je = cstart[row + xb + 1];
+          const int jb = cstart[ca], je0 = cstart[row + xb + 1];
           const TileRun R = sr[find_run_by_cell(sr, nr, ca)];
           const int base = R.lds_off + (jb - R.slot_start);
-          for (int j0 = jb; j0 < je; j0 += 32) {
+          // the piece is sorted by x, so x_i - x_j falls along it: keep the stretch
+          // with |x_i - x_j| <= reach (fixed-point units of x)
+          int ja = jb, je = je0;
+          {
+            int lo = jb, hi = je0;                       // first j with x_i - x_j <= reach
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if ((int)(qi.x - posq[mid].x) > reach_x) lo = mid + 1; else hi = mid;
+            }
+            ja = lo;
+            hi = je0;                                    // first j with x_i - x_j < -reach
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if ((int)(qi.x - posq[mid].x) >= -reach_x) lo = mid + 1; else hi = mid;
+            }
+            je = lo;
+          }
+          for (int j0 = ja; j0 < je; j0 += kCandChunk) {
             const int jn = je - j0;
             const uint4* cand = posq + j0;
             uint32_t in_bits = 0, maybe_bits = 0;
-            // candidates 31 .. 0, so that candidate t ends at bit t of the masks
+            // candidates kCandChunk-1 .. 0, so that candidate t ends at bit t of the masks
 #pragma unroll
-            for (int u = 28; u >= 0; u -= 4) {
+            for (int u = kCandChunk - 4; u >= 0; u -= 4) {
               uint4 q[4];
 #pragma unroll
               for (int v = 0; v < 4; ++v) q[v] = cand[u + v];
@@ -834,9 +934,9 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
                 maybe_bits = (maybe_bits << 1) + (r2 <= cut_hi ? 1u : 0u);
               }
             }
-            uint32_t valid = jn >= 32 ? 0xFFFFFFFFu : ((1u << jn) - 1u);
+            uint32_t valid = jn >= kCandChunk ? kCandAll : ((1u << jn) - 1u);
             const int self = i - j0;
-            if (self >= 0 && self < 32) valid &= ~(1u << self);
+            if (self >= 0 && self < kCandChunk) valid &= ~(1u << self);
             in_bits &= valid;
             uint32_t border = maybe_bits & valid & ~in_bits;
             while (border) {
@@ -881,10 +981,14 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
       if (k & 1) { w0 = wi == 0 ? pend : w0; w1 = wi == 1 ? pend : w1; w2 = wi == 2 ? pend : w2; w3 = wi == 3 ? pend : w3; }
       out[(size_t)(k >> 3) * 64] = make_uint4(w0, w1, w2, w3);
     }
-    cnt[i] = k;
+    cnt[L] = k;
     if (k > cap) ctl->overflow = 1;
   } else if (i < n) {
-    cnt[i] = 0;
+    cnt[L] = 0;
+  }
+  if (i < n) {
+    pred[i] = k;
+    lane_slot[L] = i;
   }
   int m = k;
 #pragma unroll
@@ -906,9 +1010,9 @@ template <bool ONETYPE, bool ENERGY, bool ACCUM, bool CUBIC>
 __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     int n, int ntiles, const uint4* __restrict__ posq, const double4* __restrict__ pos,
     const uint4* __restrict__ nbr, const int* __restrict__ cnt, int capg, Geom g,
-    const TileRun* __restrict__ runs, const int* __restrict__ nruns, const LJParam* __restrict__ prm,
-    int ntypes, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-    double* __restrict__ partial) {
+    const TileRun* __restrict__ runs, const int* __restrict__ nruns, const int* __restrict__ lane_slot,
+    const LJParam* __restrict__ prm, int ntypes, double* __restrict__ fx, double* __restrict__ fy,
+    double* __restrict__ fz, double* __restrict__ partial) {
   // one type: LDS holds x, y, z (12 bytes); several types: x, y, z, type
   using Slot = typename std::conditional<ONETYPE, uint3, uint4>::type;
   extern __shared__ uint4 lds_raw[];
@@ -916,29 +1020,29 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
   __shared__ TileRun sr[kMaxRuns];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = stage_tile_q<Slot>(runs, nruns, posq, lds, sr, tile);
-  const int i = tile * blockDim.x + threadIdx.x;
+  const int L = tile * blockDim.x + threadIdx.x;   // list lane
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
-  if (i < n) {
+  if (L < n) {
+    const int i = lane_slot[L];                      // particle slot
     const uint4 qi = posq[i];
-    const uint4* np = nbr + tiled_group(i, capg, 0);
-    const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
+    const uint4* np = nbr + tiled_group(L, capg, 0);
+    const int c = nr > 0 ? min(cnt[L], 8 * capg) : 0;
     const int rowi = (int)qi.w * ntypes;
     const LJParam& p0 = prm[0];
     double ff1 = CUBIC ? p0.uff1 : p0.dff1, ff2 = CUBIC ? p0.uff2 : p0.dff2;
-    double rc2 = CUBIC ? p0.urc2 : p0.drc2, lo = CUBIC ? p0.ulo : p0.dlo, hi = CUBIC ? p0.uhi : p0.dhi;
+    double rc2 = CUBIC ? p0.urc2 : p0.drc2, mid = CUBIC ? p0.umid : p0.dmid, hw = CUBIC ? p0.uhw : p0.dhw;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     const int ng = (c + 7) >> 3;
     uint4 wn = ng > 0 ? np[0] : make_uint4(0u, 0u, 0u, 0u);
     for (int kg = 0; kg < ng; ++kg) {
       const uint32_t w[4] = {wn.x, wn.y, wn.z, wn.w};
       wn = (kg + 1 < ng) ? np[(size_t)(kg + 1) * 64] : make_uint4(0u, 0u, 0u, 0u);
-      uint32_t nearbits = 0;
+      bool anynear = false;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int k = 8 * kg + u;
-        const bool valid = k < c;
+        const bool valid = 8 * kg + u < c;
         const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
         const Slot qj = lds[e];
         if (!ONETYPE) {
@@ -947,13 +1051,14 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
           const LJParam& p = prm[rowi + tj];
           ff1 = CUBIC ? p.uff1 : p.dff1; ff2 = CUBIC ? p.uff2 : p.dff2;
           rc2 = p.set ? (CUBIC ? p.urc2 : p.drc2) : -1.0;
-          lo = CUBIC ? p.ulo : p.dlo; hi = CUBIC ? p.uhi : p.dhi;
+          mid = CUBIC ? p.umid : p.dmid; hw = p.set ? (CUBIC ? p.uhw : p.dhw) : -1.0;
         }
         double ex, ey, ez;
         const double r2 = pair_r2<CUBIC>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
-        const bool near = valid && r2 >= lo && r2 <= hi;
-        const bool in = valid && !near && r2 <= rc2;
-        nearbits |= (near ? 1u : 0u) << u;
+        // pairs in the window around rc^2 are settled after the batch from the FP64 positions
+        const bool near = __builtin_fabs(r2 - mid) <= hw;
+        const bool in = valid & !near & (r2 <= rc2);
+        anynear |= valid & near;
         double inv;
         double ff = lj_ff_seeded(r2, ff1, ff2, inv);
         ff = in ? ff : 0.0;
@@ -972,36 +1077,38 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
           acc[5] += ex * ey * ff; acc[6] += ex * ez * ff; acc[7] += ey * ez * ff;
         }
       }
-      // pairs at the cutoff: decide from the FP64 positions (rare)
-      while (nearbits) {
-        const int u = __ffs(nearbits) - 1;
-        nearbits &= nearbits - 1u;
-        const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
-        const Slot qj = lds[e];
-        int tj = 0;
-        if constexpr (!ONETYPE) tj = (int)qj.w;
-        const LJParam& p = prm[ONETYPE ? 0 : rowi + tj];
-        const int j = slot_of_lds(sr, nr, e);
-        const double4 pi = pos[i], pj = pos[j];
-        double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
-        hx = __builtin_fma(-g.L[0], __builtin_rint(hx * g.invL[0]), hx);
-        hy = __builtin_fma(-g.L[1], __builtin_rint(hy * g.invL[1]), hy);
-        hz = __builtin_fma(-g.L[2], __builtin_rint(hz * g.invL[2]), hz);
-        if (!(p.set && hx * hx + hy * hy + hz * hz <= p.drc2)) continue;
-        double ex, ey, ez;
-        const double r2 = pair_r2<CUBIC>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
-        double inv;
-        const double ff = lj_ff_seeded(r2, CUBIC ? p.uff1 : p.dff1, CUBIC ? p.uff2 : p.dff2, inv);
-        Ax = __builtin_fma(ex, ff, Ax);
-        Ay = __builtin_fma(ey, ff, Ay);
-        Az = __builtin_fma(ez, ff, Az);
-        if (ENERGY) {
-          const double f2 = (CUBIC ? p.usig2 : p.sig2) * inv;
-          const double f6 = f2 * f2 * f2;
-          acc[0] += p.eps4 * (f6 * f6 - f6) - p.dshift;
-          acc[1] += r2 * ff;
-          acc[2] += ex * ex * ff; acc[3] += ey * ey * ff; acc[4] += ez * ez * ff;
-          acc[5] += ex * ey * ff; acc[6] += ex * ez * ff; acc[7] += ey * ez * ff;
+      if (anynear) {
+        // rare: the batch holds a pair in the cutoff window; decide it from the FP64 positions
+        for (int u = 0; u < 8 && 8 * kg + u < c; ++u) {
+          const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
+          const Slot qj = lds[e];
+          int tj = 0;
+          if constexpr (!ONETYPE) tj = (int)qj.w;
+          const LJParam& p = prm[ONETYPE ? 0 : rowi + tj];
+          if (!p.set) continue;
+          double ex, ey, ez;
+          const double r2 = pair_r2<CUBIC>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
+          if (!(__builtin_fabs(r2 - (CUBIC ? p.umid : p.dmid)) <= (CUBIC ? p.uhw : p.dhw))) continue;
+          const int j = slot_of_lds(sr, nr, e);
+          const double4 pi = pos[i], pj = pos[j];
+          double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
+          hx = __builtin_fma(-g.L[0], __builtin_rint(hx * g.invL[0]), hx);
+          hy = __builtin_fma(-g.L[1], __builtin_rint(hy * g.invL[1]), hy);
+          hz = __builtin_fma(-g.L[2], __builtin_rint(hz * g.invL[2]), hz);
+          if (!(hx * hx + hy * hy + hz * hz <= p.drc2)) continue;
+          double inv;
+          const double ff = lj_ff_seeded(r2, CUBIC ? p.uff1 : p.dff1, CUBIC ? p.uff2 : p.dff2, inv);
+          Ax = __builtin_fma(ex, ff, Ax);
+          Ay = __builtin_fma(ey, ff, Ay);
+          Az = __builtin_fma(ez, ff, Az);
+          if (ENERGY) {
+            const double f2 = (CUBIC ? p.usig2 : p.sig2) * inv;
+            const double f6 = f2 * f2 * f2;
+            acc[0] += p.eps4 * (f6 * f6 - f6) - p.dshift;
+            acc[1] += r2 * ff;
+            acc[2] += ex * ex * ff; acc[3] += ey * ey * ff; acc[4] += ez * ez * ff;
+            acc[5] += ex * ey * ff; acc[6] += ex * ez * ff; acc[7] += ey * ez * ff;
+          }
         }
       }
     }
@@ -1030,21 +1137,22 @@ template <bool ONETYPE, bool ENERGY, bool ACCUM>
 __global__ __launch_bounds__(256) void k_lj_tiled_fp64(
     int n, int ntiles, const double4* __restrict__ pos, const uint4* __restrict__ nbr,
     const int* __restrict__ cnt, int capg, Geom g, const TileRun* __restrict__ runs,
-    const int* __restrict__ nruns, const LJParam* __restrict__ prm, int ntypes,
-    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    const int* __restrict__ nruns, const int* __restrict__ lane_slot, const LJParam* __restrict__ prm,
+    int ntypes, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
     double* __restrict__ partial) {
   extern __shared__ double4 ldsd[];
   __shared__ TileRun sr[kMaxRuns];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = stage_tile<double4>(runs, nruns, pos, ldsd, sr, tile);
-  const int i = tile * blockDim.x + threadIdx.x;
+  const int L = tile * blockDim.x + threadIdx.x;   // list lane
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
-  if (i < n) {
+  if (L < n) {
+    const int i = lane_slot[L];                      // particle slot
     const double4 pi = pos[i];
-    const uint4* np = nbr + tiled_group(i, capg, 0);
-    const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
+    const uint4* np = nbr + tiled_group(L, capg, 0);
+    const int c = nr > 0 ? min(cnt[L], 8 * capg) : 0;
     const int rowi = (int)pi.w * ntypes;
     double ff1 = prm[0].dff1, ff2 = prm[0].dff2, rc2 = prm[0].drc2, shift = prm[0].dshift;
     double sig2 = prm[0].sig2, eps4 = prm[0].eps4;
@@ -1187,23 +1295,25 @@ template <bool MIXED, bool ENERGY, bool ACCUM>
 __global__ __launch_bounds__(256) void k_tab_tiled(
     int n, int ntiles, const uint4* __restrict__ posq, const double4* __restrict__ pos, const uint4* __restrict__ nbr,
     const int* __restrict__ cnt, int capg, Geom g, const TileRun* __restrict__ runs,
-    const int* __restrict__ nruns, const TabParam* __restrict__ prm, const double* __restrict__ pool, int ntypes,
-    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ partial) {
+    const int* __restrict__ nruns, const int* __restrict__ lane_slot, const TabParam* __restrict__ prm,
+    const double* __restrict__ pool, int ntypes, double* __restrict__ fx, double* __restrict__ fy,
+    double* __restrict__ fz, double* __restrict__ partial) {
   extern __shared__ uint4 lds_raw[];
   __shared__ TileRun sr[kMaxRuns];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   int nr;
   if (MIXED) nr = stage_tile_q<uint4>(runs, nruns, posq, lds_raw, sr, tile);
   else nr = stage_tile<double4>(runs, nruns, pos, reinterpret_cast<double4*>(lds_raw), sr, tile);
-  const int i = tile * blockDim.x + threadIdx.x;
+  const int L = tile * blockDim.x + threadIdx.x;   // list lane
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
-  if (i < n) {
+  if (L < n) {
+    const int i = lane_slot[L];                      // particle slot
     const double4 pi = pos[i];
     const uint4 qi = MIXED ? posq[i] : make_uint4(0u, 0u, 0u, 0u);
-    const uint32_t* np = reinterpret_cast<const uint32_t*>(nbr + tiled_group(i, capg, 0));
-    const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
+    const uint32_t* np = reinterpret_cast<const uint32_t*>(nbr + tiled_group(L, capg, 0));
+    const int c = nr > 0 ? min(cnt[L], 8 * capg) : 0;
     const int rowi = (int)pi.w * ntypes;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     for (int k = 0; k < c; ++k) {
@@ -1316,6 +1426,63 @@ __global__ __launch_bounds__(256) void k_bond_tab(
 #pragma unroll
     for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
     block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
+// ------------------------------------------------------------------ radial distribution
+//
+// Pair-distance histogram of RadialDistrF::computeArray
+// (src/analysis/RadialDistrF.cpp:47-125): every pair i < j, minimum image by a
+// single +-L correction per component, bin = (int)(|d| / dr), kept when
+// bin < nbins.  FP64 without contraction and with IEEE sqrt and division, so
+// every pair falls into the reference's bin.  A workgroup takes 256 particles i
+// and a chunk of particles j staged through LDS; counts gather in an LDS
+// histogram (LDS_HIST) or directly in global memory.
+template <bool LDS_HIST>
+__global__ __launch_bounds__(256) void k_rdf(int n, const double4* __restrict__ pos, Geom g, double dr, int nbins,
+                                             int jchunk, unsigned long long* __restrict__ counts) {
+#pragma clang fp contract(off)
+  extern __shared__ unsigned int hist[];
+  __shared__ double4 pj_s[256];
+  const int i0 = blockIdx.x * 256;
+  const int j0 = blockIdx.y * jchunk;
+  if (j0 + jchunk - 1 <= i0) return;               // no pair j > i in this block (uniform)
+  if (LDS_HIST) {
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0u;
+  }
+  const int i = i0 + threadIdx.x;
+  const double4 pi = i < n ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+  const int jend = min(j0 + jchunk, n);
+  for (int jt = j0; jt < jend; jt += 256) {
+    __syncthreads();
+    if (jt + (int)threadIdx.x < jend) pj_s[threadIdx.x] = pos[jt + threadIdx.x];
+    __syncthreads();
+    const int m = min(256, jend - jt);
+    if (i < n) {
+      for (int k = 0; k < m; ++k) {
+        const int j = jt + k;
+        if (j <= i) continue;
+        const double4 pj = pj_s[k];
+        double d[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          if (d[c] < -g.halfL[c]) d[c] += g.L[c];
+          if (d[c] > g.halfL[c]) d[c] -= g.L[c];
+        }
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double q = r / dr;
+        if (q < (double)nbins) {
+          const int bin = (int)q;
+          if (LDS_HIST) atomicAdd(&hist[bin], 1u);
+          else atomicAdd(&counts[bin], 1ull);
+        }
+      }
+    }
+  }
+  if (LDS_HIST) {
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+      if (hist[b]) atomicAdd(&counts[b], (unsigned long long)hist[b]);
   }
 }
 
@@ -1572,7 +1739,10 @@ __global__ __launch_bounds__(256) void k_step_fused(
   }
 }
 
-__global__ __launch_bounds__(1024) void k_finish_max(int nblk, const double* __restrict__ blockmax, Ctl* ctl) {
+// host_out (pinned host memory, may be null) receives the same bits, so the host
+// reads the step's displacement without a copy or a stream synchronisation.
+__global__ __launch_bounds__(1024) void k_finish_max(int nblk, const double* __restrict__ blockmax, Ctl* ctl,
+                                                     unsigned long long* host_out) {
   __shared__ double red[16];
   double m = 0.0;
   for (int b = threadIdx.x; b < nblk; b += blockDim.x) m = fmax(m, blockmax[b]);
@@ -1582,7 +1752,9 @@ __global__ __launch_bounds__(1024) void k_finish_max(int nblk, const double* __r
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t = fmax(t, red[w]);
-    ctl->maxsq_bits = (unsigned long long)__double_as_longlong(t);
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(t);
+    ctl->maxsq_bits = bits;
+    if (host_out) *reinterpret_cast<volatile unsigned long long*>(host_out) = bits;
   }
 }
 
@@ -1654,20 +1826,25 @@ int scan_blocks(int ncell) { return (ncell + kScanChunk - 1) / kScanChunk; }
 void launch_step_fused(hipStream_t s, int n, const int* id, double4* pos, double4* vel, double* fx, double* fy,
                        double* fz, int lang, double pref1, double pref2, unsigned long long seed,
                        unsigned long long call, const unsigned char* excl, int nid, int do_int2, double dt,
-                       uint4* posq, const Geom& g, double* blockmax, Ctl* ctl) {
+                       uint4* posq, const Geom& g, double* blockmax, Ctl* ctl, unsigned long long* host_out) {
   if (n <= 0) return;
   const int nb = nblocks(n, 256);
   hipLaunchKernelGGL(k_step_fused, dim3(nb), dim3(256), 0, s, n, id, pos, vel, fx, fy, fz, lang, pref1, pref2, seed,
                      call, excl, nid, do_int2, dt, posq, g, blockmax);
-  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, blockmax, ctl);
+  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, blockmax, ctl, host_out);
+}
+
+void launch_permute_int(hipStream_t s, int n, const int* order, const int* src, int* dst) {
+  if (n > 0) hipLaunchKernelGGL(k_permute_int, dim3(nblocks(n, 256)), dim3(256), 0, s, n, order, src, dst);
 }
 
 void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start, int* order) {
   if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, s, n, cell_of, rank, start, order);
 }
 
-void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const int* id) {
-  hipLaunchKernelGGL(k_sort_cells, dim3(nblocks(ncell, 256)), dim3(256), 0, s, ncell, start, order, id);
+void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const double4* pos) {
+  hipLaunchKernelGGL(k_sort_cells, dim3(nblocks(ncell, kSortThreads)), dim3(kSortThreads), 0, s, ncell, start, order,
+                     pos);
 }
 
 void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, const double4* pos,
@@ -1727,8 +1904,8 @@ void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum,
 void launch_tile_plan(hipStream_t s, int n, int T, int ntiles, const int* cell_of_slot, const int* cstart,
                       const Geom& g, int budget, TileRun* runs, int* nruns, Ctl* ctl) {
   if (ntiles > 0)
-    hipLaunchKernelGGL(k_tile_plan, dim3(nblocks(ntiles, 64)), dim3(64), 0, s, n, T, ntiles, cell_of_slot, cstart, g,
-                       budget, runs, nruns, ctl);
+    hipLaunchKernelGGL(k_tile_plan, dim3(nblocks(ntiles, kPlanThreads)), dim3(kPlanThreads), 0, s, n, T, ntiles,
+                       cell_of_slot, cstart, g, budget, runs, nruns, ctl);
 }
 
 void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Geom& g) {
@@ -1740,11 +1917,11 @@ static bool cubic_box(const Geom& g) { return g.qsd[0] == g.qsd[1] && g.qsd[1] =
 void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
-                            const int* ex_ids, uint4* nbr, int* cnt, int capg, Ctl* ctl) {
+                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot, Ctl* ctl) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
-#define NB(EX, CU) hipLaunchKernelGGL((k_nbr_build_tiled<EX, CU>), grid, block, 0, s, n, ntiles, posq, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, ctl)
+#define NB(EX, CU) hipLaunchKernelGGL((k_nbr_build_tiled<EX, CU>), grid, block, 0, s, n, ntiles, posq, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, pred, lane_slot, ctl)
   const bool cu = cubic_box(g);
   if (ex_start) { if (cu) NB(true, true); else NB(true, false); }
   else { if (cu) NB(false, true); else NB(false, false); }
@@ -1753,15 +1930,15 @@ void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, cons
 
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
-                     const Geom& g, const TileRun* runs, const int* nruns, const LJParam* prm, int ntypes,
-                     double* fx, double* fy, double* fz, double* partial) {
+                     const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
+                     int ntypes, double* fx, double* fy, double* fz, double* partial) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
   const bool cubic = cubic_box(g);
-#define LJM(OT, EN, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, true>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); \
-                             else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, false>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, prm, ntypes, fx, fy, fz, partial); } while (0)
-#define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64<OT, EN, AC>), grid, block, lds_bytes, s, n, ntiles, pos, nbr, cnt, capg, g, runs, nruns, prm, ntypes, fx, fy, fz, partial)
+#define LJM(OT, EN, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, true>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial); \
+                             else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, false>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial); } while (0)
+#define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64<OT, EN, AC>), grid, block, lds_bytes, s, n, ntiles, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial)
   if (mixed) {
     if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }
     else if (onetype) { if (accum) LJM(true, false, true); else LJM(true, false, false); }
@@ -1794,12 +1971,12 @@ void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, cons
 
 void launch_tab_tiled(hipStream_t s, bool mixed, bool energy, bool accum, int n, int T, int lds_bytes,
                       const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
-                      const Geom& g, const TileRun* runs, const int* nruns, const TabParam* prm, const double* pool,
-                      int ntypes, double* fx, double* fy, double* fz, double* partial) {
+                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const TabParam* prm,
+                      const double* pool, int ntypes, double* fx, double* fy, double* fz, double* partial) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
-#define TT(MX, EN, AC) hipLaunchKernelGGL((k_tab_tiled<MX, EN, AC>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, prm, pool, ntypes, fx, fy, fz, partial)
+#define TT(MX, EN, AC) hipLaunchKernelGGL((k_tab_tiled<MX, EN, AC>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, pool, ntypes, fx, fy, fz, partial)
   if (mixed) {
     if (energy) TT(true, true, false); else if (accum) TT(true, false, true); else TT(true, false, false);
   } else {
@@ -1826,6 +2003,20 @@ void launch_bond_tab(hipStream_t s, bool energy, bool accum, int n, const double
 #define BT(EN, AC) hipLaunchKernelGGL((k_bond_tab<EN, AC>), grid, block, 0, s, n, pos, id, slot_of_id, nid, bstart, bpartner, g, prm, pool, fx, fy, fz, partial)
   if (energy) BT(true, false); else if (accum) BT(false, true); else BT(false, false);
 #undef BT
+}
+
+void launch_rdf(hipStream_t s, int n, const double4* pos, const Geom& g, double dr, int nbins,
+                unsigned long long* counts) {
+  if (n <= 1 || nbins <= 0) return;
+  // j chunks keep gridDim.y within its limit and give every workgroup a few thousand pairs per i
+  int jchunk = 4096;
+  while ((n + jchunk - 1) / jchunk > 32768) jchunk *= 2;
+  dim3 grid(nblocks(n, 256), (n + jchunk - 1) / jchunk), block(256);
+  if (nbins <= kRdfLdsBins)
+    hipLaunchKernelGGL(k_rdf<true>, grid, block, (size_t)nbins * sizeof(unsigned int), s, n, pos, g, dr, nbins, jchunk,
+                       counts);
+  else
+    hipLaunchKernelGGL(k_rdf<false>, grid, block, 0, s, n, pos, g, dr, nbins, jchunk, counts);
 }
 
 void launch_zero3(hipStream_t s, int n, double* a, double* b, double* c) {

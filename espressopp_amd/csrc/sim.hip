@@ -90,7 +90,10 @@ struct VerletListState {
   double cut = 0.0;
   int cap = 0;          // entries per particle row (untiled) or 2 x words per row (tiled)
   bool tiled = false;   // entries are 16-bit offsets into the tile's staged region
-  DBuf<int> nbr, cnt;
+  DBuf<int> nbr, cnt;   // tiled: indexed by list lane, lane_slot maps a lane to its particle slot
+  DBuf<int> lane_slot;
+  DBuf<int> pred, pred2;   // list length of each slot's particle at the last build (lane order of the next)
+  int64_t pred_n = -1;     // particle count pred is valid for
   int64_t builds = 0;
   int maxcount = 0;
   bool stale = true;   // slot order changed since the last build
@@ -274,6 +277,13 @@ void table_rows(const HostTable& t, std::vector<double>& pool) {
   emit(t.e, &t.f);
 }
 
+// host -> device copy ordered on the handle's (non-blocking) stream; returns when done
+void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t stream, const char* what) {
+  if (!bytes) return;
+  hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), what);
+  hip_check(hipStreamSynchronize(stream), what);
+}
+
 }  // namespace
 
 struct espp_sim {
@@ -309,6 +319,9 @@ struct espp_sim {
 
   DBuf<Ctl> ctl;
   Ctl* hctl = nullptr;
+  // per-step max displacement, written by the device into coherent pinned memory
+  unsigned long long* hmax = nullptr;
+  unsigned long long* dmax = nullptr;     // device view of hmax
   DBuf<double> partial, red;
   double* hred = nullptr;
 
@@ -382,6 +395,7 @@ struct espp_sim {
     for (auto* f : fpls) delete f;
     for (auto* a : ias) delete a;
     if (hctl) (void)hipHostFree(hctl);
+    if (hmax) (void)hipHostFree(hmax);
     if (hred) (void)hipHostFree(hred);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -389,6 +403,27 @@ struct espp_sim {
   bool mixed() const { return precision == ESPP_MIXED; }
 
   void sync() { hip_check(hipStreamSynchronize(stream), "stream synchronize"); }
+
+  // Wait until the device has written *hmax (it was set to kNoValue before the
+  // launch).  The stream is queried now and then so that a failed launch or an
+  // idle stream without a value ends the wait with an error.
+  static constexpr unsigned long long kNoValue = ~0ull;   // never the bits of a max |dx|^2 >= 0
+  unsigned long long wait_host_value() {
+    for (unsigned long it = 1;; ++it) {
+      const unsigned long long v = __atomic_load_n(hmax, __ATOMIC_ACQUIRE);
+      if (v != kNoValue) return v;
+      if ((it & 1023) == 0) {
+        const hipError_t e = hipStreamQuery(stream);
+        if (e == hipSuccess) {
+          const unsigned long long w = __atomic_load_n(hmax, __ATOMIC_ACQUIRE);
+          if (w != kNoValue) return w;
+          fail(ESPP_ERR_DEVICE, "the integrate kernel finished without reporting its displacement");
+        }
+        if (e != hipErrorNotReady) hip_check(e, "integrate");
+      }
+      __builtin_ia32_pause();
+    }
+  }
 
   void read_ctl() {
     hip_check(hipMemcpyAsync(hctl, ctl.p, sizeof(Ctl), hipMemcpyDeviceToHost, stream), "ctl readback");
@@ -435,11 +470,18 @@ struct espp_sim {
     launch_scan_multi(stream, (int)ncell(), cell_count.p, cstart.p, scan_sum.p, scan_off.p);
     if (ni > 0) {
       launch_scatter(stream, ni, cell_of.p, rank.p, cstart.p, order.p);
-      launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, id.p);
+      launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, pos.p);
       launch_permute(stream, ni, order.p, cell_of.p, pos.p, vel.p, fx.p, fy.p, fz.p, id.p, img.p, pos2.p,
                      vel2.p, fx2.p, fy2.p, fz2.p, id2.p, img2.p, cell_of_slot.p, slot_of_id.p, posq.p, g);
       pos.swap(pos2); vel.swap(vel2); fx.swap(fx2); fy.swap(fy2); fz.swap(fz2);
       id.swap(id2); img.swap(img2);
+      for (auto* v : vls) {
+        if (v->pred_n == n && v->pred.n >= (size_t)n) {
+          v->pred2.resize((size_t)n);
+          launch_permute_int(stream, ni, order.p, v->pred.p, v->pred2.p);
+          v->pred.swap(v->pred2);
+        }
+      }
     }
     prof_end(4, pe);
     plan_valid = false;
@@ -528,9 +570,17 @@ struct espp_sim {
       if (v.tiled) {
         // 16-bit entries, eight per 16-byte group: cap / 2 ints per particle
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)(v.cap / 2) * 64);
+        v.lane_slot.resize((size_t)std::max<int64_t>(n, 1));
+        if (v.pred_n != n) {
+          // no list lengths yet for this particle set: lanes in slot order
+          v.pred.resize((size_t)std::max<int64_t>(n, 1));
+          launch_fill_int(stream, (int)n, v.pred.p, 0);
+          v.pred_n = n;
+        }
         launch_nbr_build_tiled(stream, (int)n, T, posq.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
                                cutsq, cut_lo_q, cut_hi_q, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
-                               v.has_excl ? v.ex_ids.p : nullptr, (uint4*)v.nbr.p, v.cnt.p, v.cap / 8, ctl.p);
+                               v.has_excl ? v.ex_ids.p : nullptr, (uint4*)v.nbr.p, v.cnt.p, v.cap / 8, v.pred.p,
+                               v.lane_slot.p, ctl.p);
       } else {
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
         launch_nbr_build(stream, (int)n, pos.p, id.p, cell_of_slot.p, cstart.p, g, cutsq,
@@ -566,9 +616,9 @@ struct espp_sim {
     for (int i = 0; i < nid; ++i) ids.insert(ids.end(), adj[(size_t)i].begin(), adj[(size_t)i].end());
     v.ex_start.resize(start.size());
     v.ex_ids.resize(std::max<size_t>(ids.size(), 1));
-    hip_check(hipMemcpy(v.ex_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice), "excl upload");
+    upload_bytes(v.ex_start.p, start.data(), start.size() * sizeof(int), stream, "excl upload");
     if (!ids.empty())
-      hip_check(hipMemcpy(v.ex_ids.p, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice), "excl upload");
+      upload_bytes(v.ex_ids.p, ids.data(), ids.size() * sizeof(int), stream, "excl upload");
     v.ex_nid = nid;
   }
 
@@ -588,8 +638,8 @@ struct espp_sim {
     }
     f.bstart.resize(deg.size());
     f.bpartner.resize(part.size());
-    hip_check(hipMemcpy(f.bstart.p, deg.data(), deg.size() * sizeof(int), hipMemcpyHostToDevice), "bond upload");
-    hip_check(hipMemcpy(f.bpartner.p, part.data(), part.size() * sizeof(int), hipMemcpyHostToDevice), "bond upload");
+    upload_bytes(f.bstart.p, deg.data(), deg.size() * sizeof(int), stream, "bond upload");
+    upload_bytes(f.bpartner.p, part.data(), part.size() * sizeof(int), stream, "bond upload");
     f.built_nid = nid;
     f.dirty = false;
   }
@@ -620,16 +670,20 @@ struct espp_sim {
         const double win = std::isfinite(rc) ? 4.0 * std::sqrt(3.0) * qmax * rc + 1e-300 : 0.0;
         p.dlo = p.drc2 - win;
         p.dhi = p.drc2 + win;
+        p.dmid = p.drc2;
+        p.dhw = win;
         p.uff1 = p.dff1 / std::pow(q2, 7);
         p.uff2 = p.dff2 / std::pow(q2, 4);
         p.urc2 = p.drc2 / q2;
         p.ulo = p.dlo / q2;
         p.uhi = p.dhi / q2;
         p.usig2 = p.sig2 / q2;
+        p.umid = p.urc2;
+        p.uhw = win / q2;
         t[(size_t)i * nt + j] = p;
       }
     a.dtab.resize(t.size());
-    hip_check(hipMemcpy(a.dtab.p, t.data(), t.size() * sizeof(LJParam), hipMemcpyHostToDevice), "lj table upload");
+    upload_bytes(a.dtab.p, t.data(), t.size() * sizeof(LJParam), stream, "lj table upload");
     a.dtab_nt = nt;
     a.dirty = false;
   }
@@ -677,8 +731,8 @@ struct espp_sim {
     if (pool.empty()) pool.push_back(0.0);
     a.ttab.resize(prm.size());
     a.tpool.resize(pool.size());
-    hip_check(hipMemcpy(a.ttab.p, prm.data(), prm.size() * sizeof(TabParam), hipMemcpyHostToDevice), "table upload");
-    hip_check(hipMemcpy(a.tpool.p, pool.data(), pool.size() * sizeof(double), hipMemcpyHostToDevice), "table upload");
+    upload_bytes(a.ttab.p, prm.data(), prm.size() * sizeof(TabParam), stream, "table upload");
+    upload_bytes(a.tpool.p, pool.data(), pool.size() * sizeof(double), stream, "table upload");
     a.dtab_nt = nt;
     a.dirty = false;
   }
@@ -690,7 +744,8 @@ struct espp_sim {
       hipEvent_t pe = energy ? nullptr : prof_begin();
       if (v.tiled)
         launch_tab_tiled(stream, mixed(), energy, accum, (int)n, T, max_region * (mixed() ? 16 : 32), posq.p, pos.p,
-                         (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, a.ttab.p, a.tpool.p,
+                         (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, v.lane_slot.p, a.ttab.p,
+                         a.tpool.p,
                          a.dtab_nt, fx.p, fy.p, fz.p, partial_out);
       else
         launch_tab(stream, energy, accum, (int)n, pos.p, v.nbr.p, v.cnt.p, v.cap, g, a.ttab.p, a.tpool.p, a.dtab_nt,
@@ -716,7 +771,8 @@ struct espp_sim {
       if (v.tiled)
         launch_lj_tiled(stream, mixed(), onetype, energy, accum, (int)n, T,
                         max_region * (mixed() ? (onetype ? 12 : 16) : 32), posq.p,
-                        pos.p, (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, a.dtab.p, a.dtab_nt,
+                        pos.p, (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, v.lane_slot.p,
+                        a.dtab.p, a.dtab_nt,
                         fx.p, fy.p, fz.p, partial_out);
       else
         launch_lj(stream, mixed(), onetype, energy, accum, (int)n, posq.p, pos.p, v.nbr.p, v.cnt.p, v.cap, g,
@@ -748,7 +804,7 @@ struct espp_sim {
     std::vector<unsigned char> f((size_t)std::max(nid, 1), 0);
     for (int32_t e : lang_excl) if (e >= 0 && e < nid) f[(size_t)e] = 1;
     excl_flag.resize(f.size());
-    hip_check(hipMemcpy(excl_flag.p, f.data(), f.size(), hipMemcpyHostToDevice), "langevin exclusion upload");
+    upload_bytes(excl_flag.p, f.data(), f.size(), stream, "langevin exclusion upload");
     excl_nid = nid;
     excl_dirty = false;
   }
@@ -800,14 +856,16 @@ struct espp_sim {
       // thermostat + integrate2 of the previous step fused with integrate1 of this one
       const bool first = (i == 0);
       hipEvent_t pe1 = prof_begin();
+      __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
       launch_step_fused(stream, (int)n, id.p, pos.p, vel.p, fx.p, fy.p, fz.p, (!first && lang) ? 1 : 0, pref1, pref2,
                         seed, (!first && lang) ? therm_calls++ : 0ull, excl_flag.p, excl_nid, first ? 0 : 1, dt,
-                        mixed() ? posq.p : nullptr, g, blockmax.p, ctl.p);
+                        mixed() ? posq.p : nullptr, g, blockmax.p, ctl.p, dmax);
       prof_end(2, pe1);
-      read_ctl();
-      double maxSq;
-      std::memcpy(&maxSq, &hctl->maxsq_bits, sizeof(double));
-      if (n == 0) maxSq = 0.0;
+      double maxSq = 0.0;
+      if (n > 0) {
+        const unsigned long long bits = wait_host_value();
+        std::memcpy(&maxSq, &bits, sizeof(double));
+      }
       maxDist += std::sqrt(maxSq);
       auto tb = clk::now();
       t_int1 += std::chrono::duration<double>(tb - ta).count();
@@ -904,9 +962,13 @@ void set_geom(Geom& g, const double box[3], const int32_t cells[3]) {
 }
 
 template <class T>
-std::vector<T> download(const DBuf<T>& b, size_t count) {
+std::vector<T> download(const DBuf<T>& b, size_t count, hipStream_t stream) {
+  // ordered after all work of the handle's (non-blocking) stream
   std::vector<T> h(count);
-  if (count) hip_check(hipMemcpy(h.data(), b.p, count * sizeof(T), hipMemcpyDeviceToHost), "download");
+  if (count) {
+    hip_check(hipMemcpyAsync(h.data(), b.p, count * sizeof(T), hipMemcpyDeviceToHost, stream), "download");
+    hip_check(hipStreamSynchronize(stream), "download");
+  }
   return h;
 }
 
@@ -959,6 +1021,8 @@ int espp_create(const double box[3], const int32_t cell_grid[3], double skin, ui
       hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
       s->ctl.resize(1);
       hip_check(hipHostMalloc((void**)&s->hctl, sizeof(Ctl), hipHostMallocDefault), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&s->hmax, 64, hipHostMallocCoherent | hipHostMallocMapped), "hipHostMalloc");
+      hip_check(hipHostGetDevicePointer((void**)&s->dmax, s->hmax, 0), "hipHostGetDevicePointer");
       hip_check(hipHostMalloc((void**)&s->hred, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
       s->red.resize(16);
       int64_t nc = s->ncell();
@@ -1039,13 +1103,13 @@ int espp_add_particles(espp_sim* s, int64_t n, const int32_t* id, const int32_t*
     }
     s->reserve_particles(s->n + n);
     size_t off = (size_t)s->n;
-    hip_check(hipMemcpy(s->pos.p + off, hp.data(), (size_t)n * sizeof(double4), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemcpy(s->vel.p + off, hv.data(), (size_t)n * sizeof(double4), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemcpy(s->img.p + off, hi.data(), (size_t)n * sizeof(int4), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemcpy(s->id.p + off, id, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemcpy(s->fx.p + off, zero.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemcpy(s->fy.p + off, zero.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemcpy(s->fz.p + off, zero.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice), "upload");
+    upload_bytes(s->pos.p + off, hp.data(), (size_t)n * sizeof(double4), s->stream, "upload");
+    upload_bytes(s->vel.p + off, hv.data(), (size_t)n * sizeof(double4), s->stream, "upload");
+    upload_bytes(s->img.p + off, hi.data(), (size_t)n * sizeof(int4), s->stream, "upload");
+    upload_bytes(s->id.p + off, id, (size_t)n * sizeof(int32_t), s->stream, "upload");
+    upload_bytes(s->fx.p + off, zero.data(), (size_t)n * sizeof(double), s->stream, "upload");
+    upload_bytes(s->fy.p + off, zero.data(), (size_t)n * sizeof(double), s->stream, "upload");
+    upload_bytes(s->fz.p + off, zero.data(), (size_t)n * sizeof(double), s->stream, "upload");
     s->n += n;
     s->host_ids.insert(s->host_ids.end(), newids.begin(), newids.end());
     s->nid = std::max(s->nid, maxid + 1);
@@ -1082,30 +1146,31 @@ int espp_get_particles(espp_sim* s, int32_t field, double* out) {
     size_t n = (size_t)s->n;
     if (n == 0) return;
     s->sync();
-    std::vector<int> ids = download(s->id, n);
+    std::vector<int> ids = download(s->id, n, s->stream);
     std::vector<int64_t> order = s->slots_by_id(ids);
     if (field == ESPP_FIELD_POS || field == ESPP_FIELD_TYPE) {
-      std::vector<double4> p = download(s->pos, n);
+      std::vector<double4> p = download(s->pos, n, s->stream);
       for (size_t k = 0; k < n; ++k) {
         const double4& q = p[(size_t)order[k]];
         if (field == ESPP_FIELD_POS) { out[3 * k] = q.x; out[3 * k + 1] = q.y; out[3 * k + 2] = q.z; }
         else out[k] = q.w;
       }
     } else if (field == ESPP_FIELD_VEL || field == ESPP_FIELD_MASS) {
-      std::vector<double4> v = download(s->vel, n);
+      std::vector<double4> v = download(s->vel, n, s->stream);
       for (size_t k = 0; k < n; ++k) {
         const double4& q = v[(size_t)order[k]];
         if (field == ESPP_FIELD_VEL) { out[3 * k] = q.x; out[3 * k + 1] = q.y; out[3 * k + 2] = q.z; }
         else out[k] = q.w;
       }
     } else if (field == ESPP_FIELD_FORCE) {
-      std::vector<double> a = download(s->fx, n), b = download(s->fy, n), c = download(s->fz, n);
+      std::vector<double> a = download(s->fx, n, s->stream), b = download(s->fy, n, s->stream),
+                          c = download(s->fz, n, s->stream);
       for (size_t k = 0; k < n; ++k) {
         size_t o = (size_t)order[k];
         out[3 * k] = a[o]; out[3 * k + 1] = b[o]; out[3 * k + 2] = c[o];
       }
     } else if (field == ESPP_FIELD_IMAGE) {
-      std::vector<int4> im = download(s->img, n);
+      std::vector<int4> im = download(s->img, n, s->stream);
       for (size_t k = 0; k < n; ++k) {
         const int4& q = im[(size_t)order[k]];
         out[3 * k] = q.x; out[3 * k + 1] = q.y; out[3 * k + 2] = q.z;
@@ -1123,7 +1188,7 @@ int espp_set_particles(espp_sim* s, int32_t field, int64_t m, const int32_t* ids
     if (m == 0) return;
     size_t n = (size_t)s->n;
     s->sync();
-    std::vector<int> dev_ids = download(s->id, n);
+    std::vector<int> dev_ids = download(s->id, n, s->stream);
     std::vector<int64_t> slot_of((size_t)std::max(s->nid, 1), -1);
     for (size_t k = 0; k < n; ++k) slot_of[(size_t)dev_ids[k]] = (int64_t)k;
     auto slot = [&](int32_t pid) {
@@ -1132,7 +1197,7 @@ int espp_set_particles(espp_sim* s, int32_t field, int64_t m, const int32_t* ids
       return (size_t)slot_of[(size_t)pid];
     };
     if (field == ESPP_FIELD_POS || field == ESPP_FIELD_TYPE) {
-      std::vector<double4> p = download(s->pos, n);
+      std::vector<double4> p = download(s->pos, n, s->stream);
       for (int64_t k = 0; k < m; ++k) {
         double4& q = p[slot(ids[k])];
         if (field == ESPP_FIELD_POS) { q.x = values[3 * k]; q.y = values[3 * k + 1]; q.z = values[3 * k + 2]; }
@@ -1142,17 +1207,17 @@ int espp_set_particles(espp_sim* s, int32_t field, int64_t m, const int32_t* ids
           s->maxtype = std::max(s->maxtype, (int)q.w);
         }
       }
-      hip_check(hipMemcpy(s->pos.p, p.data(), n * sizeof(double4), hipMemcpyHostToDevice), "upload");
+      upload_bytes(s->pos.p, p.data(), n * sizeof(double4), s->stream, "upload");
       // re-binning refreshes the cell order, the fixed-point mirror and the lists
       s->sorted = false;
     } else if (field == ESPP_FIELD_VEL || field == ESPP_FIELD_MASS) {
-      std::vector<double4> v = download(s->vel, n);
+      std::vector<double4> v = download(s->vel, n, s->stream);
       for (int64_t k = 0; k < m; ++k) {
         double4& q = v[slot(ids[k])];
         if (field == ESPP_FIELD_VEL) { q.x = values[3 * k]; q.y = values[3 * k + 1]; q.z = values[3 * k + 2]; }
         else { need(values[k] > 0.0, "particle mass must be positive"); q.w = values[k]; }
       }
-      hip_check(hipMemcpy(s->vel.p, v.data(), n * sizeof(double4), hipMemcpyHostToDevice), "upload");
+      upload_bytes(s->vel.p, v.data(), n * sizeof(double4), s->stream, "upload");
     } else {
       fail(ESPP_ERR_ARG, "field cannot be set");
     }
@@ -1211,7 +1276,7 @@ int espp_vl_size(espp_sim* s, int32_t vl, int64_t* npairs) {
     need(npairs != nullptr, "NULL argument");
     s->ensure_lists();
     s->sync();
-    std::vector<int> c = download(v.cnt, (size_t)s->n);
+    std::vector<int> c = download(v.cnt, (size_t)s->n, s->stream);
     int64_t tot = 0;
     for (int x : c) tot += x;
     *npairs = tot / 2;
@@ -1233,15 +1298,16 @@ int espp_vl_pairs(espp_sim* s, int32_t vl, int32_t* out, int64_t cap, int64_t* t
     s->ensure_lists();
     s->sync();
     size_t n = (size_t)s->n;
-    std::vector<int> c = download(v.cnt, n), ids = download(s->id, n);
+    std::vector<int> c = download(v.cnt, n, s->stream), ids = download(s->id, n, s->stream);
     const size_t rowlen = v.tiled ? (size_t)v.cap / 2 : (size_t)v.cap;
-    std::vector<int> nb = download(v.nbr, (size_t)((n + 63) / 64) * rowlen * 64);
+    std::vector<int> nb = download(v.nbr, (size_t)((n + 63) / 64) * rowlen * 64, s->stream);
     std::vector<TileRun> runs;
-    std::vector<int> nruns;
+    std::vector<int> nruns, lane_slot;
     if (v.tiled) {
       size_t ntiles = (n + (size_t)s->T - 1) / (size_t)s->T;
-      runs = download(s->runs, ntiles * kMaxRuns);
-      nruns = download(s->nruns, ntiles);
+      runs = download(s->runs, ntiles * kMaxRuns, s->stream);
+      nruns = download(s->nruns, ntiles, s->stream);
+      lane_slot = download(v.lane_slot, n, s->stream);
     }
     auto entry_slot = [&](size_t i, int q) -> int {
       if (!v.tiled) return nb[(i / 64) * rowlen * 64 + (size_t)q * 64 + (i % 64)];
@@ -1256,9 +1322,10 @@ int espp_vl_pairs(espp_sim* s, int32_t vl, int32_t* out, int64_t cap, int64_t* t
       return sr[r].slot_start + (e - sr[r].lds_off);
     };
     int64_t k = 0;
-    for (size_t i = 0; i < n; ++i) {
-      for (int q = 0; q < c[i]; ++q) {
-        int j = entry_slot(i, q);
+    for (size_t l = 0; l < n; ++l) {         // list lane l holds the list of slot i
+      const size_t i = v.tiled ? (size_t)lane_slot[l] : l;
+      for (int q = 0; q < c[l]; ++q) {
+        int j = entry_slot(l, q);
         if (ids[i] < ids[(size_t)j]) {
           if (k < cap) { out[2 * k] = ids[i]; out[2 * k + 1] = ids[(size_t)j]; }
           ++k;
@@ -1438,6 +1505,30 @@ int espp_tabbond_create(espp_sim* s, int32_t fpl, int32_t itype, int64_t n, cons
   });
 }
 
+int espp_rdf(espp_sim* s, int32_t nbins, double* out) {
+  return guarded([&] {
+    need(s && out, "NULL argument");
+    need(nbins > 0 && nbins <= (1 << 24), "number of bins out of range");
+    s->ensure_sorted();
+    DBuf<unsigned long long> counts;
+    counts.resize((size_t)nbins);
+    hip_check(hipMemsetAsync(counts.p, 0, (size_t)nbins * sizeof(unsigned long long), s->stream), "rdf clear");
+    // dr from the y edge: RadialDistrF assumes a cubic box (src/analysis/RadialDistrF.cpp:57-58)
+    const double dr = s->g.halfL[1] / (double)nbins;
+    launch_rdf(s->stream, (int)s->n, s->pos.p, s->g, dr, nbins, counts.p);
+    s->sync();
+    std::vector<unsigned long long> c = download(counts, (size_t)nbins, s->stream);
+    // normalisation of RadialDistrF.cpp:128-137; the factor is formed in long double as there (M_PIl)
+    const double rho = (double)s->n / (s->g.L[0] * s->g.L[1] * s->g.L[2]);
+    const double factor = (double)(2.0L * 3.141592653589793238462643383279502884L * (long double)dr *
+                                   (long double)rho * 1.0L * (long double)s->n);
+    for (int i = 0; i < nbins; ++i) {
+      const double radius = (i + 0.5) * dr;
+      out[i] = (double)c[(size_t)i] / (factor * (radius * radius + dr * dr / 12.0));
+    }
+  });
+}
+
 int espp_ia_energy(espp_sim* s, int32_t ia, double* e) {
   return guarded([&] {
     InteractionState& a = get_ia(s, ia);
@@ -1601,7 +1692,7 @@ int espp_vl_stats(espp_sim* s, int32_t vl, int64_t* entries, int32_t* cap, int32
     VerletListState& v = get_vl(s, vl);
     s->ensure_lists();
     s->sync();
-    std::vector<int> c = download(v.cnt, (size_t)s->n);
+    std::vector<int> c = download(v.cnt, (size_t)s->n, s->stream);
     int64_t tot = 0;
     for (int x : c) tot += x;
     if (entries) *entries = tot;

@@ -196,6 +196,10 @@ int espp_num_resorts(espp_sim *sim, int64_t *n);
  * Resort, Lost (seconds of the last run()). */
 int espp_timers(espp_sim *sim, double out10[10]);
 
+/* analysis.RadialDistrF(system).compute(nbins) (src/analysis/RadialDistrF.cpp:47-140):
+ * histogram of all pair distances up to L_y / 2 in nbins bins, normalised to g(r). */
+int espp_rdf(espp_sim *sim, int32_t nbins, double *out);
+
 /* Sum of m v^2 over all particles (src/analysis/Temperature.hpp:112-122). */
 int espp_kinetic(espp_sim *sim, double *mv2);
 

@@ -33,7 +33,8 @@
  *   - velocity Verlet loop and skin trigger: src/integrator/VelocityVerlet.cpp:63-169,236-305,307-371
  *   - Langevin: src/integrator/LangevinThermostat.cpp:141-214 (uniform noise,
  *       pref1=-gamma, pref2=sqrt(24 T gamma/dt), heatUp *sqrt(3) at run() start)
- *   - observables: src/analysis/Temperature.hpp:60-125, src/analysis/Pressure.cpp:41-121
+ *   - observables: src/analysis/Temperature.hpp:60-125, src/analysis/Pressure.cpp:41-121,
+ *       src/analysis/RadialDistrF.cpp:47-140
  *
  * Arithmetic follows the reference's operation order with no FMA contraction
  * (the reference builds with -std=c++17, ISO mode, baseline x86-64): build this
@@ -1170,6 +1171,49 @@ double or_ia_virial(or_sim *s, int ii, double *w6) {
     }
     if (w6) memcpy(w6, t, sizeof(t));
     return w;
+}
+
+/* RadialDistrF::computeArray (src/analysis/RadialDistrF.cpp:47-140): all pairs,
+ * minimum image by one +-L correction per component, bin = (int)(|d| / dr) with
+ * dr = (L_y / 2) / nbins, normalised by 2 pi dr rho N (r^2 + dr^2 / 12), the
+ * prefactor formed in long double as the reference does with M_PIl. */
+int or_rdf(or_sim *s, int nbins, double *out) {
+    if (nbins <= 0) { or_seterr(s, "nbins must be positive"); return -1; }
+    double *hist = (double *)calloc((size_t)nbins, sizeof(double));
+    if (!hist) { or_seterr(s, "out of memory"); return -1; }
+    const double dr = s->halfL[1] / (double)nbins;
+    const int64_t n = s->np;
+#pragma omp parallel num_threads(s->nthreads)
+    {
+        double *h = (double *)calloc((size_t)nbins, sizeof(double));
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t i = 0; i < n; ++i) {
+            const double *p1 = &s->x[3 * i];
+            for (int64_t j = i + 1; j < n; ++j) {
+                const double *p2 = &s->x[3 * j];
+                double d[3];
+                for (int c = 0; c < 3; ++c) {
+                    d[c] = p1[c] - p2[c];
+                    if (d[c] < -s->halfL[c]) d[c] += s->L[c];
+                    if (d[c] > s->halfL[c]) d[c] -= s->L[c];
+                }
+                double q = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) / dr;
+                if (q < (double)nbins) h[(int)q] += 1.0;
+            }
+        }
+#pragma omp critical
+        for (int b = 0; b < nbins; ++b) hist[b] += h[b];
+        free(h);
+    }
+    const double rho = (double)n / (s->L[0] * s->L[1] * s->L[2]);
+    const double factor = (double)(2.0L * 3.141592653589793238462643383279502884L * (long double)dr *
+                                   (long double)rho * 1.0L * (long double)n);
+    for (int b = 0; b < nbins; ++b) {
+        double radius = (b + 0.5) * dr;
+        out[b] = hist[b] / (factor * (radius * radius + dr * dr / 12.0));
+    }
+    free(hist);
+    return 0;
 }
 
 double or_temperature(or_sim *s) {

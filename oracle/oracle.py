@@ -62,6 +62,7 @@ def lib():
             "or_get_therm_calls": (u64, [vp]),
             "or_run": (i32, [vp, i32]),
             "or_temperature": (dbl, [vp]),
+            "or_rdf": (i32, [vp, i32, vp]),
             "or_pressure": (dbl, [vp]),
             "or_get_by_id": (i32, [vp, i32, vp]),
             "or_get_ids": (i32, [vp, vp]),
@@ -282,6 +283,11 @@ class OracleSim:
 
     def temperature(self):
         return lib().or_temperature(self._h)
+
+    def rdf(self, nbins):
+        out = np.zeros(int(nbins))
+        self._check(lib().or_rdf(self._h, int(nbins), _ptr(out)))
+        return out
 
     def pressure(self):
         return lib().or_pressure(self._h)

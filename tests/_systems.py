@@ -85,3 +85,70 @@ def force_rel_err(f, f_ref, floor_frac=0.0):
 def pair_set(pairs):
     p = np.sort(np.asarray(pairs, dtype=np.int64).reshape(-1, 2), axis=1)
     return set(map(tuple, p.tolist()))
+
+
+def melt_chains(nchains, nbeads, rho, seed, bond=0.97, min_dist=0.7, tries=500):
+    """Random-walk chains of fixed bond length in a cubic box of density rho whose
+    beads keep min_dist from every other bead (the overlap push-off of a
+    Kremer-Grest start), found by rejection on a cell grid."""
+    n = nchains * nbeads
+    L = (n / rho) ** (1.0 / 3.0)
+    rng = np.random.default_rng(seed)
+    nc = max(int(L / min_dist), 3)
+    cs = L / nc
+    grid = {}
+    pos = np.zeros((n, 3))
+    md2 = min_dist * min_dist
+
+    def free(p, skip):
+        c = np.floor(p / cs).astype(int)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in grid.get(((c[0] + dx) % nc, (c[1] + dy) % nc, (c[2] + dz) % nc), ()):
+                        if j == skip:
+                            continue
+                        d = pos[j] - p
+                        d -= L * np.round(d / L)
+                        if d @ d < md2:
+                            return False
+        return True
+
+    def put(k, p):
+        pos[k] = p
+        c = np.floor(p / cs).astype(int) % nc
+        grid.setdefault((c[0], c[1], c[2]), []).append(k)
+
+    def take(k):
+        c = np.floor(pos[k] / cs).astype(int) % nc
+        grid[(c[0], c[1], c[2])].remove(k)
+
+    bonds = []
+    for ch in range(nchains):
+        start = ch * nbeads
+        for _attempt in range(1000):            # a trapped chain is removed and regrown
+            while True:
+                p = rng.uniform(0, L, 3)
+                if free(p, -1):
+                    break
+            put(start, p)
+            grown = 1
+            for b in range(1, nbeads):
+                for _ in range(tries):
+                    u = rng.normal(size=3)
+                    cand = (pos[start + b - 1] + bond * u / np.linalg.norm(u)) % L
+                    if free(cand, start + b - 1):
+                        break
+                else:
+                    break
+                put(start + b, cand)
+                grown += 1
+            if grown == nbeads:
+                break
+            for k in range(start, start + grown):
+                take(k)
+        else:
+            raise RuntimeError("chain growth trapped")
+        bonds.extend((start + b - 1, start + b) for b in range(1, nbeads))
+    # beads are stored folded; bond vectors use the minimum image
+    return pos, (L, L, L), np.asarray(bonds, dtype=np.int32)

@@ -531,3 +531,21 @@ def test_parallel_tempering_replicas(gpu):
             assert reps[n][2].temperature == pytest.approx(T[m]) and reps[m][2].temperature == pytest.approx(T[n])
     assert [(m, n) for m, n, _, _, y in pt.log if y] == swaps
     pt.run(10)
+
+
+@pytest.mark.parametrize("nbins", [100, 500, 20000])
+def test_radial_distribution_matches_oracle(gpu, nbins):
+    # RadialDistrF on the GPU equals the oracle bit for bit: the same FP64 pair
+    # distances fall into the same bins (20000 bins: the global-histogram kernel)
+    pid, pos, _, box = read_fixture()
+    rc, skin = 2.5, 0.3
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells_for(box, rc, skin), rc, skin, "fp64", ids=pid)
+    # the same small moves on both sides, some particles leaving the box (no refold before a rebuild)
+    moved = pos + np.random.default_rng(3).uniform(-0.1, 0.1, pos.shape)
+    order = np.argsort(pid)
+    system.storage.setParticleArrays("pos", pid[order], moved[order])
+    ref.set("pos", moved[order])
+    g = np.asarray(espressopp.analysis.RadialDistrF(system).compute(nbins))
+    want = np.asarray(ref.rdf(nbins))
+    assert np.abs(g - want).max() <= 1e-12 * np.abs(want).max()
+    assert g[: int(0.5 / (box[1] / 2 / nbins))].max() == 0.0       # no pair closer than 0.5 sigma

@@ -296,3 +296,54 @@ def test_tabulated_bond_matches_fene():
         out[kind] = (s.get("f"), s.energy(ia))
     assert np.abs(out["tab"][0] - out["fene"][0]).max() < 1e-5
     assert out["tab"][1] == pytest.approx(out["fene"][1], rel=1e-8)
+
+
+# ---------------------------------------------------------------- radial distribution
+def _rdf_counts(g, nbins, n, box):
+    # back from g(r) to pair counts with the reference's normalisation
+    # (src/analysis/RadialDistrF.cpp:143-151)
+    L = np.asarray(box, dtype=np.float64)
+    dr = L[1] / 2 / nbins
+    rho = n / (L[0] * L[1] * L[2])
+    factor = 2.0 * math.pi * dr * rho * n
+    r = (np.arange(nbins) + 0.5) * dr
+    return np.asarray(g) * factor * (r * r + dr * dr / 12.0)
+
+
+def test_rdf_simple_cubic_shells():
+    # pair counts of an M^3 simple-cubic lattice: every lattice vector of the
+    # minimum-image cube with |v| < L/2 appears once per particle (each pair once)
+    M, a = 8, 1.0
+    g = (np.arange(M) + 0.25) * a
+    pos = np.array([[x, y, z] for x in g for y in g for z in g])
+    box = (M * a,) * 3
+    s = OracleSim(box, (2, 2, 2), 0.3)
+    s.add_particles(np.arange(len(pos)), pos)
+    s.decompose()
+    nbins = 40                                    # dr = 0.1: shells sit well inside bins
+    counts = _rdf_counts(s.rdf(nbins), nbins, len(pos), box)
+    want = np.zeros(nbins)
+    h = M // 2
+    for vx in range(-h, h + 1):
+        for vy in range(-h, h + 1):
+            for vz in range(-h, h + 1):
+                r = math.sqrt(vx * vx + vy * vy + vz * vz) * a
+                if 0 < r < M * a / 2 and max(abs(vx), abs(vy), abs(vz)) < h:
+                    want[int(r / (M * a / 2 / nbins))] += 1
+    want *= len(pos) / 2
+    assert np.allclose(counts, want, rtol=1e-9, atol=1e-6)
+
+
+def test_rdf_ideal_gas_is_one():
+    rng = np.random.default_rng(17)
+    n, L = 4000, 20.0
+    s = OracleSim((L, L, L), (2, 2, 2), 0.3, threads=4)
+    s.add_particles(np.arange(n), rng.uniform(0, L, (n, 3)))
+    s.decompose()
+    nbins = 20
+    g = np.asarray(s.rdf(nbins))
+    counts = _rdf_counts(g, nbins, n, (L, L, L))
+    sel = counts > 2000
+    assert sel.sum() >= 15
+    # Poisson error of each bin's count
+    assert np.all(np.abs(g[sel] - 1.0) < 5.0 / np.sqrt(counts[sel]))
