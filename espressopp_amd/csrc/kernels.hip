@@ -701,23 +701,42 @@ __device__ __forceinline__ int find_run_by_cell(const TileRun* sr, int nr, int c
   return lo;
 }
 
+// The part of a TileRun the force kernels keep in LDS: 8 bytes per run, with a
+// closing entry whose lds_off is the region size (a run's count is the next
+// run's lds_off minus its own).  The small table keeps the LDS of a force
+// workgroup low enough for four workgroups per CU.
+struct RunRef {
+  int slot_start, lds_off;
+};
+
+__device__ __forceinline__ int load_run_refs(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
+                                             RunRef* rr, int t) {
+  const int nr = nruns[t];
+  if ((int)threadIdx.x < nr) {
+    const TileRun R = runs[(size_t)t * kMaxRuns + threadIdx.x];
+    rr[threadIdx.x] = RunRef{R.slot_start, R.lds_off};
+    if ((int)threadIdx.x == nr - 1) rr[nr] = RunRef{0, R.lds_off + R.count};
+  }
+  if (nr == 0 && threadIdx.x == 0) rr[0] = RunRef{0, 0};
+  __syncthreads();
+  return nr;
+}
+
 // slot of LDS entry e (runs sorted by lds_off)
-__device__ __forceinline__ int slot_of_lds(const TileRun* sr, int nr, int e) {
+__device__ __forceinline__ int slot_of_lds(const RunRef* rr, int nr, int e) {
   int lo = 0, hi = nr - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (sr[mid].lds_off <= e) lo = mid; else hi = mid - 1;
+    if (rr[mid].lds_off <= e) lo = mid; else hi = mid - 1;
   }
-  return sr[lo].slot_start + (e - sr[lo].lds_off);
+  return rr[lo].slot_start + (e - rr[lo].lds_off);
 }
 
-template <typename V>
-__device__ __forceinline__ int stage_tile(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
-                                          const V* __restrict__ src, V* lds, TileRun* sr, int t) {
-  const int nr = nruns[t];
-  if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)t * kMaxRuns + threadIdx.x];
-  __syncthreads();
-  const int total = nr > 0 ? sr[nr - 1].lds_off + sr[nr - 1].count : 0;
+template <typename V, typename S, typename Conv>
+__device__ __forceinline__ int stage_region(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
+                                            const V* __restrict__ src, S* lds, RunRef* rr, int t, Conv conv) {
+  const int nr = load_run_refs(runs, nruns, rr, t);
+  const int total = rr[nr].lds_off;
   const int bs = blockDim.x;
   // eight independent loads in flight per thread, then eight LDS stores; a
   // thread's entries rise by bs, so its run index only ever moves forward
@@ -728,18 +747,24 @@ __device__ __forceinline__ int stage_tile(const TileRun* __restrict__ runs, cons
     for (int u = 0; u < 8; ++u) {
       const int e = base + u * bs + (int)threadIdx.x;
       if (e < total) {
-        while (r + 1 < nr && sr[r + 1].lds_off <= e) ++r;
-        v[u] = src[sr[r].slot_start + (e - sr[r].lds_off)];
+        while (rr[r + 1].lds_off <= e) ++r;
+        v[u] = src[rr[r].slot_start + (e - rr[r].lds_off)];
       }
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int e = base + u * bs + (int)threadIdx.x;
-      if (e < total) lds[e] = v[u];
+      if (e < total) lds[e] = conv(v[u]);
     }
   }
   __syncthreads();
   return nr;
+}
+
+template <typename V>
+__device__ __forceinline__ int stage_tile(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
+                                          const V* __restrict__ src, V* lds, RunRef* rr, int t) {
+  return stage_region(runs, nruns, src, lds, rr, t, [](const V& v) { return v; });
 }
 
 template <typename S>
@@ -752,31 +777,8 @@ __device__ __forceinline__ uint3 narrow_slot<uint3>(const uint4& v) { return mak
 // stage_tile for the fixed-point mirror, storing S (uint3 or uint4) per particle
 template <typename S>
 __device__ __forceinline__ int stage_tile_q(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
-                                            const uint4* __restrict__ src, S* lds, TileRun* sr, int t) {
-  const int nr = nruns[t];
-  if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)t * kMaxRuns + threadIdx.x];
-  __syncthreads();
-  const int total = nr > 0 ? sr[nr - 1].lds_off + sr[nr - 1].count : 0;
-  const int bs = blockDim.x;
-  int r = 0;   // run of the thread's current entry (entries rise by bs)
-  for (int base = 0; base < total; base += 8 * bs) {
-    uint4 v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int e = base + u * bs + (int)threadIdx.x;
-      if (e < total) {
-        while (r + 1 < nr && sr[r + 1].lds_off <= e) ++r;
-        v[u] = src[sr[r].slot_start + (e - sr[r].lds_off)];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int e = base + u * bs + (int)threadIdx.x;
-      if (e < total) lds[e] = narrow_slot<S>(v[u]);
-    }
-  }
-  __syncthreads();
-  return nr;
+                                            const uint4* __restrict__ src, S* lds, RunRef* rr, int t) {
+  return stage_region(runs, nruns, src, lds, rr, t, [](const uint4& v) { return narrow_slot<S>(v); });
 }
 
 // d.sqr() <= cutsq of Real3D (src/Real3D.hpp), evaluated left to right without contraction
@@ -1017,7 +1019,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
   using Slot = typename std::conditional<ONETYPE, uint3, uint4>::type;
   extern __shared__ uint4 lds_raw[];
   Slot* lds = reinterpret_cast<Slot*>(lds_raw);
-  __shared__ TileRun sr[kMaxRuns];
+  __shared__ RunRef sr[kMaxRuns + 1];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = stage_tile_q<Slot>(runs, nruns, posq, lds, sr, tile);
   const int L = tile * blockDim.x + threadIdx.x;   // list lane
@@ -1141,7 +1143,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_fp64(
     int ntypes, double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
     double* __restrict__ partial) {
   extern __shared__ double4 ldsd[];
-  __shared__ TileRun sr[kMaxRuns];
+  __shared__ RunRef sr[kMaxRuns + 1];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = stage_tile<double4>(runs, nruns, pos, ldsd, sr, tile);
   const int L = tile * blockDim.x + threadIdx.x;   // list lane
@@ -1299,7 +1301,7 @@ __global__ __launch_bounds__(256) void k_tab_tiled(
     const double* __restrict__ pool, int ntypes, double* __restrict__ fx, double* __restrict__ fy,
     double* __restrict__ fz, double* __restrict__ partial) {
   extern __shared__ uint4 lds_raw[];
-  __shared__ TileRun sr[kMaxRuns];
+  __shared__ RunRef sr[kMaxRuns + 1];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   int nr;
   if (MIXED) nr = stage_tile_q<uint4>(runs, nruns, posq, lds_raw, sr, tile);
