@@ -61,6 +61,10 @@ SIGNATURES = {
     "espp_timers": [_vp, _vp],
     "espp_kinetic": [_vp, _P(_c_dbl)],
     "espp_rdf": [_vp, _c_int, _vp],
+    "espp_get_rng_state": [_vp, _P(_c_u64), _P(_c_u64)],
+    "espp_set_rng_state": [_vp, _c_u64, _c_u64],
+    "espp_get_run_state": [_vp, _P(_c_dbl), _P(_c_int)],
+    "espp_set_run_state": [_vp, _c_dbl, _c_int],
     "espp_kinetic_tensor": [_vp, _c_u64, _vp, _vp, _vp],
 }
 

@@ -1505,6 +1505,39 @@ int espp_tabbond_create(espp_sim* s, int32_t fpl, int32_t itype, int64_t n, cons
   });
 }
 
+int espp_get_rng_state(espp_sim* s, uint64_t* seed, uint64_t* calls) {
+  return guarded([&] {
+    need(s && seed && calls, "NULL argument");
+    *seed = s->seed;
+    *calls = s->therm_calls;
+  });
+}
+
+int espp_set_rng_state(espp_sim* s, uint64_t seed, uint64_t calls) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    s->seed = seed;
+    s->therm_calls = calls;
+  });
+}
+
+int espp_get_run_state(espp_sim* s, double* maxdist, int32_t* resort) {
+  return guarded([&] {
+    need(s && maxdist && resort, "NULL argument");
+    *maxdist = s->maxDist;
+    *resort = s->resortFlag ? 1 : 0;
+  });
+}
+
+int espp_set_run_state(espp_sim* s, double maxdist, int32_t resort) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    need(maxdist >= 0.0, "negative displacement");
+    s->maxDist = maxdist;
+    s->resortFlag = resort != 0;
+  });
+}
+
 int espp_rdf(espp_sim* s, int32_t nbins, double* out) {
   return guarded([&] {
     need(s && out, "NULL argument");

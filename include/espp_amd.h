@@ -196,6 +196,17 @@ int espp_num_resorts(espp_sim *sim, int64_t *n);
  * Resort, Lost (seconds of the last run()). */
 int espp_timers(espp_sim *sim, double out10[10]);
 
+/* esutil::RNG saveState / loadState (src/esutil/RNG.cpp:86-143).  The Langevin
+ * noise here is Philox4x32-10 keyed by (seed, thermostat call, particle id), so
+ * its whole state is the seed and the number of thermostat calls made. */
+int espp_get_rng_state(espp_sim *sim, uint64_t *seed, uint64_t *calls);
+int espp_set_rng_state(espp_sim *sim, uint64_t seed, uint64_t calls);
+
+/* VelocityVerlet's skin bookkeeping between runs (src/integrator/VelocityVerlet.cpp:63-169):
+ * the accumulated displacement maxDist and whether a resort is pending. */
+int espp_get_run_state(espp_sim *sim, double *maxdist, int32_t *resort);
+int espp_set_run_state(espp_sim *sim, double maxdist, int32_t resort);
+
 /* analysis.RadialDistrF(system).compute(nbins) (src/analysis/RadialDistrF.cpp:47-140):
  * histogram of all pair distances up to L_y / 2 in nbins bins, normalised to g(r). */
 int espp_rdf(espp_sim *sim, int32_t nbins, double *out);

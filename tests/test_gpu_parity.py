@@ -549,3 +549,50 @@ def test_radial_distribution_matches_oracle(gpu, nbins):
     want = np.asarray(ref.rdf(nbins))
     assert np.abs(g - want).max() <= 1e-12 * np.abs(want).max()
     assert g[: int(0.5 / (box[1] / 2 / nbins))].max() == 0.0       # no pair closer than 0.5 sigma
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checkpoint_resume_continues_the_run(gpu, tmp_path, precision):
+    # a thermostatted run split by a checkpoint and a restore into a fresh system continues
+    # the same Langevin noise and rebuild schedule: it ends where the unbroken run ends (up to
+    # summation order, the restored system's cells being sorted afresh)
+    pos, box = jittered_lattice(3000, 0.8442, seed=51)
+    vel = mb_velocities(3000, 1.0, seed=52)
+    cells = cells_for(box, 2.5, 0.3)
+
+    def make():
+        return gpu_system(box, cells, 0.3, precision, seed=77)
+
+    def attach(system):
+        vl = espressopp.VerletList(system, cutoff=2.5)
+        lj = espressopp.interaction.VerletListLennardJones(vl)
+        lj.setPotential(0, 0, espressopp.interaction.LennardJones(1.0, 1.0, 2.5, "auto"))
+        system.addInteraction(lj)
+        integ = espressopp.integrator.VelocityVerlet(system)
+        integ.dt = 0.005
+        lang = espressopp.integrator.LangevinThermostat(system)
+        lang.gamma = 1.0
+        lang.temperature = 1.0
+        integ.addExtension(lang)
+        return integ
+
+    a = make()
+    a.storage.addParticlesArray(np.arange(3000), pos, None, vel)
+    a.storage.decompose()
+    ia = attach(a)
+    ia.run(60)
+    fn = str(tmp_path / "chk.npz")
+    espressopp.tools.checkpoint.save(fn, a, ia)
+    ia.run(60)
+    b = make()
+    ib = attach(b)
+    espressopp.tools.checkpoint.restore(fn, b, ib)
+    assert ib.step == 60
+    ib.run(60)
+    pa, pb = a.storage.getParticleArrays(), b.storage.getParticleArrays()
+    ua = pa["pos"] + pa["image"] * np.asarray(box)
+    ub = pb["pos"] + pb["image"] * np.asarray(box)
+    assert np.array_equal(pa["id"], pb["id"])
+    assert np.abs(ua - ub).max() < 1e-8
+    assert np.abs(pa["v"] - pb["v"]).max() < 1e-7
+    assert ia.step == ib.step == 120

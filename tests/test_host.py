@@ -95,3 +95,68 @@ def test_tabulated_potential_checks_its_table(tmp_path):
         espressopp.interaction.Tabulated(2, str(fn), 2.0)     # Akima needs four rows
     with pytest.raises(ValueError):
         espressopp.interaction.Tabulated(4, str(fn), 2.0)
+
+
+_LAMMPS_BONDED = """LAMMPS data file, a short chain
+
+3 atoms
+2 bonds
+0 angles
+0 dihedrals
+0 impropers
+
+1 atom types
+1 bond types
+0 angle types
+
+0.0 10.0 xlo xhi
+-1.0 9.0 ylo yhi
+2.0 14.0 zlo zhi
+
+Atoms
+
+1 1 1 1.0 2.0 3.0
+2 1 1 1.5 2.5 3.5
+3 1 1 2.0 3.0 4.5
+
+Bonds
+
+1 1 1 2
+2 1 2 3
+"""
+
+_LAMMPS_TYPES = """LAMMPS data file, charges
+
+2 atoms
+0 bonds
+0 angles
+0 dihedrals
+0 impropers
+
+2 atom types
+
+0.0 5.0 xlo xhi
+0.0 6.0 ylo yhi
+0.0 7.0 zlo zhi
+
+Atoms
+
+1 1 2 -0.5 1.0 1.0 1.0
+2 1 1 0.5 2.0 2.0 2.0
+"""
+
+
+def test_lammps_read(tmp_path):
+    # src/tools/lammps.py:33-156: return tuples by content
+    from espressopp_amd.tools import lammps
+    fn = tmp_path / "bonded.data"
+    fn.write_text(_LAMMPS_BONDED)
+    bonds, x, y, z, Lx, Ly, Lz = lammps.read(str(fn))
+    assert bonds == [(1, 2), (2, 3)]
+    assert x == [1.0, 1.5, 2.0] and z == [3.0, 3.5, 4.5]
+    assert (Lx, Ly, Lz) == (10.0, 10.0, 12.0)
+    fn2 = tmp_path / "types.data"
+    fn2.write_text(_LAMMPS_TYPES)
+    p_type, bonds, angles, q, x, y, z, Lx, Ly, Lz = lammps.read(str(fn2))
+    assert p_type == [2, 1] and q == [-0.5, 0.5] and bonds == [] and angles == []
+    assert (x, Lx, Ly, Lz) == ([1.0, 2.0], 5.0, 6.0, 7.0)
