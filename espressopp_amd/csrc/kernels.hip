@@ -262,53 +262,68 @@ __global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __r
   order[start[cell_of[i]] + rank[i]] = i;
 }
 
-// Inside each cell, order members by x, ties by previous slot (insertion sort;
-// cells hold a few tens of particles at liquid densities).  The order is
-// deterministic, and a row of consecutive cells is sorted by x, which lets the
-// list build clip a row to the candidates within reach in x.
-constexpr int kSortThreads = 64, kSortLds = 48;   // cells with up to kSortLds members sort in LDS
-__global__ __launch_bounds__(kSortThreads) void k_sort_cells(int ncell, const int* __restrict__ start,
-                                                             int* __restrict__ order, const double4* __restrict__ pos) {
-  __shared__ double sx[kSortLds * kSortThreads];
-  __shared__ int so[kSortLds * kSortThreads];
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+// Inside each cell, order members by x, ties by previous slot.  A cell is
+// handled by 32 lanes: its members' keys are staged in LDS and each member's
+// place is the number of members with a smaller key (cells hold a few tens of
+// particles at liquid densities; larger cells take several members per lane).
+// The order is deterministic, and a row of consecutive cells is sorted by x,
+// which lets the list build clip a row to the candidates within reach in x.
+constexpr int kSortLanes = 32, kSortCellsPerBlock = 8, kSortMax = 96;
+
+// Sort key of a coordinate: the bits of the double mapped to a signed integer
+// whose order is the numeric order.  It is a total order (NaNs included), so
+// the ranks below always form a permutation.
+__device__ __forceinline__ long long x_key(double x) {
+  const long long b = __double_as_longlong(x);
+  return b ^ ((b >> 63) & 0x7fffffffffffffffll);
+}
+
+__global__ __launch_bounds__(kSortLanes * kSortCellsPerBlock) void k_sort_cells(
+    int ncell, const int* __restrict__ start, int* __restrict__ order, const double4* __restrict__ pos) {
+  __shared__ long long sx[kSortCellsPerBlock][kSortMax];
+  __shared__ int so[kSortCellsPerBlock][kSortMax];
+  const int w = threadIdx.x / kSortLanes, l = threadIdx.x % kSortLanes;
+  const int c = blockIdx.x * kSortCellsPerBlock + w;
+  const bool live = c < ncell;
+  const int b = live ? start[c] : 0, m = live ? start[c + 1] - b : 0;
+  const bool fits = m <= kSortMax;
+  if (live && fits) {
+    for (int k = l; k < m; k += kSortLanes) {
+      const int o = order[b + k];
+      so[w][k] = o;
+      sx[w][k] = x_key(pos[o].x);
+    }
+  }
+  __syncthreads();
+  if (live && fits) {
+    for (int k = l; k < m; k += kSortLanes) {
+      const long long xk = sx[w][k];
+      const int ok = so[w][k];
+      int rank = 0;
+      for (int q = 0; q < m; ++q) {
+        const long long xq = sx[w][q];
+        rank += (xq < xk || (xq == xk && so[w][q] < ok)) ? 1 : 0;
+      }
+      order[b + rank] = ok;
+    }
+  }
+}
+
+// The same ordering for cells with more than kSortMax members (one thread per
+// cell, insertion sort in global memory); every other cell returns at once.
+__global__ void k_sort_cells_big(int ncell, const int* __restrict__ start, int* __restrict__ order,
+                                 const double4* __restrict__ pos) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const int b = start[c], e = start[c + 1];
-  // key (x, previous slot); members were x-sorted at the last binning and move
-  // little, so the insertion sort does few moves
-  if (e - b <= kSortLds) {
-    // keys gathered once (independent loads), sorted in LDS (entries interleaved by thread)
-    const int m0 = e - b, t = threadIdx.x;
-    for (int k = 0; k < m0; ++k) {
-      const int o = order[b + k];
-      so[k * kSortThreads + t] = o;
-      sx[k * kSortThreads + t] = pos[o].x;
-    }
-    for (int k = 1; k < m0; ++k) {
-      const int o = so[k * kSortThreads + t];
-      const double xo = sx[k * kSortThreads + t];
-      int m = k - 1;
-      while (m >= 0) {
-        const double xm = sx[m * kSortThreads + t];
-        const int om = so[m * kSortThreads + t];
-        if (!(xm > xo || (xm == xo && om > o))) break;
-        sx[(m + 1) * kSortThreads + t] = xm;
-        so[(m + 1) * kSortThreads + t] = om;
-        --m;
-      }
-      sx[(m + 1) * kSortThreads + t] = xo;
-      so[(m + 1) * kSortThreads + t] = o;
-    }
-    for (int k = 0; k < m0; ++k) order[b + k] = so[k * kSortThreads + t];
-    return;
-  }
+  if (e - b <= kSortMax) return;
   for (int k = b + 1; k < e; ++k) {
     const int o = order[k];
-    const double xo = pos[o].x;
+    const long long xo = x_key(pos[o].x);
     int m = k - 1;
     while (m >= b) {
       const int om = order[m];
-      const double xm = pos[om].x;
+      const long long xm = x_key(pos[om].x);
       if (!(xm > xo || (xm == xo && om > o))) break;
       order[m + 1] = om;
       --m;
@@ -333,9 +348,11 @@ __global__ void k_permute(int n, const int* __restrict__ order, const int* __res
   double4 p = pos[o];
   pos2[s] = p;
   vel2[s] = vel[o];
-  fx2[s] = fx[o];
-  fy2[s] = fy[o];
-  fz2[s] = fz[o];
+  if (fx2) {   // null when the forces are recomputed right after the binning
+    fx2[s] = fx[o];
+    fy2[s] = fy[o];
+    fz2[s] = fz[o];
+  }
   int pid = id[o];
   id2[s] = pid;
   img2[s] = img[o];
@@ -1845,8 +1862,9 @@ void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, c
 }
 
 void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const double4* pos) {
-  hipLaunchKernelGGL(k_sort_cells, dim3(nblocks(ncell, kSortThreads)), dim3(kSortThreads), 0, s, ncell, start, order,
-                     pos);
+  hipLaunchKernelGGL(k_sort_cells, dim3(nblocks(ncell, kSortCellsPerBlock)), dim3(kSortLanes * kSortCellsPerBlock), 0,
+                     s, ncell, start, order, pos);
+  hipLaunchKernelGGL(k_sort_cells_big, dim3(nblocks(ncell, 256)), dim3(256), 0, s, ncell, start, order, pos);
 }
 
 void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, const double4* pos,

@@ -456,7 +456,7 @@ struct espp_sim {
 
   // Fold (optionally), bin into cells, sort each cell by id and permute every
   // per-particle array into cell order.
-  void bin(bool do_fold) {
+  void bin(bool do_fold, bool carry_forces = true) {
     int ni = (int)n;
     hipEvent_t pe = prof_begin();
     launch_reset_ctl(stream, ctl.p);
@@ -472,8 +472,10 @@ struct espp_sim {
       launch_scatter(stream, ni, cell_of.p, rank.p, cstart.p, order.p);
       launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, pos.p);
       launch_permute(stream, ni, order.p, cell_of.p, pos.p, vel.p, fx.p, fy.p, fz.p, id.p, img.p, pos2.p,
-                     vel2.p, fx2.p, fy2.p, fz2.p, id2.p, img2.p, cell_of_slot.p, slot_of_id.p, posq.p, g);
-      pos.swap(pos2); vel.swap(vel2); fx.swap(fx2); fy.swap(fy2); fz.swap(fz2);
+                     vel2.p, carry_forces ? fx2.p : nullptr, fy2.p, fz2.p, id2.p, img2.p, cell_of_slot.p,
+                     slot_of_id.p, posq.p, g);
+      pos.swap(pos2); vel.swap(vel2);
+      if (carry_forces) { fx.swap(fx2); fy.swap(fy2); fz.swap(fz2); }
       id.swap(id2); img.swap(img2);
       for (auto* v : vls) {
         if (v->pred_n == n && v->pred.n >= (size_t)n) {
@@ -644,8 +646,8 @@ struct espp_sim {
     f.dirty = false;
   }
 
-  void decompose() {
-    bin(true);
+  void decompose(bool carry_forces = true) {
+    bin(true, carry_forces);
     for (auto* v : vls) build_vl(*v);
   }
 
@@ -871,7 +873,7 @@ struct espp_sim {
       t_int1 += std::chrono::duration<double>(tb - ta).count();
       if (maxDist > skinHalf) resortFlag = true;
       if (resortFlag) {
-        decompose();
+        decompose(false);   // forces are recomputed below
         maxDist = 0.0;
         resortFlag = false;
         nResorts++;
