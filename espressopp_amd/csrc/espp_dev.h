@@ -31,10 +31,9 @@ struct LJParam {
   float ff1, ff2, rc2, shift;      // FP32 copies (untiled mixed kernel)
   double dff1, dff2, drc2, dshift; // FP64, length units
   double sig2, eps4;               // energy: 4 eps ((sig2/r2)^6 - (sig2/r2)^3)
-  double dlo, dhi;                 // cutoff window (length units) settled from FP64 positions
-  double dmid, dhw;                // the window as centre and half width
+  double dlo, dhi;                 // window of r^2 (length units) settled from FP64 positions
   // the same in fixed-point units of a cubic box (r^2 unit = (L / 2^(32+k))^2)
-  double uff1, uff2, urc2, ulo, uhi, usig2, umid, uhw;
+  double uff1, uff2, urc2, ulo, uhi, usig2;
   int set;
   int pad;
 };

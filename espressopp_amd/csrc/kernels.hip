@@ -72,6 +72,18 @@ __device__ __forceinline__ double lj_ff_seeded(double r2, double ff1, double ff2
   return r6 * __builtin_fma(ff1, r6, -ff2) * u;
 }
 
+// lj_ff_seeded for a pair that may be out of range: such a pair seeds the
+// reciprocal with rcp(+inf) = 0, after which u, r6 and the factor are exactly 0
+// (r2 is finite), so no select on the FP64 result is needed.
+__device__ __forceinline__ double lj_ff_masked(double r2, bool in, double ff1, double ff2, double& inv) {
+  const float rf = in ? (float)r2 : __builtin_inff();
+  double u = (double)__builtin_amdgcn_rcpf(rf);
+  u = u * __builtin_fma(-r2, u, 2.0);
+  inv = u;
+  const double r6 = u * u * u;
+  return r6 * __builtin_fma(ff1, r6, -ff2) * u;
+}
+
 // ------------------------------------------------------------------ binning
 
 // Rank of this lane among all particles counted into cell `key`: lanes of one
@@ -1051,7 +1063,10 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     const int rowi = (int)qi.w * ntypes;
     const LJParam& p0 = prm[0];
     double ff1 = CUBIC ? p0.uff1 : p0.dff1, ff2 = CUBIC ? p0.uff2 : p0.dff2;
-    double rc2 = CUBIC ? p0.urc2 : p0.drc2, mid = CUBIC ? p0.umid : p0.dmid, hw = CUBIC ? p0.uhw : p0.dhw;
+    // r^2 below lo is inside the cutoff; r^2 in [lo, hi] may be on either side
+    // after fixed-point quantisation and is settled after the batch from the
+    // FP64 positions; r^2 above hi is outside
+    double lo = CUBIC ? p0.ulo : p0.dlo, hi = CUBIC ? p0.uhi : p0.dhi;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     const int ng = (c + 7) >> 3;
     uint4 wn = ng > 0 ? np[0] : make_uint4(0u, 0u, 0u, 0u);
@@ -1069,18 +1084,16 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
           if constexpr (!ONETYPE) tj = (int)qj.w;
           const LJParam& p = prm[rowi + tj];
           ff1 = CUBIC ? p.uff1 : p.dff1; ff2 = CUBIC ? p.uff2 : p.dff2;
-          rc2 = p.set ? (CUBIC ? p.urc2 : p.drc2) : -1.0;
-          mid = CUBIC ? p.umid : p.dmid; hw = p.set ? (CUBIC ? p.uhw : p.dhw) : -1.0;
+          lo = p.set ? (CUBIC ? p.ulo : p.dlo) : -1.0;
+          hi = p.set ? (CUBIC ? p.uhi : p.dhi) : -1.0;
         }
         double ex, ey, ez;
         const double r2 = pair_r2<CUBIC>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
-        // pairs in the window around rc^2 are settled after the batch from the FP64 positions
-        const bool near = __builtin_fabs(r2 - mid) <= hw;
-        const bool in = valid & !near & (r2 <= rc2);
-        anynear |= valid & near;
+        const bool in = valid & (r2 < lo);
+        // in the window: at most hi and not in (in implies at most hi, hence the xor)
+        anynear |= (valid & (r2 <= hi)) ^ in;
         double inv;
-        double ff = lj_ff_seeded(r2, ff1, ff2, inv);
-        ff = in ? ff : 0.0;
+        const double ff = lj_ff_masked(r2, in, ff1, ff2, inv);
         Ax = __builtin_fma(ex, ff, Ax);
         Ay = __builtin_fma(ey, ff, Ay);
         Az = __builtin_fma(ez, ff, Az);
@@ -1107,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
           if (!p.set) continue;
           double ex, ey, ez;
           const double r2 = pair_r2<CUBIC>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
-          if (!(__builtin_fabs(r2 - (CUBIC ? p.umid : p.dmid)) <= (CUBIC ? p.uhw : p.dhw))) continue;
+          if (r2 < (CUBIC ? p.ulo : p.dlo) || !(r2 <= (CUBIC ? p.uhi : p.dhi))) continue;
           const int j = slot_of_lds(sr, nr, e);
           const double4 pi = pos[i], pj = pos[j];
           double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;

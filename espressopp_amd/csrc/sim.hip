@@ -672,16 +672,12 @@ struct espp_sim {
         const double win = std::isfinite(rc) ? 4.0 * std::sqrt(3.0) * qmax * rc + 1e-300 : 0.0;
         p.dlo = p.drc2 - win;
         p.dhi = p.drc2 + win;
-        p.dmid = p.drc2;
-        p.dhw = win;
         p.uff1 = p.dff1 / std::pow(q2, 7);
         p.uff2 = p.dff2 / std::pow(q2, 4);
         p.urc2 = p.drc2 / q2;
         p.ulo = p.dlo / q2;
         p.uhi = p.dhi / q2;
         p.usig2 = p.sig2 / q2;
-        p.umid = p.urc2;
-        p.uhw = win / q2;
         t[(size_t)i * nt + j] = p;
       }
     a.dtab.resize(t.size());
