@@ -64,6 +64,23 @@ struct TileRun {
 constexpr int kMaxRuns = 64;      // runs per tile
 constexpr int kMaxPieces = 128;   // row segments examined while planning a tile
 
+// Integrator arguments of a step applied per particle: the thermostat on the
+// force (LangevinThermostat, aftCalcF), integrate2 when do_int2 is set, then
+// integrate1 (VelocityVerlet).  pos is read and pos_out / posq_out written
+// (the same buffers when the step runs in place).
+struct StepArgs {
+  const int* id;
+  double4* vel;
+  const double4* pos;
+  double4* pos_out;
+  uint4* posq_out;                 // fixed-point mirror of pos_out (null: none)
+  const unsigned char* excl;       // thermostat exclusion flag per id (null: none)
+  double pref1, pref2, dt;
+  unsigned long long seed, call;
+  int lang, nid, do_int2, pad;
+  double* blockmax;                // per-block max |dx|^2
+};
+
 // Per-call control block kept on the device.
 struct Ctl {
   unsigned long long maxsq_bits;   // max |dx|^2 of the last integrate1 (non-negative double bits)

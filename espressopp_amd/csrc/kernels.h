@@ -9,10 +9,9 @@ void launch_fold_cell(hipStream_t s, int n, double4* pos, int4* img, int* cell_o
 void launch_scan_cells(hipStream_t s, int ncell, const int* count, int* start);
 void launch_scan_multi(hipStream_t s, int ncell, const int* count, int* start, int* blocksum, int* blockoff);
 int scan_blocks(int ncell);
-void launch_step_fused(hipStream_t s, int n, const int* id, double4* pos, double4* vel, double* fx, double* fy,
-                       double* fz, int lang, double pref1, double pref2, unsigned long long seed,
-                       unsigned long long call, const unsigned char* excl, int nid, int do_int2, double dt,
-                       uint4* posq, const Geom& g, double* blockmax, Ctl* ctl, unsigned long long* host_out);
+// blockmax of a needs nblocks(n, 256) entries
+void launch_step_fused(hipStream_t s, int n, double* fx, double* fy, double* fz, const StepArgs& a, const Geom& g,
+                       Ctl* ctl, unsigned long long* host_out);
 void launch_permute_int(hipStream_t s, int n, const int* order, const int* src, int* dst);
 void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start,
                     int* order);
@@ -45,6 +44,15 @@ void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool 
                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
                      int ntypes, double* fx, double* fy, double* fz, double* partial);
 int lj_tiled_blocks(int n, int T);
+// Mixed tiled LJ as the last force of a step, finishing the step per particle
+// (thermostat, integrate2, integrate1 into sa.pos_out / sa.posq_out), then the
+// step's max |dx|^2 (sa.blockmax needs lj_tiled_blocks(n, T) entries) into
+// ctl and host_out.
+void launch_lj_tiled_step(hipStream_t s, bool onetype, bool accum, int n, int T, int lds_bytes, const uint4* posq,
+                          const double4* pos, const uint4* nbr, const int* cnt, int capg, const Geom& g,
+                          const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
+                          int ntypes, double* fx, double* fy, double* fz, const StepArgs& sa, Ctl* ctl,
+                          unsigned long long* host_out);
 void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, const double4* pos,
                  const uint4* posq, const int* id, const int* slot_of_id, int nid,
                  const int* bstart, const int* bpartner, const Geom& g, const FENEParam& fp,

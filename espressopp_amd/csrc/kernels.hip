@@ -1027,6 +1027,76 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
   if ((threadIdx.x & 63) == 0) atomicMax(&ctl->maxcount, m);
 }
 
+// ------------------------------------------------------------------ step helpers
+
+// Philox4x32-10 (Salmon et al., SC'11); same stream as the oracle.
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Thermostat on particle i's force (LangevinThermostat::frictionThermo,
+// src/integrator/LangevinThermostat.cpp:141-177); false when it does not apply.
+__device__ __forceinline__ bool step_thermostat(int i, const double4& v, double& f0, double& f1, double& f2,
+                                                const StepArgs& a) {
+#pragma clang fp contract(off)
+  if (!a.lang) return false;
+  const int pid = a.id[i];
+  if (a.excl && pid < a.nid && a.excl[pid]) return false;
+  uint32_t o[4];
+  philox((uint32_t)pid, (uint32_t)a.call, (uint32_t)(a.call >> 32), 0x4C414E47u, (uint32_t)a.seed,
+         (uint32_t)(a.seed >> 32), o);
+  const double u0 = ((double)o[0] + 0.5) * 2.3283064365386963e-10;
+  const double u1 = ((double)o[1] + 0.5) * 2.3283064365386963e-10;
+  const double u2 = ((double)o[2] + 0.5) * 2.3283064365386963e-10;
+  const double m = v.w, massf = sqrt(m);
+  f0 += (a.pref1 * v.x) * m + (a.pref2 * (u0 - 0.5)) * massf;
+  f1 += (a.pref1 * v.y) * m + (a.pref2 * (u1 - 0.5)) * massf;
+  f2 += (a.pref1 * v.z) * m + (a.pref2 * (u2 - 0.5)) * massf;
+  return true;
+}
+
+// integrate2 (when do_int2) and integrate1 of particle i (velocity v, position
+// p) on force f (src/integrator/VelocityVerlet.cpp:236-305); returns |dx|^2.
+__device__ __forceinline__ double step_kick_drift(int i, double4 v, double4 p, double f0, double f1, double f2,
+                                                  const StepArgs& a, const Geom& g) {
+#pragma clang fp contract(off)
+  const double dt = a.dt;
+  const double dtfm = 0.5 * dt / v.w;
+  if (a.do_int2) {
+    v.x += dtfm * f0; v.y += dtfm * f1; v.z += dtfm * f2;
+  }
+  v.x += dtfm * f0; v.y += dtfm * f1; v.z += dtfm * f2;
+  const double d0 = v.x * dt, d1 = v.y * dt, d2 = v.z * dt;
+  p.x += d0; p.y += d1; p.z += d2;
+  a.vel[i] = v;
+  a.pos_out[i] = p;
+  if (a.posq_out)
+    a.posq_out[i] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]), (uint32_t)(int)p.w);
+  return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+// The block's maximum of sq into a.blockmax[blockIdx.x] (every thread calls it).
+__device__ __forceinline__ void step_block_max(double sq, const StepArgs& a) {
+  __shared__ double red[4];
+  sq = wave_max(sq);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+    a.blockmax[blockIdx.x] = m;
+  }
+}
+
 // Mixed-precision LJ over the tiled list.  Neighbour positions come from the
 // 32-bit fixed-point mirror staged in LDS; their differences are exact
 // integers.  r^2 is formed exactly in FP64 from them, the reciprocal is seeded
@@ -1037,13 +1107,22 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
 // Pairs within a tiny window of the cutoff (fixed-point quantisation could flip
 // the test) are left out of the main loop and settled from the FP64 positions
 // afterwards, so the interacting set equals the FP64 path's.
-template <bool ONETYPE, bool ENERGY, bool ACCUM, bool CUBIC>
+// STEP: the kernel is the last force of the step and finishes it per particle:
+// thermostat, integrate2 of this step and integrate1 of the next on the
+// completed force (step_thermostat / step_kick_drift, the arithmetic of
+// k_step_fused), writing the new positions to separate buffers (sa.pos_out,
+// sa.posq_out) because other workgroups still stage the current ones.  The
+// force itself is not stored: the next step recomputes it, and the last step
+// of a run takes the unfused path.  The particle's velocity and position are
+// loaded before the pair loop so that the loads overlap it.
+template <bool ONETYPE, bool ENERGY, bool ACCUM, bool CUBIC, bool STEP>
 __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     int n, int ntiles, const uint4* __restrict__ posq, const double4* __restrict__ pos,
     const uint4* __restrict__ nbr, const int* __restrict__ cnt, int capg, Geom g,
     const TileRun* __restrict__ runs, const int* __restrict__ nruns, const int* __restrict__ lane_slot,
     const LJParam* __restrict__ prm, int ntypes, double* __restrict__ fx, double* __restrict__ fy,
-    double* __restrict__ fz, double* __restrict__ partial) {
+    double* __restrict__ fz, double* __restrict__ partial, StepArgs sa) {
+  static_assert(!(STEP && ENERGY), "the step variant computes forces only");
   // one type: LDS holds x, y, z (12 bytes); several types: x, y, z, type
   using Slot = typename std::conditional<ONETYPE, uint3, uint4>::type;
   extern __shared__ uint4 lds_raw[];
@@ -1052,12 +1131,15 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = stage_tile_q<Slot>(runs, nruns, posq, lds, sr, tile);
   const int L = tile * blockDim.x + threadIdx.x;   // list lane
+  double stepsq = 0.0;                             // |dx|^2 of the drift (STEP)
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
   if (L < n) {
     const int i = lane_slot[L];                      // particle slot
     const uint4 qi = posq[i];
+    double4 step_v, step_p;
+    if (STEP) { step_v = sa.vel[i]; step_p = sa.pos[i]; }
     const uint4* np = nbr + tiled_group(L, capg, 0);
     const int c = nr > 0 ? min(cnt[L], 8 * capg) : 0;
     const int rowi = (int)qi.w * ntypes;
@@ -1152,11 +1234,17 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
         for (int q = 1; q < kNPart; ++q) acc[q] *= g.qs2;
       }
     }
-    if (!ENERGY) {
+    if (STEP) {
+      double f0 = Ax, f1 = Ay, f2 = Az;
+      if (ACCUM) { f0 = fx[i] + Ax; f1 = fy[i] + Ay; f2 = fz[i] + Az; }
+      step_thermostat(i, step_v, f0, f1, f2, sa);
+      stepsq = step_kick_drift(i, step_v, step_p, f0, f1, f2, sa, g);
+    } else if (!ENERGY) {
       if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
       else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
     }
   }
+  if (STEP) step_block_max(stepsq, sa);
   if (ENERGY) {
 #pragma unroll
     for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
@@ -1622,20 +1710,6 @@ __global__ void k_zero3(int n, double* __restrict__ a, double* __restrict__ b, d
 
 // ------------------------------------------------------------------ integrator
 
-// Philox4x32-10 (Salmon et al., SC'11); same stream as the oracle.
-__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                       uint32_t k0, uint32_t k1, uint32_t out[4]) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
-    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
-  }
-  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
-}
-
 // Langevin friction + uniform noise (LangevinThermostat::frictionThermo,
 // src/integrator/LangevinThermostat.cpp:141-177) followed, when do_int2 is set,
 // by VelocityVerlet::integrate2 (src/integrator/VelocityVerlet.cpp:289-305).
@@ -1716,59 +1790,19 @@ __global__ __launch_bounds__(256) void k_integrate1(
 //
 // Thermostat and integrate2 of step s-1 followed by integrate1 of step s on the
 // same forces (VelocityVerlet::run, src/integrator/VelocityVerlet.cpp:121-165:
-// aftCalcF -> integrate2 -> befIntP -> integrate1), one pass over the particles.
-// Writes the block's maximum |dx|^2; k_finish_max reduces them.
-__global__ __launch_bounds__(256) void k_step_fused(
-    int n, const int* __restrict__ id, double4* __restrict__ pos, double4* __restrict__ vel,
-    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz, int lang, double pref1,
-    double pref2, unsigned long long seed, unsigned long long call, const unsigned char* __restrict__ excl,
-    int nid, int do_int2, double dt, uint4* __restrict__ posq, Geom g, double* __restrict__ blockmax) {
-#pragma clang fp contract(off)
-  __shared__ double red[4];
+// aftCalcF -> integrate2 -> befIntP -> integrate1), one pass over the particles,
+// in place.  Writes the block's maximum |dx|^2; k_finish_max reduces them.
+__global__ __launch_bounds__(256) void k_step_fused(int n, double* __restrict__ fx, double* __restrict__ fy,
+                                                    double* __restrict__ fz, StepArgs a, Geom g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0;
   if (i < n) {
-    double4 v = vel[i];
+    const double4 v = a.vel[i];
     double f0 = fx[i], f1 = fy[i], f2 = fz[i];
-    if (lang) {
-      const int pid = id[i];
-      const bool skip = excl && pid < nid && excl[pid];
-      if (!skip) {
-        uint32_t o[4];
-        philox((uint32_t)pid, (uint32_t)call, (uint32_t)(call >> 32), 0x4C414E47u, (uint32_t)seed,
-               (uint32_t)(seed >> 32), o);
-        const double u0 = ((double)o[0] + 0.5) * 2.3283064365386963e-10;
-        const double u1 = ((double)o[1] + 0.5) * 2.3283064365386963e-10;
-        const double u2 = ((double)o[2] + 0.5) * 2.3283064365386963e-10;
-        const double m = v.w, massf = sqrt(m);
-        f0 += (pref1 * v.x) * m + (pref2 * (u0 - 0.5)) * massf;
-        f1 += (pref1 * v.y) * m + (pref2 * (u1 - 0.5)) * massf;
-        f2 += (pref1 * v.z) * m + (pref2 * (u2 - 0.5)) * massf;
-        fx[i] = f0; fy[i] = f1; fz[i] = f2;
-      }
-    }
-    const double dtfm = 0.5 * dt / v.w;
-    if (do_int2) {
-      v.x += dtfm * f0; v.y += dtfm * f1; v.z += dtfm * f2;
-    }
-    v.x += dtfm * f0; v.y += dtfm * f1; v.z += dtfm * f2;
-    double4 p = pos[i];
-    const double d0 = v.x * dt, d1 = v.y * dt, d2 = v.z * dt;
-    p.x += d0; p.y += d1; p.z += d2;
-    sq = d0 * d0 + d1 * d1 + d2 * d2;
-    vel[i] = v;
-    pos[i] = p;
-    if (posq)
-      posq[i] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]), (uint32_t)(int)p.w);
+    if (step_thermostat(i, v, f0, f1, f2, a)) { fx[i] = f0; fy[i] = f1; fz[i] = f2; }
+    sq = step_kick_drift(i, v, a.pos[i], f0, f1, f2, a, g);
   }
-  sq = wave_max(sq);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
-    blockmax[blockIdx.x] = m;
-  }
+  step_block_max(sq, a);
 }
 
 // host_out (pinned host memory, may be null) receives the same bits, so the host
@@ -1855,15 +1889,12 @@ void launch_scan_multi(hipStream_t s, int ncell, const int* count, int* start, i
 
 int scan_blocks(int ncell) { return (ncell + kScanChunk - 1) / kScanChunk; }
 
-void launch_step_fused(hipStream_t s, int n, const int* id, double4* pos, double4* vel, double* fx, double* fy,
-                       double* fz, int lang, double pref1, double pref2, unsigned long long seed,
-                       unsigned long long call, const unsigned char* excl, int nid, int do_int2, double dt,
-                       uint4* posq, const Geom& g, double* blockmax, Ctl* ctl, unsigned long long* host_out) {
+void launch_step_fused(hipStream_t s, int n, double* fx, double* fy, double* fz, const StepArgs& a, const Geom& g,
+                       Ctl* ctl, unsigned long long* host_out) {
   if (n <= 0) return;
   const int nb = nblocks(n, 256);
-  hipLaunchKernelGGL(k_step_fused, dim3(nb), dim3(256), 0, s, n, id, pos, vel, fx, fy, fz, lang, pref1, pref2, seed,
-                     call, excl, nid, do_int2, dt, posq, g, blockmax);
-  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, blockmax, ctl, host_out);
+  hipLaunchKernelGGL(k_step_fused, dim3(nb), dim3(256), 0, s, n, fx, fy, fz, a, g);
+  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, a.blockmax, ctl, host_out);
 }
 
 void launch_permute_int(hipStream_t s, int n, const int* order, const int* src, int* dst) {
@@ -1969,8 +2000,9 @@ void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool 
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
   const bool cubic = cubic_box(g);
-#define LJM(OT, EN, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, true>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial); \
-                             else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, false>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial); } while (0)
+  const StepArgs sa{};
+#define LJM(OT, EN, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, true, false>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial, sa); \
+                             else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, false, false>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial, sa); } while (0)
 #define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64<OT, EN, AC>), grid, block, lds_bytes, s, n, ntiles, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial)
   if (mixed) {
     if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }
@@ -1986,6 +2018,23 @@ void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool 
 }
 
 int lj_tiled_blocks(int n, int T) { return nblocks(n, T); }
+
+void launch_lj_tiled_step(hipStream_t s, bool onetype, bool accum, int n, int T, int lds_bytes, const uint4* posq,
+                          const double4* pos, const uint4* nbr, const int* cnt, int capg, const Geom& g,
+                          const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
+                          int ntypes, double* fx, double* fy, double* fz, const StepArgs& sa, Ctl* ctl,
+                          unsigned long long* host_out) {
+  if (n <= 0) return;
+  const int ntiles = nblocks(n, T);
+  dim3 grid(ntiles), block(T);
+  const bool cubic = cubic_box(g);
+#define LJS(OT, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, false, AC, true, true>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, nullptr, sa); \
+                         else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, false, AC, false, true>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, nullptr, sa); } while (0)
+  if (onetype) { if (accum) LJS(true, true); else LJS(true, false); }
+  else { if (accum) LJS(false, true); else LJS(false, false); }
+#undef LJS
+  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, ntiles, sa.blockmax, ctl, host_out);
+}
 
 void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, const double4* pos,
                  const uint4* posq, const int* id, const int* slot_of_id, int nid, const int* bstart,

@@ -301,6 +301,7 @@ struct espp_sim {
   DBuf<int> id, id2;
   DBuf<int4> img, img2;
   DBuf<uint4> posq;
+  DBuf<uint4> posq2;              // the next step's mirror while a fused force kernel reads posq
   DBuf<int> cell_of, rank, order, cell_of_slot, cell_count, cstart;
   DBuf<int> scan_sum, scan_off;   // per-chunk totals and offsets of the cell scan
   DBuf<double> blockmax;          // per-block max |dx|^2 of the fused step
@@ -443,6 +444,7 @@ struct espp_sim {
     cell_of.resize(m); rank.resize(m); order.resize(m); cell_of_slot.grow_keep(m, keep, stream);
     // the tiled list build reads up to 31 entries past a candidate range
     posq.grow_keep(m + kPosqPad, keep, stream);
+    posq2.resize(m + kPosqPad);
   }
 
   int64_t ncell() const { return (int64_t)g.n[0] * g.n[1] * g.n[2]; }
@@ -796,6 +798,39 @@ struct espp_sim {
     if (first) launch_zero3(stream, (int)n, fx.p, fy.p, fz.p);
   }
 
+  // Forces of a step whose last interaction, a tiled mixed LJ, also finishes the
+  // step: thermostat, integrate2, and integrate1 of the next step into pos2 /
+  // posq2, which then become pos / posq; the step's max |dx|^2 arrives in hmax.
+  // Returns false, having launched nothing, when the last interaction is not
+  // such a kernel.
+  bool calc_forces_step(StepArgs sa) {
+    if (!mixed() || active.empty() || n == 0) return false;
+    InteractionState& last = *ias[(size_t)active.back()];
+    if (last.kind != 0) return false;
+    ensure_lists();
+    VerletListState& v = *vls[(size_t)last.list];
+    if (!v.tiled) return false;
+    for (size_t q = 0; q + 1 < active.size(); ++q) run_interaction(*ias[(size_t)active[q]], false, q > 0, nullptr);
+    upload_lj(last);
+    const bool onetype = last.dtab_nt == 1;
+    blockmax.resize((size_t)std::max(lj_tiled_blocks((int)n, T), 1));
+    sa.id = id.p;
+    sa.vel = vel.p;
+    sa.pos = pos.p;
+    sa.pos_out = pos2.p;
+    sa.posq_out = posq2.p;
+    sa.blockmax = blockmax.p;
+    __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
+    hipEvent_t pe = prof_begin();
+    launch_lj_tiled_step(stream, onetype, active.size() > 1, (int)n, T, max_region * (onetype ? 12 : 16), posq.p,
+                         pos.p, (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, v.lane_slot.p,
+                         last.dtab.p, last.dtab_nt, fx.p, fy.p, fz.p, sa, ctl.p, dmax);
+    prof_end(0, pe);
+    pos.swap(pos2);
+    posq.swap(posq2);
+    return true;
+  }
+
   void upload_langevin_excl() {
     if (!excl_dirty && excl_nid == nid) return;
     sync();
@@ -848,23 +883,51 @@ struct espp_sim {
       launch_thermo_integrate2(stream, (int)n, id.p, vel.p, fx.p, fy.p, fz.p, 1, pref1, pref2 * std::sqrt(3.0), seed,
                                therm_calls++, excl_flag.p, excl_nid, 0, 0.5 * dt);
     }
-    blockmax.resize((size_t)std::max(kinetic_blocks((int)n), 1));
-    for (int i = 0; i < nsteps; ++i) {
-      auto ta = clk::now();
-      // thermostat + integrate2 of the previous step fused with integrate1 of this one
-      const bool first = (i == 0);
-      hipEvent_t pe1 = prof_begin();
-      __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
-      launch_step_fused(stream, (int)n, id.p, pos.p, vel.p, fx.p, fy.p, fz.p, (!first && lang) ? 1 : 0, pref1, pref2,
-                        seed, (!first && lang) ? therm_calls++ : 0ull, excl_flag.p, excl_nid, first ? 0 : 1, dt,
-                        mixed() ? posq.p : nullptr, g, blockmax.p, ctl.p, dmax);
-      prof_end(2, pe1);
+    auto step_args = [&](bool thermo, int do_int2) {
+      StepArgs sa{};
+      sa.excl = excl_flag.p;
+      sa.nid = excl_nid;
+      sa.pref1 = pref1;
+      sa.pref2 = pref2;
+      sa.dt = dt;
+      sa.seed = seed;
+      sa.lang = thermo ? 1 : 0;
+      sa.call = thermo ? therm_calls : 0ull;
+      sa.do_int2 = do_int2;
+      return sa;
+    };
+    auto add_max_dist = [&]() {
       double maxSq = 0.0;
       if (n > 0) {
         const unsigned long long bits = wait_host_value();
         std::memcpy(&maxSq, &bits, sizeof(double));
       }
       maxDist += std::sqrt(maxSq);
+    };
+    // Each step: integrate1, the displacement check (resort), forces, and the
+    // thermostat with integrate2.  Integrate1 runs with the thermostat and
+    // integrate2 of the previous step (k_step_fused), or inside the previous
+    // step's force kernel when that kernel can finish the step (fused_prev).
+    bool fused_prev = false;
+    for (int i = 0; i < nsteps; ++i) {
+      auto ta = clk::now();
+      if (!fused_prev) {
+        const bool first = (i == 0);
+        StepArgs sa = step_args(!first && lang, first ? 0 : 1);
+        if (!first && lang) therm_calls++;
+        blockmax.resize((size_t)std::max(kinetic_blocks((int)n), 1));
+        sa.id = id.p;
+        sa.vel = vel.p;
+        sa.pos = pos.p;
+        sa.pos_out = pos.p;
+        sa.posq_out = mixed() ? posq.p : nullptr;
+        sa.blockmax = blockmax.p;
+        hipEvent_t pe1 = prof_begin();
+        __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
+        launch_step_fused(stream, (int)n, fx.p, fy.p, fz.p, sa, g, ctl.p, dmax);
+        prof_end(2, pe1);
+        add_max_dist();
+      }
       auto tb = clk::now();
       t_int1 += std::chrono::duration<double>(tb - ta).count();
       if (maxDist > skinHalf) resortFlag = true;
@@ -876,7 +939,13 @@ struct espp_sim {
       }
       auto tc = clk::now();
       t_resort += std::chrono::duration<double>(tc - tb).count();
-      calc_forces();
+      fused_prev = i + 1 < nsteps && calc_forces_step(step_args(lang, 1));
+      if (fused_prev) {
+        if (lang) therm_calls++;
+        add_max_dist();
+      } else {
+        calc_forces();
+      }
       t_force += std::chrono::duration<double>(clk::now() - tc).count();
       step++;
     }
