@@ -77,7 +77,7 @@ PROF_SIGNATURES = {
     "espp_vl_stats": [_vp, _c_int, _P(_c_i64), _P(_c_int), _P(_c_int)],
     "espp_tile_stats": [_vp, _P(_c_int), _P(_c_int)],
 }
-KERNEL = {"pair": 0, "nbr_build": 1, "integrate1": 2, "integrate2": 3, "binning": 4, "bond": 5}
+KERNEL = {"pair": 0, "nbr_build": 1, "integrate1": 2, "integrate2": 3, "binning": 4, "bond": 5, "pair_step": 6}
 _RESTYPE = {"espp_last_error": ctypes.c_char_p, "espp_version": ctypes.c_char_p}
 
 _lib = None

@@ -903,8 +903,10 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     const int myid = HAS_EXCL ? id[i] : 0;
     uint4* out = nbr + tiled_group(L, capg, 0);
     const int cap = 8 * capg;
-    // entries not yet stored: w0..w3 hold the current group, half-filled word in `pend`
-    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, pend = 0;
+    // entries not yet stored: (w3:w2:w1:w0) is a 128-bit shift register of 16-bit
+    // entries, each new entry entering at the top; after 8 entries it holds the
+    // group in the list layout (first entry in the low half of w0)
+    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
     const float ry = (float)(g.qsd[1] / g.qsd[0]), rz = (float)(g.qsd[2] / g.qsd[0]);
     // reach in x, fixed-point units: the prefilter's top plus two units of rounding
     const int reach_x = (int)fminf(sqrtf(cut_hi) * 1.0001f + 2.0f, 2.0e9f);
@@ -986,30 +988,31 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
               else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
               if (exact_sqr_le(ex, ey, ez, cutsq)) in_bits |= 1u << t;
             }
+            const uint32_t ebase = (uint32_t)(base + (j0 - jb));   // entry of candidate 0
             while (in_bits) {
               const int t = __ffs(in_bits) - 1;
               in_bits &= in_bits - 1u;
-              const int j = j0 + t;
-              if (HAS_EXCL && is_excluded(myid, id[j], ex_start, ex_ids)) continue;
-              const uint32_t e = (uint32_t)(base + (j - jb));
-              if (k & 1) {
-                const uint32_t w = pend | (e << 16);
-                const int wi = (k >> 1) & 3;
-                w0 = wi == 0 ? w : w0; w1 = wi == 1 ? w : w1; w2 = wi == 2 ? w : w2; w3 = wi == 3 ? w : w3;
-                if (wi == 3 && k < cap) out[(size_t)(k >> 3) * 64] = make_uint4(w0, w1, w2, w3);
-              } else {
-                pend = e;
-              }
+              if (HAS_EXCL && is_excluded(myid, id[j0 + t], ex_start, ex_ids)) continue;
+              w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+              w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+              w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+              w3 = __builtin_amdgcn_alignbit(ebase + (uint32_t)t, w3, 16);
               ++k;
+              if ((k & 7) == 0 && k <= cap) out[(size_t)((k >> 3) - 1) * 64] = make_uint4(w0, w1, w2, w3);
             }
           }
         }
       }
     }
-    // last partial group; entries past the count are never read
+    // last partial group: shift its entries down to their places (entries past the
+    // count are never read)
     if ((k & 7) != 0 && k < cap) {
-      const int wi = (k >> 1) & 3;
-      if (k & 1) { w0 = wi == 0 ? pend : w0; w1 = wi == 1 ? pend : w1; w2 = wi == 2 ? pend : w2; w3 = wi == 3 ? pend : w3; }
+      for (int r = k & 7; r < 8; ++r) {
+        w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+        w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+        w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+        w3 = w3 >> 16;
+      }
       out[(size_t)(k >> 3) * 64] = make_uint4(w0, w1, w2, w3);
     }
     cnt[L] = k;

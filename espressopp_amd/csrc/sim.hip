@@ -825,7 +825,7 @@ struct espp_sim {
     launch_lj_tiled_step(stream, onetype, active.size() > 1, (int)n, T, max_region * (onetype ? 12 : 16), posq.p,
                          pos.p, (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, v.lane_slot.p,
                          last.dtab.p, last.dtab_nt, fx.p, fy.p, fz.p, sa, ctl.p, dmax);
-    prof_end(0, pe);
+    prof_end(6, pe);
     pos.swap(pos2);
     posq.swap(posq2);
     return true;

@@ -19,11 +19,15 @@ extern "C" {
 /* kernel ids for espp_prof_stats */
 #define ESPP_K_PAIR 0        /* Lennard-Jones pair-force kernel */
 #define ESPP_K_NBR_BUILD 1   /* Verlet-list build */
-#define ESPP_K_INTEGRATE1 2  /* per step: thermostat + second half kick of the previous
-                                step + half kick + drift + max displacement (fused) */
+#define ESPP_K_INTEGRATE1 2  /* thermostat + second half kick of the previous step + half
+                                kick + drift + max displacement, where the previous step's
+                                force kernel did not do it */
 #define ESPP_K_INTEGRATE2 3  /* thermostat + half kick after the last step of a run */
 #define ESPP_K_BINNING 4     /* fold, counting sort and permutation (all kernels of one binning) */
 #define ESPP_K_BOND 5        /* FENE bond-force kernel */
+#define ESPP_K_PAIR_STEP 6   /* Lennard-Jones pair forces that also finish the step: thermostat,
+                                half kick, and the next step's half kick, drift and max
+                                displacement (every step of a run but the last) */
 
 /* hipDeviceSynchronize on the current device. */
 int espp_device_synchronize(void);
