@@ -38,7 +38,8 @@ void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Ge
 void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
-                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot, Ctl* ctl);
+                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot,
+                            float clip_margin, Ctl* ctl);
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,

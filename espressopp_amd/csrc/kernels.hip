@@ -875,7 +875,8 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g, double cutsq, float cut_lo,
     float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,  // cut_lo/hi in units of qs[0]^2
     const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint4* __restrict__ nbr,
-    int* __restrict__ cnt, int capg, int* __restrict__ pred, int* __restrict__ lane_slot, Ctl* ctl) {
+    int* __restrict__ cnt, int capg, int* __restrict__ pred, int* __restrict__ lane_slot, float clip_margin,
+    Ctl* ctl) {
   __shared__ TileRun sr[kMaxRuns];
   __shared__ int skey[256], sval[256];
   const int tile = xcd_tile(blockIdx.x, ntiles);
@@ -908,8 +909,12 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     // group in the list layout (first entry in the low half of w0)
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
     const float ry = (float)(g.qsd[1] / g.qsd[0]), rz = (float)(g.qsd[2] / g.qsd[0]);
-    // reach in x, fixed-point units: the prefilter's top plus two units of rounding
-    const int reach_x = (int)fminf(sqrtf(cut_hi) * 1.0001f + 2.0f, 2.0e9f);
+    // reach in x, fixed-point units: the prefilter's top plus two units of rounding,
+    // plus clip_margin when the x order inside the cells may be stale: a particle
+    // moved by at most d since its cell was sorted leaves the order wrong by at
+    // most 2d, and a binary search with that much more reach keeps every candidate
+    // within the prefilter's top
+    const int reach_x = (int)fminf(sqrtf(cut_hi) * 1.0001f + 2.0f + clip_margin, 2.0e9f);
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nz; sz = -1; } else if (qz >= nz) { qz -= nz; sz = 1; }
@@ -1984,11 +1989,12 @@ static bool cubic_box(const Geom& g) { return g.qsd[0] == g.qsd[1] && g.qsd[1] =
 void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
-                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot, Ctl* ctl) {
+                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot,
+                            float clip_margin, Ctl* ctl) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
-#define NB(EX, CU) hipLaunchKernelGGL((k_nbr_build_tiled<EX, CU>), grid, block, 0, s, n, ntiles, posq, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, pred, lane_slot, ctl)
+#define NB(EX, CU) hipLaunchKernelGGL((k_nbr_build_tiled<EX, CU>), grid, block, 0, s, n, ntiles, posq, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, pred, lane_slot, clip_margin, ctl)
   const bool cu = cubic_box(g);
   if (ex_start) { if (cu) NB(true, true); else NB(true, false); }
   else { if (cu) NB(false, true); else NB(false, false); }

@@ -333,6 +333,7 @@ struct espp_sim {
 
   int64_t step = 0;
   double maxDist = 0.0;
+  double disp_since_bin = 0.0;    // bound on any particle's displacement since the last binning
   bool resortFlag = true;
   int64_t nResorts = 0;
 
@@ -488,6 +489,7 @@ struct espp_sim {
       }
     }
     prof_end(4, pe);
+    disp_since_bin = 0.0;
     plan_valid = false;
     for (auto* v : vls) v->stale = true;
     if (do_fold) {
@@ -568,6 +570,8 @@ struct espp_sim {
     // the tiled build compares in fixed-point units of the x scale
     const double q2 = g.qsd[0] * g.qsd[0];
     const float cut_lo_q = (float)(cutsq * (1.0 - win) / q2), cut_hi_q = (float)(cutsq * (1.0 + win) / q2);
+    // particles moved since the binning sorted the cells by x: widen the x clip
+    const float clip_margin = (float)(2.0 * disp_since_bin * g.qinv[0] * (1.0 + 1e-6)) + (disp_since_bin > 0.0 ? 4.0f : 0.0f);
     for (int attempt = 0; attempt < 4; ++attempt) {
       launch_reset_ctl(stream, ctl.p);
       hipEvent_t pe = prof_begin();
@@ -584,7 +588,7 @@ struct espp_sim {
         launch_nbr_build_tiled(stream, (int)n, T, posq.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
                                cutsq, cut_lo_q, cut_hi_q, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
                                v.has_excl ? v.ex_ids.p : nullptr, (uint4*)v.nbr.p, v.cnt.p, v.cap / 8, v.pred.p,
-                               v.lane_slot.p, ctl.p);
+                               v.lane_slot.p, clip_margin, ctl.p);
       } else {
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
         launch_nbr_build(stream, (int)n, pos.p, id.p, cell_of_slot.p, cstart.p, g, cutsq,
@@ -903,6 +907,7 @@ struct espp_sim {
         std::memcpy(&maxSq, &bits, sizeof(double));
       }
       maxDist += std::sqrt(maxSq);
+      disp_since_bin += std::sqrt(maxSq);
     };
     // Each step: integrate1, the displacement check (resort), forces, and the
     // thermostat with integrate2.  Integrate1 runs with the thermostat and

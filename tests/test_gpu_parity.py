@@ -376,6 +376,37 @@ def test_pairs_and_forces_262k(gpu, precision):
     assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=1e-12 if precision == "fp64" else 1e-6)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_verlet_list_built_after_a_run(gpu, precision):
+    # A list built between runs sees cells whose x order dates from the last binning while
+    # the particles have moved since (no resort): its pair set must still be every pair
+    # within cutoff + skin at the current positions (periodic k-d tree over the folded
+    # positions as the check)
+    from scipy.spatial import cKDTree
+    n = 6400
+    pos, box = jittered_lattice(n, 0.8, seed=31)
+    rc, skin = 1.5, 1.0
+    cells = cells_for(box, rc, skin)
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells, rc, skin, precision,
+                                             vel=mb_velocities(n, 1.5, seed=32))
+    integ = espressopp.integrator.VelocityVerlet(system)
+    integ.dt = 0.005
+    integ.run(10)
+    assert integ.getNumResorts() == 0
+    for cut in (1.2, 1.5):
+        vl2 = espressopp.VerletList(system, cutoff=cut)
+        a = system.storage.getParticleArrays()
+        x = np.mod(a["pos"], np.asarray(box))
+        x[x >= np.asarray(box)] = 0.0
+        tree = cKDTree(x, boxsize=np.asarray(box))
+        want = tree.query_pairs(cut + skin, output_type="ndarray")
+        ids = a["id"].astype(np.int64)
+        want = np.sort(ids[want], axis=1)
+        got = np.sort(vl2.pairs_array().reshape(-1, 2).astype(np.int64), axis=1)
+        assert len(got) == len(want)
+        assert np.array_equal(np.sort(got[:, 0] * n + got[:, 1]), np.sort(want[:, 0] * n + want[:, 1]))
+
+
 def _write_table(path, r, e, f, junk=True):
     with open(path, "w") as fh:
         if junk:
