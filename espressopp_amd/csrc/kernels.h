@@ -20,7 +20,8 @@ void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, 
                     const double4* vel, const double* fx, const double* fy, const double* fz,
                     const int* id, const int4* img, double4* pos2, double4* vel2, double* fx2,
                     double* fy2, double* fz2, int* id2, int4* img2, int* cell_of_slot,
-                    int* slot_of_id, uint4* posq, const Geom& g);
+                    int* slot_of_id, uint4* posq, float4* posf, const Geom& g);
+void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf);
 void launch_fill_int(hipStream_t s, int n, int* a, int v);
 void launch_reset_ctl(hipStream_t s, Ctl* ctl);
 void launch_nbr_build(hipStream_t s, int n, const double4* pos, const int* id,
@@ -35,7 +36,7 @@ void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum,
 void launch_tile_plan(hipStream_t s, int n, int T, int ntiles, const int* cell_of_slot, const int* cstart,
                       const Geom& g, int budget, TileRun* runs, int* nruns, Ctl* ctl);
 void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Geom& g);
-void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, const double4* pos,
+void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
                             const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot,

@@ -353,7 +353,7 @@ __global__ void k_permute(int n, const int* __restrict__ order, const int* __res
                           double* __restrict__ fy2, double* __restrict__ fz2,
                           int* __restrict__ id2, int4* __restrict__ img2,
                           int* __restrict__ cell_of_slot, int* __restrict__ slot_of_id,
-                          uint4* __restrict__ posq, Geom g) {
+                          uint4* __restrict__ posq, float4* __restrict__ posf, Geom g) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int o = order[s];
@@ -373,6 +373,15 @@ __global__ void k_permute(int n, const int* __restrict__ order, const int* __res
   if (posq)
     posq[s] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]),
                          (uint32_t)(int)p.w);
+  posf[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+}
+
+// FP32 copy of the positions, the candidates of the tiled list build.
+__global__ void k_posf(int n, const double4* __restrict__ pos, float4* __restrict__ posf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 p = pos[i];
+  posf[i] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
 }
 
 __global__ void k_fill_int(int n, int* __restrict__ a, int v) {
@@ -861,19 +870,21 @@ constexpr uint32_t kCandAll = (1u << kCandChunk) - 1u;
 // Tiled Verlet-list build (cell grids with at least 3 cells per dimension).
 // For each of its 9 neighbour x-rows a particle scans the candidates of the
 // three cells (cx-1, cx, cx+1), one contiguous slot range or two when the row
-// wraps, read from the fixed-point mirror (lanes of one cell read the same
-// candidates, so the loads broadcast; the mirror is padded so reads past a
-// range stay inside it).  For kCandChunk candidates at a time two bit masks are formed
-// without branches: FP32 r^2 (fixed-point units) below the window (surely in)
-// and below its top (in or undecided).  Undecided pairs, within a small window
+// wraps, clipped in x by binary search (cells are sorted by x) and read from an
+// FP32 copy of the positions (lanes of one cell read the same candidates, so
+// the loads broadcast; the copy is padded so reads past a range stay inside
+// it).  Particle i's coordinates are shifted by each piece's periodic image
+// once, so a candidate costs three FP32 differences.  For kCandChunk candidates
+// at a time two bit masks are formed without branches: FP32 r^2 below the
+// window (surely in) and below its top (in or undecided).  Undecided pairs, within a small window
 // of (rc+skin)^2, are settled by the reference's FP64 test in the reference's
 // orientation (see k_nbr_build).  Accepted candidates are appended as their
 // offset in the tile's staged region, eight entries per 16-byte store.
-template <bool HAS_EXCL, bool CUBIC>
-__global__ __launch_bounds__(256) void k_nbr_build_tiled(
-    int n, int ntiles, const uint4* __restrict__ posq, const double4* __restrict__ pos, const int* __restrict__ id,
+template <bool HAS_EXCL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_nbr_build_tiled(
+    int n, int ntiles, const float4* __restrict__ posf, const double4* __restrict__ pos, const int* __restrict__ id,
     const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g, double cutsq, float cut_lo,
-    float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,  // cut_lo/hi in units of qs[0]^2
+    float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,
     const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint4* __restrict__ nbr,
     int* __restrict__ cnt, int capg, int* __restrict__ pred, int* __restrict__ lane_slot, float clip_margin,
     Ctl* ctl) {
@@ -897,7 +908,7 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
   const int L = s0 + skey[threadIdx.x];  // list lane (slots past n rank last, with lanes past n)
   int k = 0;
   if (i < n && nr > 0) {
-    const uint4 qi = posq[i];
+    const double4 pi = pos[i];
     const int c = cell_of_slot[i];
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
@@ -908,19 +919,21 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
     // entries, each new entry entering at the top; after 8 entries it holds the
     // group in the list layout (first entry in the low half of w0)
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
-    const float ry = (float)(g.qsd[1] / g.qsd[0]), rz = (float)(g.qsd[2] / g.qsd[0]);
-    // reach in x, fixed-point units: the prefilter's top plus two units of rounding,
-    // plus clip_margin when the x order inside the cells may be stale: a particle
-    // moved by at most d since its cell was sorted leaves the order wrong by at
-    // most 2d, and a binary search with that much more reach keeps every candidate
-    // within the prefilter's top
-    const int reach_x = (int)fminf(sqrtf(cut_hi) * 1.0001f + 2.0f + clip_margin, 2.0e9f);
+    // reach in x: the prefilter's top with a relative margin for FP32 rounding, plus
+    // clip_margin when the x order inside the cells may be stale: a particle moved
+    // by at most d since its cell was sorted leaves the order wrong by at most 2d,
+    // and a binary search with that much more reach keeps every candidate within
+    // the prefilter's top
+    const float reach = sqrtf(cut_hi) * 1.0001f + clip_margin;
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nz; sz = -1; } else if (qz >= nz) { qz -= nz; sz = 1; }
+      // particle i moved by minus the candidate cells' periodic shift, in FP32
+      const float zs = (float)(pi.z - (double)sz * g.L[2]);
       for (int oy = -1; oy <= 1; ++oy) {
         int qy = cy + oy, sy = 0;
         if (qy < 0) { qy += ny; sy = -1; } else if (qy >= ny) { qy -= ny; sy = 1; }
+        const float ys = (float)(pi.y - (double)sy * g.L[1]);
         const int row = (qy + ny * qz) * nx;
         // x pieces of the row segment (cx-1 .. cx+1), split where the row wraps
         int pa0, pb0, pa1 = 0, pb1 = -1;
@@ -930,43 +943,44 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
         for (int p = 0; p < 2; ++p) {
           const int xa = p ? pa1 : pa0, xb = p ? pb1 : pb0;
           if (xb < xa) break;
+          // the piece lies across the periodic boundary from i: cell nx-1 seen from
+          // cx = 0, cell 0 seen from cx = nx-1
+          const int sxp = (cx == 0 && p == 0) ? -1 : ((cx == nx - 1 && p == 1) ? 1 : 0);
+          const float xs = (float)(pi.x - (double)sxp * g.L[0]);
           const int ca = row + xa;
           const int jb = cstart[ca], je0 = cstart[row + xb + 1];
           const TileRun R = sr[find_run_by_cell(sr, nr, ca)];
           const int base = R.lds_off + (jb - R.slot_start);
           // the piece is sorted by x, so x_i - x_j falls along it: keep the stretch
-          // with |x_i - x_j| <= reach (fixed-point units of x)
+          // with |x_i - x_j| <= reach
           int ja = jb, je = je0;
           {
             int lo = jb, hi = je0;                       // first j with x_i - x_j <= reach
             while (lo < hi) {
               const int mid = (lo + hi) >> 1;
-              if ((int)(qi.x - posq[mid].x) > reach_x) lo = mid + 1; else hi = mid;
+              if (xs - posf[mid].x > reach) lo = mid + 1; else hi = mid;
             }
             ja = lo;
             hi = je0;                                    // first j with x_i - x_j < -reach
             while (lo < hi) {
               const int mid = (lo + hi) >> 1;
-              if ((int)(qi.x - posq[mid].x) >= -reach_x) lo = mid + 1; else hi = mid;
+              if (xs - posf[mid].x >= -reach) lo = mid + 1; else hi = mid;
             }
             je = lo;
           }
           for (int j0 = ja; j0 < je; j0 += kCandChunk) {
             const int jn = je - j0;
-            const uint4* cand = posq + j0;
+            const float4* cand = posf + j0;
             uint32_t in_bits = 0, maybe_bits = 0;
             // candidates kCandChunk-1 .. 0, so that candidate t ends at bit t of the masks
 #pragma unroll
             for (int u = kCandChunk - 4; u >= 0; u -= 4) {
-              uint4 q[4];
+              float4 q[4];
 #pragma unroll
               for (int v = 0; v < 4; ++v) q[v] = cand[u + v];
 #pragma unroll
               for (int v = 3; v >= 0; --v) {
-                const float dx = (float)(int)(qi.x - q[v].x);
-                float dy = (float)(int)(qi.y - q[v].y);
-                float dz = (float)(int)(qi.z - q[v].z);
-                if (!CUBIC) { dy *= ry; dz *= rz; }
+                const float dx = xs - q[v].x, dy = ys - q[v].y, dz = zs - q[v].z;
                 const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                 in_bits = (in_bits << 1) + (r2 < cut_lo ? 1u : 0u);
                 maybe_bits = (maybe_bits << 1) + (r2 <= cut_hi ? 1u : 0u);
@@ -986,7 +1000,7 @@ __global__ __launch_bounds__(256) void k_nbr_build_tiled(
               int ox = cjx - cx, sx = 0;
               if (ox > 1) { ox = -1; sx = -1; } else if (ox < -1) { ox = 1; sx = 1; }
               const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
-              const double4 pi = pos[i], pj = pos[j];
+              const double4 pj = pos[j];
               const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
               double ex, ey, ez;
               if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
@@ -1923,10 +1937,14 @@ void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, 
                     const double4* vel, const double* fx, const double* fy, const double* fz,
                     const int* id, const int4* img, double4* pos2, double4* vel2, double* fx2,
                     double* fy2, double* fz2, int* id2, int4* img2, int* cell_of_slot,
-                    int* slot_of_id, uint4* posq, const Geom& g) {
+                    int* slot_of_id, uint4* posq, float4* posf, const Geom& g) {
   if (n > 0)
     hipLaunchKernelGGL(k_permute, dim3(nblocks(n, 256)), dim3(256), 0, s, n, order, cell_of, pos, vel, fx, fy, fz, id, img,
-                       pos2, vel2, fx2, fy2, fz2, id2, img2, cell_of_slot, slot_of_id, posq, g);
+                       pos2, vel2, fx2, fy2, fz2, id2, img2, cell_of_slot, slot_of_id, posq, posf, g);
+}
+
+void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf) {
+  if (n > 0) hipLaunchKernelGGL(k_posf, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, posf);
 }
 
 void launch_fill_int(hipStream_t s, int n, int* a, int v) {
@@ -1986,7 +2004,7 @@ void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Ge
 
 static bool cubic_box(const Geom& g) { return g.qsd[0] == g.qsd[1] && g.qsd[1] == g.qsd[2]; }
 
-void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, const double4* pos,
+void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
                             const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot,
@@ -1994,10 +2012,9 @@ void launch_nbr_build_tiled(hipStream_t s, int n, int T, const uint4* posq, cons
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
-#define NB(EX, CU) hipLaunchKernelGGL((k_nbr_build_tiled<EX, CU>), grid, block, 0, s, n, ntiles, posq, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, pred, lane_slot, clip_margin, ctl)
-  const bool cu = cubic_box(g);
-  if (ex_start) { if (cu) NB(true, true); else NB(true, false); }
-  else { if (cu) NB(false, true); else NB(false, false); }
+#define NB(EX) hipLaunchKernelGGL((k_nbr_build_tiled<EX>), grid, block, 0, s, n, ntiles, posf, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, pred, lane_slot, clip_margin, ctl)
+  if (ex_start) NB(true);
+  else NB(false);
 #undef NB
 }
 

@@ -302,6 +302,7 @@ struct espp_sim {
   DBuf<int4> img, img2;
   DBuf<uint4> posq;
   DBuf<uint4> posq2;              // the next step's mirror while a fused force kernel reads posq
+  DBuf<float4> posf;              // FP32 positions, candidates of the tiled list build
   DBuf<int> cell_of, rank, order, cell_of_slot, cell_count, cstart;
   DBuf<int> scan_sum, scan_off;   // per-chunk totals and offsets of the cell scan
   DBuf<double> blockmax;          // per-block max |dx|^2 of the fused step
@@ -334,6 +335,7 @@ struct espp_sim {
   int64_t step = 0;
   double maxDist = 0.0;
   double disp_since_bin = 0.0;    // bound on any particle's displacement since the last binning
+  double coord_bound = 0.0;       // bound on |coordinate| of the current positions
   bool resortFlag = true;
   int64_t nResorts = 0;
 
@@ -446,6 +448,7 @@ struct espp_sim {
     // the tiled list build reads up to 31 entries past a candidate range
     posq.grow_keep(m + kPosqPad, keep, stream);
     posq2.resize(m + kPosqPad);
+    posf.resize(m + kPosqPad);
   }
 
   int64_t ncell() const { return (int64_t)g.n[0] * g.n[1] * g.n[2]; }
@@ -476,7 +479,7 @@ struct espp_sim {
       launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, pos.p);
       launch_permute(stream, ni, order.p, cell_of.p, pos.p, vel.p, fx.p, fy.p, fz.p, id.p, img.p, pos2.p,
                      vel2.p, carry_forces ? fx2.p : nullptr, fy2.p, fz2.p, id2.p, img2.p, cell_of_slot.p,
-                     slot_of_id.p, posq.p, g);
+                     slot_of_id.p, posq.p, posf.p, g);
       pos.swap(pos2); vel.swap(vel2);
       if (carry_forces) { fx.swap(fx2); fy.swap(fy2); fz.swap(fz2); }
       id.swap(id2); img.swap(img2);
@@ -493,6 +496,7 @@ struct espp_sim {
     plan_valid = false;
     for (auto* v : vls) v->stale = true;
     if (do_fold) {
+      coord_bound = std::max(g.L[0], std::max(g.L[1], g.L[2]));
       read_ctl();
       if (hctl->nanpos) fail(ESPP_ERR_NAN, "particle has moved to outer space (one or more coordinates are nan)");
     }
@@ -563,15 +567,20 @@ struct espp_sim {
     }
     v.cnt.resize((size_t)std::max<int64_t>(n, 1));
     v.tiled = T > 0;
-    if (v.tiled && !mixed()) launch_posq(stream, (int)n, pos.p, posq.p, g);
-    // FP32 prefilter window: FP32 rounding plus fixed-point quantisation of both positions
-    double qmax = std::max(g.L[0], std::max(g.L[1], g.L[2])) / 4294967296.0;
-    double win = 1e-5 + 8.0 * qmax / std::max(cutVerlet, 1e-30);
-    // the tiled build compares in fixed-point units of the x scale
-    const double q2 = g.qsd[0] * g.qsd[0];
-    const float cut_lo_q = (float)(cutsq * (1.0 - win) / q2), cut_hi_q = (float)(cutsq * (1.0 + win) / q2);
+    // the FP32 candidate positions are the binning's until a particle moves
+    if (v.tiled && disp_since_bin > 0.0) launch_posf(stream, (int)n, pos.p, posf.p);
+    // FP32 prefilter window of the tiled build.  Its coordinates (a position, or
+    // one shifted by a box length) are below M in magnitude; each is rounded to
+    // FP32 once (half an ulp of M), so a difference is off by at most one ulp and
+    // r^2 by at most 2 sqrt(3) (rc+skin) ulp, plus a few 1e-7 of FP32 rounding of
+    // r^2 itself.  The window takes twice that.
+    const double lmax = std::max(g.L[0], std::max(g.L[1], g.L[2]));
+    const double M = coord_bound + lmax + cutVerlet;
+    const double ulp = std::ldexp(1.0, std::ilogb(M) - 23);
+    const double win = 1e-6 + 4.0 * std::sqrt(3.0) * ulp / std::max(cutVerlet, 1e-30);
+    const float cut_lo_q = (float)(cutsq * (1.0 - win)), cut_hi_q = (float)(cutsq * (1.0 + win));
     // particles moved since the binning sorted the cells by x: widen the x clip
-    const float clip_margin = (float)(2.0 * disp_since_bin * g.qinv[0] * (1.0 + 1e-6)) + (disp_since_bin > 0.0 ? 4.0f : 0.0f);
+    const float clip_margin = disp_since_bin > 0.0 ? (float)(2.0 * disp_since_bin * (1.0 + 1e-6) + 4.0 * ulp) : 0.0f;
     for (int attempt = 0; attempt < 4; ++attempt) {
       launch_reset_ctl(stream, ctl.p);
       hipEvent_t pe = prof_begin();
@@ -585,7 +594,7 @@ struct espp_sim {
           launch_fill_int(stream, (int)n, v.pred.p, 0);
           v.pred_n = n;
         }
-        launch_nbr_build_tiled(stream, (int)n, T, posq.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
+        launch_nbr_build_tiled(stream, (int)n, T, posf.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
                                cutsq, cut_lo_q, cut_hi_q, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
                                v.has_excl ? v.ex_ids.p : nullptr, (uint4*)v.nbr.p, v.cnt.p, v.cap / 8, v.pred.p,
                                v.lane_slot.p, clip_margin, ctl.p);
@@ -908,6 +917,7 @@ struct espp_sim {
       }
       maxDist += std::sqrt(maxSq);
       disp_since_bin += std::sqrt(maxSq);
+      coord_bound += std::sqrt(maxSq);
     };
     // Each step: integrate1, the displacement check (resort), forces, and the
     // thermostat with integrate2.  Integrate1 runs with the thermostat and
@@ -1183,6 +1193,7 @@ int espp_add_particles(espp_sim* s, int64_t n, const int32_t* id, const int32_t*
     upload_bytes(s->fy.p + off, zero.data(), (size_t)n * sizeof(double), s->stream, "upload");
     upload_bytes(s->fz.p + off, zero.data(), (size_t)n * sizeof(double), s->stream, "upload");
     s->n += n;
+    s->coord_bound = std::max(s->coord_bound, std::max(s->g.L[0], std::max(s->g.L[1], s->g.L[2])));
     s->host_ids.insert(s->host_ids.end(), newids.begin(), newids.end());
     s->nid = std::max(s->nid, maxid + 1);
     s->sorted = false;
@@ -1272,7 +1283,10 @@ int espp_set_particles(espp_sim* s, int32_t field, int64_t m, const int32_t* ids
       std::vector<double4> p = download(s->pos, n, s->stream);
       for (int64_t k = 0; k < m; ++k) {
         double4& q = p[slot(ids[k])];
-        if (field == ESPP_FIELD_POS) { q.x = values[3 * k]; q.y = values[3 * k + 1]; q.z = values[3 * k + 2]; }
+        if (field == ESPP_FIELD_POS) {
+          q.x = values[3 * k]; q.y = values[3 * k + 1]; q.z = values[3 * k + 2];
+          s->coord_bound = std::max(s->coord_bound, std::max(std::fabs(q.x), std::max(std::fabs(q.y), std::fabs(q.z))));
+        }
         else {
           need(values[k] >= 0.0, "particle types must be non-negative");
           q.w = std::floor(values[k]);
