@@ -1172,14 +1172,23 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     // FP64 positions; r^2 above hi is outside
     double lo = CUBIC ? p0.ulo : p0.dlo, hi = CUBIC ? p0.uhi : p0.dhi;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
+    // kGroups 16-byte groups (8 entries each) per iteration, the next ones loaded
+    // during it
+    constexpr int kGroups = 2;
     const int ng = (c + 7) >> 3;
-    uint4 wn = ng > 0 ? np[0] : make_uint4(0u, 0u, 0u, 0u);
-    for (int kg = 0; kg < ng; ++kg) {
-      const uint32_t w[4] = {wn.x, wn.y, wn.z, wn.w};
-      wn = (kg + 1 < ng) ? np[(size_t)(kg + 1) * 64] : make_uint4(0u, 0u, 0u, 0u);
+    uint4 wn[kGroups];
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q) wn[q] = q < ng ? np[(size_t)q * 64] : make_uint4(0u, 0u, 0u, 0u);
+    for (int kg = 0; kg < ng; kg += kGroups) {
+      uint32_t w[4 * kGroups];
+#pragma unroll
+      for (int q = 0; q < kGroups; ++q) {
+        w[4 * q] = wn[q].x; w[4 * q + 1] = wn[q].y; w[4 * q + 2] = wn[q].z; w[4 * q + 3] = wn[q].w;
+        wn[q] = (kg + kGroups + q < ng) ? np[(size_t)(kg + kGroups + q) * 64] : make_uint4(0u, 0u, 0u, 0u);
+      }
       bool anynear = false;
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 8 * kGroups; ++u) {
         const bool valid = 8 * kg + u < c;
         const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
         const Slot qj = lds[e];
@@ -1215,7 +1224,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
       }
       if (anynear) {
         // rare: the batch holds a pair in the cutoff window; decide it from the FP64 positions
-        for (int u = 0; u < 8 && 8 * kg + u < c; ++u) {
+        for (int u = 0; u < 8 * kGroups && 8 * kg + u < c; ++u) {
           const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
           const Slot qj = lds[e];
           int tj = 0;
