@@ -134,7 +134,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--fp64-steps", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--prof-steps", type=int, default=100,
                     help="NVE steps after the timed run with per-kernel HIP events")
     ap.add_argument("--no-cpu", action="store_true")
