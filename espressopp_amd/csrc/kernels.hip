@@ -894,10 +894,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   const int nr = nruns[tile];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)tile * kMaxRuns + threadIdx.x];
   // Lanes take the tile's particles in order of their list length at the last
-  // build (carried with the particles through binning), so the lanes of a
-  // wavefront have lists of similar length in every kernel that walks them.
+  // build (carried with the particles through binning), rounded up to the pair
+  // kernels' 16-entry iterations, and in slot order within one length class.
+  // The lanes of a wavefront then run the same number of iterations, and
+  // neighbouring particles, whose lists are alike, share a wavefront, so their
+  // gathers of the staged positions fall on nearby LDS addresses.
   const int s0 = tile * blockDim.x;
-  skey[threadIdx.x] = (s0 + (int)threadIdx.x < n) ? pred[s0 + threadIdx.x] : 0x7FFFFFFF;
+  skey[threadIdx.x] = (s0 + (int)threadIdx.x < n) ? (pred[s0 + threadIdx.x] + 15) >> 4 : 0x7FFFFFFF;
   sval[threadIdx.x] = threadIdx.x;
   block_sort_pairs(skey, sval);
   skey[sval[threadIdx.x]] = threadIdx.x;  // rank of each particle (sval is no longer needed)
