@@ -13,8 +13,8 @@ import pytest
 
 from oracle.oracle import OracleSim
 
-from _systems import (cells_for, force_rel_err, gpu_system, jittered_lattice, mb_velocities, pair_set,
-                      random_walk_chains, read_fixture)
+from _systems import (cells_for, force_rel_err, gpu_system, jittered_lattice, mb_velocities, melt_chains,
+                      pair_set, random_walk_chains, read_fixture)
 
 pytestmark = pytest.mark.gpu
 
@@ -277,6 +277,83 @@ def test_fene_wca_melt(gpu, precision):
     assert fene.computeEnergy() == pytest.approx(ref.energy(rfene), rel=etol)
     assert wca.computeEnergy() == pytest.approx(ref.energy(ria), rel=etol)
     assert fene.computeVirial() == pytest.approx(ref.virial(rfene)[0], rel=etol)
+
+
+def _lattice_box(box, rho, seed, jitter=0.05):
+    # jittered simple-cubic lattice filling an orthorhombic box at about density rho
+    box = np.asarray(box, dtype=float)
+    a = (1.0 / rho) ** (1.0 / 3.0)
+    m = np.maximum((box / a).astype(int), 1)
+    g = [(np.arange(k) + 0.5) * (L / k) for k, L in zip(m, box)]
+    pos = np.stack(np.meshgrid(*g, indexing="ij"), axis=-1).reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    return pos + rng.uniform(-jitter, jitter, pos.shape)
+
+
+@pytest.mark.parametrize("case", ["two_types_orthorhombic", "fene_then_wca"])
+def test_step_variants_match_oracle(gpu, case):
+    # Multi-step trajectories through the force kernel that also finishes the step: several
+    # types in an orthorhombic box, and an LJ interaction that accumulates onto FENE forces
+    if case == "two_types_orthorhombic":
+        box = (14.2, 16.5, 19.3)
+        pos = _lattice_box(box, 0.7, seed=41)
+        n = len(pos)
+        types = (np.arange(n) % 3 == 0).astype(np.int32)
+        rc, skin = 2.5, 0.3
+        lj = ((0, 0, 1.0, 1.0), (0, 1, 1.2, 1.05), (1, 1, 0.8, 0.95))
+        bonds = None
+        dt = 0.005
+    else:
+        # a looser start than a dense melt, so 40 plain steps stay well resolved
+        pos, box, bonds = melt_chains(30, 40, 0.5, seed=42, min_dist=0.95)
+        n = len(pos)
+        types = None
+        rc, skin = 2 ** (1 / 6), 0.3
+        lj = ((0, 0, 1.0, 1.0),)
+        dt = 0.003
+    vel = mb_velocities(n, 1.0, seed=43)
+    cells = cells_for(box, rc, skin)
+    steps = 40
+    out = {}
+    for precision in ("fp64", "mixed"):
+        system = gpu_system(box, cells, skin, precision)
+        system.storage.addParticlesArray(np.arange(n), pos, types, vel)
+        system.storage.decompose()
+        if bonds is not None:
+            fpl = espressopp.FixedPairList(system.storage)
+            fpl.addBonds(bonds.tolist())
+            system.addInteraction(espressopp.interaction.FixedPairListFENE(
+                system, fpl, espressopp.interaction.FENE(30.0, 0.0, 1.5)))
+        vl = espressopp.VerletList(system, cutoff=rc)
+        ia = espressopp.interaction.VerletListLennardJones(vl)
+        for t1, t2, eps, sig in lj:
+            ia.setPotential(t1, t2, espressopp.interaction.LennardJones(eps, sig, rc, "auto"))
+        system.addInteraction(ia)
+        integ = espressopp.integrator.VelocityVerlet(system)
+        integ.dt = dt
+        integ.run(steps)
+        out[precision] = (system.storage.getParticleArrays(), integ.getNumResorts())
+    ref = OracleSim(box, cells, skin)
+    ref.add_particles(np.arange(n), pos, types=types, vel=vel)
+    ref.decompose()
+    if bonds is not None:
+        rfpl = ref.fpl_create()
+        ref.fpl_add(rfpl, bonds)
+        ref.fene_create(rfpl, 30.0, 0.0, 1.5)
+    ria = ref.lj_create(ref.vl_create(rc))
+    for t1, t2, eps, sig in lj:
+        ref.lj_set(ria, t1, t2, eps, sig, rc, "auto")
+    ref.run(steps, dt=dt)
+    rp = ref.get("pos") + ref.get("image") * np.asarray(box)
+    rv = ref.get("v")
+    for precision, tol in (("fp64", 1e-9), ("mixed", 1e-4)):
+        a, resorts = out[precision]
+        gp = a["pos"] + a["image"] * np.asarray(box)
+        assert np.abs(gp - rp).max() < tol, precision
+        assert np.abs(a["v"] - rv).max() < 100 * tol, precision
+        # the forces a caller reads after the run are those of the last step
+        assert force_rel_err(a["f"], ref.get("f"), 0.1).max() <= (1e-9 if precision == "fp64" else 1e-4)
+    assert out["fp64"][1] == ref.resorts
 
 
 def test_pressure_and_temperature(gpu):
