@@ -1176,8 +1176,9 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     double lo = CUBIC ? p0.ulo : p0.dlo, hi = CUBIC ? p0.uhi : p0.dhi;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     // kGroups 16-byte groups (8 entries each) per iteration, the next ones loaded
-    // during it
-    constexpr int kGroups = 2;
+    // during it; with several types one group, so the per-entry type-pair
+    // coefficients of a batch stay within the register budget
+    constexpr int kGroups = ONETYPE ? 2 : 1;
     const int ng = (c + 7) >> 3;
     uint4 wn[kGroups];
 #pragma unroll
