@@ -825,22 +825,42 @@ __device__ __forceinline__ bool exact_sqr_le(double ex, double ey, double ez, do
   return ex * ex + ey * ey + ez * ez <= cutsq;
 }
 
-// Ascending bitonic sort of the block's (key, value) pairs in LDS, one pair per
-// thread, blockDim.x a power of two; equal keys keep value order.
-__device__ void block_sort_pairs(int* skey, int* sval) {
-  const int t = threadIdx.x, nt = blockDim.x;
-  for (int size = 2; size <= nt; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      __syncthreads();
-      const int o = t ^ stride;
-      if (o > t) {
-        const int ka = skey[t], kb = skey[o], va = sval[t], vb = sval[o];
-        const bool gt = ka > kb || (ka == kb && va > vb);
-        if (gt == ((t & size) == 0)) { skey[t] = kb; skey[o] = ka; sval[t] = vb; sval[o] = va; }
-      }
-    }
+// Rank of each thread of the workgroup in order of (cls, thread index), for small
+// class values 0 <= cls < 64 and at most 256 threads: the number of threads with
+// a smaller class plus the lower-indexed threads of the same class.  Lanes of a
+// wavefront with equal classes are counted by one ballot per distinct class.
+__device__ int block_class_rank(int cls) {
+  __shared__ int wcount[4][64];                  // per wavefront, threads of each class
+  __shared__ int cbase[64];                      // threads of all smaller classes
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  for (int q = threadIdx.x; q < 4 * 64; q += blockDim.x) wcount[q >> 6][q & 63] = 0;
+  __syncthreads();
+  unsigned long long todo = __ballot(1);
+  int within = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lc = __shfl(cls, leader, 64);
+    const unsigned long long m = __ballot(cls == lc);
+    if (cls == lc) within = __popcll(m & ((1ull << ln) - 1ull));
+    if (ln == leader) wcount[wv][lc] = __popcll(m);
+    todo &= ~m;
   }
   __syncthreads();
+  if (threadIdx.x < 64) {
+    const int c = threadIdx.x;
+    const int tot = wcount[0][c] + wcount[1][c] + wcount[2][c] + wcount[3][c];
+    int incl = tot;                              // inclusive scan over the 64 classes
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
+      if (c >= o) incl += y;
+    }
+    cbase[c] = incl - tot;
+  }
+  __syncthreads();
+  int r = cbase[cls] + within;
+  for (int w = 0; w < wv; ++w) r += wcount[w][cls];
+  return r;
 }
 
 // Workgroup -> tile map that keeps consecutive tiles on one XCD: workgroups are
@@ -889,7 +909,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     int* __restrict__ cnt, int capg, int* __restrict__ pred, int* __restrict__ lane_slot, float clip_margin,
     Ctl* ctl) {
   __shared__ TileRun sr[kMaxRuns];
-  __shared__ int skey[256], sval[256];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = nruns[tile];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)tile * kMaxRuns + threadIdx.x];
@@ -900,15 +919,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   // neighbouring particles, whose lists are alike, share a wavefront, so their
   // gathers of the staged positions fall on nearby LDS addresses.
   const int s0 = tile * blockDim.x;
-  skey[threadIdx.x] = (s0 + (int)threadIdx.x < n) ? (pred[s0 + threadIdx.x] + 15) >> 4 : 0x7FFFFFFF;
-  sval[threadIdx.x] = threadIdx.x;
-  block_sort_pairs(skey, sval);
-  skey[sval[threadIdx.x]] = threadIdx.x;  // rank of each particle (sval is no longer needed)
-  __syncthreads();
+  const int cls = (s0 + (int)threadIdx.x < n) ? min((pred[s0 + threadIdx.x] + 15) >> 4, 62) : 63;
+  const int rank = block_class_rank(cls);
   // threads stay in slot order (lanes of one cell read the same candidates);
   // each particle's list goes to the lane of its rank
   const int i = s0 + threadIdx.x;        // particle slot
-  const int L = s0 + skey[threadIdx.x];  // list lane (slots past n rank last, with lanes past n)
+  const int L = s0 + rank;               // list lane (slots past n rank last, with lanes past n)
   int k = 0;
   if (i < n && nr > 0) {
     const double4 pi = pos[i];
