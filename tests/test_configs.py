@@ -102,3 +102,55 @@ def test_kremer_grest_melt_200k(gpu):
         assert lj.computeEnergy() == pytest.approx(ref.energy(wca), rel=etol)
         assert fb.computeEnergy() == pytest.approx(ref.energy(fene), rel=etol)
         assert lj.computeVirial() == pytest.approx(ref.virial(wca)[0], rel=10 * etol)
+
+
+def test_thermostatted_statistics_mixed_vs_fp64(gpu):
+    # Langevin LJ fluid, N = 8000, T = 1: the mixed and FP64 paths run with independent
+    # noise, and their temperature, pressure and g(r) agree within the statistical error
+    # of the samples (block means, samples 200 steps apart); both hold T at the target
+    n = 20 ** 3
+    pos, box = jittered_lattice(n, 0.8442, seed=61)
+    vel = mb_velocities(n, 1.0, seed=62)
+    cells = cells_for(box, 2.5, 0.3)
+    nsamp, gap, nbins = 30, 200, 60
+    stats = {}
+    for precision, seed in (("fp64", 101), ("mixed", 202)):
+        system = gpu_system(box, cells, 0.3, precision, seed=seed)
+        system.storage.addParticlesArray(np.arange(n), pos, None, vel)
+        system.storage.decompose()
+        vl = espressopp.VerletList(system, cutoff=2.5)
+        lj = espressopp.interaction.VerletListLennardJones(vl)
+        lj.setPotential(0, 0, espressopp.interaction.LennardJones(1.0, 1.0, 2.5, "auto"))
+        system.addInteraction(lj)
+        integ = espressopp.integrator.VelocityVerlet(system)
+        integ.dt = 0.005
+        lang = espressopp.integrator.LangevinThermostat(system)
+        lang.gamma = 1.0
+        lang.temperature = 1.0
+        integ.addExtension(lang)
+        integ.run(1000)
+        T, P, G = [], [], []
+        for _ in range(nsamp):
+            integ.run(gap)
+            T.append(espressopp.analysis.Temperature(system).compute())
+            P.append(espressopp.analysis.Pressure(system).compute())
+            G.append(espressopp.analysis.RadialDistrF(system).compute(nbins))
+        stats[precision] = (np.array(T), np.array(P), np.array(G))
+
+    def mean_se(a):
+        # standard error from means of 5 blocks of consecutive samples
+        blocks = a.reshape(5, -1, *a.shape[1:]).mean(axis=1)
+        return a.mean(axis=0), blocks.std(axis=0, ddof=1) / np.sqrt(5)
+
+    for precision in stats:
+        mt, st = mean_se(stats[precision][0])
+        assert abs(mt - 1.0) < max(5 * st, 0.01), (precision, mt, st)
+    for q in range(3):
+        m1, s1 = mean_se(stats["fp64"][q])
+        m2, s2 = mean_se(stats["mixed"][q])
+        z = np.abs(m1 - m2) / np.maximum(np.sqrt(s1 ** 2 + s2 ** 2), 1e-3)
+        if q < 2:
+            assert z < 5.0, (q, m1, m2, s1, s2)
+        else:
+            mask = m1 > 0.2                        # bins inside the first shell and beyond
+            assert z[mask].max() < 6.0, (z[mask].max(), int(np.argmax(z * mask)))
