@@ -46,6 +46,12 @@ void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool 
                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
                      int ntypes, double* fx, double* fy, double* fz, double* partial);
 int lj_tiled_blocks(int n, int T);
+// FENE bonds as the last force of a step, finishing it per particle as
+// launch_lj_tiled_step does (sa.blockmax needs nblocks(n, 256) entries).
+void launch_fene_step(hipStream_t s, bool mixed, bool accum, int n, const double4* pos, const uint4* posq,
+                      const int* id, const int* slot_of_id, int nid, const int* bstart, const int* bpartner,
+                      const Geom& g, const FENEParam& fp, double* fx, double* fy, double* fz, const StepArgs& sa,
+                      Ctl* ctl, unsigned long long* host_out);
 // Mixed tiled LJ as the last force of a step, finishing the step per particle
 // (thermostat, integrate2, integrate1 into sa.pos_out / sa.posq_out), then the
 // step's max |dx|^2 (sa.blockmax needs lj_tiled_blocks(n, T) entries) into

@@ -1673,15 +1673,20 @@ __global__ void k_posq(int n, const double4* __restrict__ pos, uint4* __restrict
 // Distance: minimum image as OrthorhombicBC::getMinimumImageVectorBox
 // (src/bc/OrthorhombicBC.cpp:86-105) in FP64; in mixed mode from the fixed-point
 // mirror, which is the same minimum image for |d| < L/2.
-template <bool MIXED, bool ENERGY, bool ACCUM>
+// STEP: the bonds are the last force of the step and the kernel finishes it
+// per particle, as the step variant of k_lj_tiled_mixed does (new positions to
+// sa.pos_out / sa.posq_out, since other threads still read the current ones).
+template <bool MIXED, bool ENERGY, bool ACCUM, bool STEP>
 __global__ __launch_bounds__(256) void k_fene(
     int n, const double4* __restrict__ pos, const uint4* __restrict__ posq,
     const int* __restrict__ id, const int* __restrict__ slot_of_id, int nid,
     const int* __restrict__ bstart, const int* __restrict__ bpartner, Geom g, FENEParam fp,
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-    double* __restrict__ partial) {
+    double* __restrict__ partial, StepArgs sa) {
 #pragma clang fp contract(off)
+  static_assert(!(STEP && ENERGY), "the step variant computes forces only");
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double stepsq = 0.0;
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
@@ -1742,11 +1747,18 @@ __global__ __launch_bounds__(256) void k_fene(
         acc[5] += dx * dy * ff; acc[6] += dx * dz * ff; acc[7] += dy * dz * ff;
       }
     }
-    if (!ENERGY) {
+    if (STEP) {
+      double f0 = Ax, f1 = Ay, f2 = Az;
+      if (ACCUM) { f0 = fx[i] + Ax; f1 = fy[i] + Ay; f2 = fz[i] + Az; }
+      const double4 v = sa.vel[i];
+      step_thermostat(i, v, f0, f1, f2, sa);
+      stepsq = step_kick_drift(i, v, sa.pos[i], f0, f1, f2, sa, g);
+    } else if (!ENERGY) {
       if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
       else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
     }
   }
+  if (STEP) step_block_max(stepsq, sa);
   if (ENERGY) {
 #pragma unroll
     for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
@@ -2097,13 +2109,27 @@ void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, cons
                  double* fz, double* partial) {
   if (n <= 0) return;
   dim3 grid(nblocks(n, 256)), block(256);
-#define FN(MX, EN, AC) hipLaunchKernelGGL((k_fene<MX, EN, AC>), grid, block, 0, s, n, pos, posq, id, slot_of_id, nid, bstart, bpartner, g, fp, fx, fy, fz, partial)
+  const StepArgs sa{};
+#define FN(MX, EN, AC) hipLaunchKernelGGL((k_fene<MX, EN, AC, false>), grid, block, 0, s, n, pos, posq, id, slot_of_id, nid, bstart, bpartner, g, fp, fx, fy, fz, partial, sa)
   if (mixed) {
     if (energy) FN(true, true, false); else if (accum) FN(true, false, true); else FN(true, false, false);
   } else {
     if (energy) FN(false, true, false); else if (accum) FN(false, false, true); else FN(false, false, false);
   }
 #undef FN
+}
+
+void launch_fene_step(hipStream_t s, bool mixed, bool accum, int n, const double4* pos, const uint4* posq,
+                      const int* id, const int* slot_of_id, int nid, const int* bstart, const int* bpartner,
+                      const Geom& g, const FENEParam& fp, double* fx, double* fy, double* fz, const StepArgs& sa,
+                      Ctl* ctl, unsigned long long* host_out) {
+  if (n <= 0) return;
+  const int nb = nblocks(n, 256);
+#define FN(MX, AC) hipLaunchKernelGGL((k_fene<MX, false, AC, true>), dim3(nb), dim3(256), 0, s, n, pos, posq, id, slot_of_id, nid, bstart, bpartner, g, fp, fx, fy, fz, nullptr, sa)
+  if (mixed) { if (accum) FN(true, true); else FN(true, false); }
+  else { if (accum) FN(false, true); else FN(false, false); }
+#undef FN
+  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, sa.blockmax, ctl, host_out);
 }
 
 void launch_tab_tiled(hipStream_t s, bool mixed, bool energy, bool accum, int n, int T, int lds_bytes,

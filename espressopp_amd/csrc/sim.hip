@@ -817,9 +817,10 @@ struct espp_sim {
   // Returns false, having launched nothing, when the last interaction is not
   // such a kernel.
   bool calc_forces_step(StepArgs sa) {
-    if (!mixed() || active.empty() || n == 0) return false;
+    if (active.empty() || n == 0) return false;
     InteractionState& last = *ias[(size_t)active.back()];
-    if (last.kind != 0) return false;
+    if (last.kind == 1) return fene_step(last, sa);
+    if (!mixed() || last.kind != 0) return false;
     ensure_lists();
     VerletListState& v = *vls[(size_t)last.list];
     if (!v.tiled) return false;
@@ -841,6 +842,29 @@ struct espp_sim {
     prof_end(6, pe);
     pos.swap(pos2);
     posq.swap(posq2);
+    return true;
+  }
+
+  // The same with FENE bonds as the last interaction (either precision).
+  bool fene_step(InteractionState& last, StepArgs sa) {
+    ensure_lists();
+    for (size_t q = 0; q + 1 < active.size(); ++q) run_interaction(*ias[(size_t)active[q]], false, q > 0, nullptr);
+    FixedPairListState& f = *fpls[(size_t)last.list];
+    if (f.dirty || f.built_nid != nid) upload_bonds(f);
+    blockmax.resize((size_t)std::max(kinetic_blocks((int)n), 1));
+    sa.id = id.p;
+    sa.vel = vel.p;
+    sa.pos = pos.p;
+    sa.pos_out = pos2.p;
+    sa.posq_out = mixed() ? posq2.p : nullptr;
+    sa.blockmax = blockmax.p;
+    __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
+    hipEvent_t pe = prof_begin();
+    launch_fene_step(stream, mixed(), active.size() > 1, (int)n, pos.p, posq.p, id.p, slot_of_id.p, nid, f.bstart.p,
+                     f.bpartner.p, g, last.fp, fx.p, fy.p, fz.p, sa, ctl.p, dmax);
+    prof_end(5, pe);
+    pos.swap(pos2);
+    if (mixed()) posq.swap(posq2);
     return true;
   }
 

@@ -290,10 +290,11 @@ def _lattice_box(box, rho, seed, jitter=0.05):
     return pos + rng.uniform(-jitter, jitter, pos.shape)
 
 
-@pytest.mark.parametrize("case", ["two_types_orthorhombic", "fene_then_wca"])
+@pytest.mark.parametrize("case", ["two_types_orthorhombic", "fene_then_wca", "wca_then_fene"])
 def test_step_variants_match_oracle(gpu, case):
     # Multi-step trajectories through the force kernel that also finishes the step: several
-    # types in an orthorhombic box, and an LJ interaction that accumulates onto FENE forces
+    # types in an orthorhombic box, an LJ interaction that accumulates onto FENE forces, and
+    # FENE bonds accumulating onto WCA forces (the order of the reference's melt benchmark)
     if case == "two_types_orthorhombic":
         box = (14.2, 16.5, 19.3)
         pos = _lattice_box(box, 0.7, seed=41)
@@ -314,21 +315,26 @@ def test_step_variants_match_oracle(gpu, case):
     vel = mb_velocities(n, 1.0, seed=43)
     cells = cells_for(box, rc, skin)
     steps = 40
+    fene_last = case == "wca_then_fene"
     out = {}
     for precision in ("fp64", "mixed"):
         system = gpu_system(box, cells, skin, precision)
         system.storage.addParticlesArray(np.arange(n), pos, types, vel)
         system.storage.decompose()
+        fene = None
         if bonds is not None:
             fpl = espressopp.FixedPairList(system.storage)
             fpl.addBonds(bonds.tolist())
-            system.addInteraction(espressopp.interaction.FixedPairListFENE(
-                system, fpl, espressopp.interaction.FENE(30.0, 0.0, 1.5)))
+            fene = espressopp.interaction.FixedPairListFENE(system, fpl, espressopp.interaction.FENE(30.0, 0.0, 1.5))
+            if not fene_last:
+                system.addInteraction(fene)
         vl = espressopp.VerletList(system, cutoff=rc)
         ia = espressopp.interaction.VerletListLennardJones(vl)
         for t1, t2, eps, sig in lj:
             ia.setPotential(t1, t2, espressopp.interaction.LennardJones(eps, sig, rc, "auto"))
         system.addInteraction(ia)
+        if fene is not None and fene_last:
+            system.addInteraction(fene)
         integ = espressopp.integrator.VelocityVerlet(system)
         integ.dt = dt
         integ.run(steps)
@@ -336,13 +342,17 @@ def test_step_variants_match_oracle(gpu, case):
     ref = OracleSim(box, cells, skin)
     ref.add_particles(np.arange(n), pos, types=types, vel=vel)
     ref.decompose()
-    if bonds is not None:
+    if bonds is not None and not fene_last:
         rfpl = ref.fpl_create()
         ref.fpl_add(rfpl, bonds)
         ref.fene_create(rfpl, 30.0, 0.0, 1.5)
     ria = ref.lj_create(ref.vl_create(rc))
     for t1, t2, eps, sig in lj:
         ref.lj_set(ria, t1, t2, eps, sig, rc, "auto")
+    if bonds is not None and fene_last:
+        rfpl = ref.fpl_create()
+        ref.fpl_add(rfpl, bonds)
+        ref.fene_create(rfpl, 30.0, 0.0, 1.5)
     ref.run(steps, dt=dt)
     rp = ref.get("pos") + ref.get("image") * np.asarray(box)
     rv = ref.get("v")
