@@ -15,7 +15,7 @@ void launch_step_fused(hipStream_t s, int n, double* fx, double* fy, double* fz,
 void launch_permute_int(hipStream_t s, int n, const int* order, const int* src, int* dst);
 void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start,
                     int* order);
-void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const double4* pos);
+void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const double4* pos, const Ctl* ctl);
 void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, const double4* pos,
                     const double4* vel, const double* fx, const double* fy, const double* fz,
                     const int* id, const int4* img, double4* pos2, double4* vel2, double* fx2,

@@ -125,7 +125,8 @@ __global__ void k_fold_cell(int n, double4* __restrict__ pos, int4* __restrict__
   bool bad = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (!(x[d] == x[d])) { bad = true; c[d] = 0; continue; }
+    // NaN, infinite, or so far outside that the image count would overflow
+    if (!(fabs(x[d] * g.invL[d]) < 1.0e9)) { bad = true; c[d] = 0; continue; }
     if (do_fold) {
       int tmp = (int)floor(x[d] * g.invL[d]);
       iv[d] += tmp;
@@ -324,9 +325,11 @@ __global__ __launch_bounds__(kSortLanes * kSortCellsPerBlock) void k_sort_cells(
 // The same ordering for cells with more than kSortMax members (one thread per
 // cell, insertion sort in global memory); every other cell returns at once.
 __global__ void k_sort_cells_big(int ncell, const int* __restrict__ start, int* __restrict__ order,
-                                 const double4* __restrict__ pos) {
+                                 const double4* __restrict__ pos, const Ctl* __restrict__ ctl) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
+  // a binning that met a lost particle (all of them land in cell 0) is abandoned
+  // by the host; skip the quadratic sort of that cell
+  if (c >= ncell || ctl->nanpos) return;
   const int b = start[c], e = start[c + 1];
   if (e - b <= kSortMax) return;
   for (int k = b + 1; k < e; ++k) {
@@ -1968,10 +1971,10 @@ void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, c
   if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, s, n, cell_of, rank, start, order);
 }
 
-void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const double4* pos) {
+void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const double4* pos, const Ctl* ctl) {
   hipLaunchKernelGGL(k_sort_cells, dim3(nblocks(ncell, kSortCellsPerBlock)), dim3(kSortLanes * kSortCellsPerBlock), 0,
                      s, ncell, start, order, pos);
-  hipLaunchKernelGGL(k_sort_cells_big, dim3(nblocks(ncell, 256)), dim3(256), 0, s, ncell, start, order, pos);
+  hipLaunchKernelGGL(k_sort_cells_big, dim3(nblocks(ncell, 256)), dim3(256), 0, s, ncell, start, order, pos, ctl);
 }
 
 void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, const double4* pos,

@@ -476,7 +476,7 @@ struct espp_sim {
     launch_scan_multi(stream, (int)ncell(), cell_count.p, cstart.p, scan_sum.p, scan_off.p);
     if (ni > 0) {
       launch_scatter(stream, ni, cell_of.p, rank.p, cstart.p, order.p);
-      launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, pos.p);
+      launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, pos.p, ctl.p);
       launch_permute(stream, ni, order.p, cell_of.p, pos.p, vel.p, fx.p, fy.p, fz.p, id.p, img.p, pos2.p,
                      vel2.p, carry_forces ? fx2.p : nullptr, fy2.p, fz2.p, id2.p, img2.p, cell_of_slot.p,
                      slot_of_id.p, posq.p, posf.p, g);
@@ -495,10 +495,11 @@ struct espp_sim {
     disp_since_bin = 0.0;
     plan_valid = false;
     for (auto* v : vls) v->stale = true;
-    if (do_fold) {
-      coord_bound = std::max(g.L[0], std::max(g.L[1], g.L[2]));
+    if (do_fold) coord_bound = std::max(g.L[0], std::max(g.L[1], g.L[2]));
+    {
       read_ctl();
-      if (hctl->nanpos) fail(ESPP_ERR_NAN, "particle has moved to outer space (one or more coordinates are nan)");
+      if (hctl->nanpos)
+        fail(ESPP_ERR_NAN, "particle has moved to outer space (one or more coordinates are nan or infinite)");
     }
     sorted = true;
   }
@@ -969,7 +970,8 @@ struct espp_sim {
       }
       auto tb = clk::now();
       t_int1 += std::chrono::duration<double>(tb - ta).count();
-      if (maxDist > skinHalf) resortFlag = true;
+      // a NaN displacement also resorts, so the binning reports the lost particle
+      if (!(maxDist <= skinHalf)) resortFlag = true;
       if (resortFlag) {
         decompose(false);   // forces are recomputed below
         maxDist = 0.0;

@@ -394,6 +394,26 @@ def test_empty_system_and_errors(gpu):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lost_particle_is_reported(gpu, precision):
+    # an infinite coordinate, and a particle sent off by a runaway velocity, end in the
+    # reference's "moved to outer space" error at the next binning instead of a long
+    # binning of one overfull cell or a silent run on NaNs
+    pos, box = jittered_lattice(4000, 0.8442, seed=71)
+    cells = cells_for(box, 2.5, 0.3)
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells, 2.5, 0.3, precision)
+    integ = espressopp.integrator.VelocityVerlet(system)
+    integ.dt = 0.005
+    integ.run(5)
+    system.storage.setParticleArrays("v", [17], [[1e200, 0.0, 0.0]])
+    with pytest.raises(RuntimeError, match="outer space"):
+        integ.run(5)
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells, 2.5, 0.3, precision)
+    system.storage.setParticleArrays("pos", [5], [[float("inf"), 1.0, 1.0]])
+    with pytest.raises(RuntimeError, match="outer space"):
+        system.storage.decompose()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_free_flight_gpu(gpu, precision):
     # testsuite/integrator/PTestVelocityVerlet.cpp:222-256: free particles cross the box and return
     pos, box = jittered_lattice(125, 0.5, seed=3, jitter=0.0)
