@@ -62,7 +62,6 @@ struct TileRun {
   int slot_start, count, lds_off, cell_a, cell_b, pad0, pad1, pad2;
 };
 constexpr int kMaxRuns = 64;      // runs per tile
-constexpr int kMaxPieces = 128;   // row segments examined while planning a tile
 
 // Integrator arguments of a step applied per particle: the thermostat on the
 // force (LangevinThermostat, aftCalcF), integrate2 when do_int2 is set, then

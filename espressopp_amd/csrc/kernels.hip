@@ -655,81 +655,121 @@ __global__ __launch_bounds__(256) void k_lj_fp64(
 
 __device__ __forceinline__ int wrap_dim(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-// One thread per tile, kPlanThreads tiles per workgroup.  Each thread keeps its
-// row pieces in LDS (interleaved by thread, so the threads of a wavefront hit
-// distinct banks).  Runs are sorted by cell index and never overlap.
-constexpr int kPlanThreads = 32;
-__global__ __launch_bounds__(kPlanThreads) void k_tile_plan(int n, int T, int ntiles, const int* __restrict__ cell_of_slot,
-                            const int* __restrict__ cstart, Geom g, int budget,
-                            TileRun* __restrict__ runs, int* __restrict__ nruns, Ctl* ctl) {
-  __shared__ int spa[kMaxPieces * kPlanThreads], spb[kMaxPieces * kPlanThreads];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
+// One wavefront per tile, four tiles per workgroup.  A tile touches at most
+// three consecutive cell rows (more and the plan fails, and the host tries a
+// smaller tile); each of those rows has 9 neighbour rows, each of which gives a
+// piece of at most 3 cells, split in two where it wraps: lane
+// (row * 9 + neighbour) * 2 + part holds one piece.  The pieces are sorted by
+// first cell across the wavefront (bitonic, by shuffles), a piece that does not
+// touch the ones before it starts a run (a prefix maximum of the piece ends),
+// and each run's slot range and LDS offset follow from the cell starts and a
+// prefix sum.  Runs are sorted by cell index and never overlap.  The largest
+// region goes to ctl by one atomic per workgroup (same-address atomics from
+// every wavefront of the grid would serialise).
+constexpr int kPlanTilesPerBlock = 16;
+__global__ __launch_bounds__(64 * kPlanTilesPerBlock) void k_tile_plan(
+    int n, int T, int ntiles, const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g,
+    int budget, TileRun* __restrict__ runs, int* __restrict__ nruns, Ctl* ctl) {
+  __shared__ int run_a[kPlanTilesPerBlock][64], run_b[kPlanTilesPerBlock][64];
+  __shared__ int region_max[kPlanTilesPerBlock];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int t = blockIdx.x * kPlanTilesPerBlock + wv;
+  const bool live = t < ntiles;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-  const int s0 = t * T, s1 = min(s0 + T, n) - 1;
+  const int s0 = live ? t * T : 0, s1 = live ? min(s0 + T, n) - 1 : 0;
   const int c0 = cell_of_slot[s0], c1 = cell_of_slot[s1];
   const int r0 = c0 / nx, r1 = c1 / nx;
-  struct Strided {
-    int* base;
-    __device__ int& operator[](int k) const { return base[k * kPlanThreads]; }
-  };
-  const Strided pa{spa + threadIdx.x}, pb{spb + threadIdx.x};
-  int np = 0;
-  bool bad = (r1 - r0 + 1) * 18 > kMaxPieces;
-  for (int r = r0; r <= r1 && !bad; ++r) {
-    const int y = r % ny, z = r / ny;
-    const int xa = (r == r0) ? c0 % nx : 0, xb = (r == r1) ? c1 % nx : nx - 1;
-    const int lo = xa - 1, hi = xb + 1;
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int zz = wrap_dim(z + dz, nz);
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int yy = wrap_dim(y + dy, ny);
-        const int rb = (yy + ny * zz) * nx;
-        if (hi - lo + 1 >= nx) {
-          pa[np] = rb; pb[np] = rb + nx - 1; ++np;
-        } else if (lo < 0) {
-          pa[np] = rb + lo + nx; pb[np] = rb + nx - 1; ++np;
-          pa[np] = rb; pb[np] = rb + hi; ++np;
-        } else if (hi >= nx) {
-          pa[np] = rb + lo; pb[np] = rb + nx - 1; ++np;
-          pa[np] = rb; pb[np] = rb + hi - nx; ++np;
-        } else {
-          pa[np] = rb + lo; pb[np] = rb + hi; ++np;
-        }
+  const bool bad = r1 - r0 + 1 > 3;
+  constexpr int kNone = 0x7FFFFFFF;
+  int pa = kNone, pb = -1;
+  if (live && !bad && lane < 54) {
+    const int ri = lane / 18, rem = lane % 18, combo = rem >> 1, part = rem & 1;
+    const int r = r0 + ri;
+    if (r <= r1) {
+      const int y = r % ny, z = r / ny;
+      const int xa = (r == r0) ? c0 % nx : 0, xb = (r == r1) ? c1 % nx : nx - 1;
+      const int lo = xa - 1, hi = xb + 1;
+      const int zz = wrap_dim(z + combo / 3 - 1, nz), yy = wrap_dim(y + combo % 3 - 1, ny);
+      const int rb = (yy + ny * zz) * nx;
+      if (hi - lo + 1 >= nx) {
+        if (part == 0) { pa = rb; pb = rb + nx - 1; }
+      } else if (lo < 0) {
+        if (part == 0) { pa = rb + lo + nx; pb = rb + nx - 1; } else { pa = rb; pb = rb + hi; }
+      } else if (hi >= nx) {
+        if (part == 0) { pa = rb + lo; pb = rb + nx - 1; } else { pa = rb; pb = rb + hi - nx; }
+      } else if (part == 0) {
+        pa = rb + lo; pb = rb + hi;
       }
     }
   }
-  int nr = 0, total = 0;
-  if (!bad) {
-    // insertion sort of the pieces by first cell, then merge overlapping or adjacent ranges
-    for (int a = 1; a < np; ++a) {
-      const int ka = pa[a], kb = pb[a];
-      int m = a - 1;
-      while (m >= 0 && pa[m] > ka) { pa[m + 1] = pa[m]; pb[m + 1] = pb[m]; --m; }
-      pa[m + 1] = ka; pb[m + 1] = kb;
-    }
-    int ca = pa[0], cb = pb[0];
-    for (int a = 1; a <= np && !bad; ++a) {
-      if (a < np && pa[a] <= cb + 1) { cb = max(cb, pb[a]); continue; }
-      if (nr >= kMaxRuns) { bad = true; break; }
-      TileRun R;
-      R.cell_a = ca; R.cell_b = cb;
-      R.slot_start = cstart[ca];
-      R.count = cstart[cb + 1] - R.slot_start;
-      R.lds_off = total;
-      R.pad0 = R.pad1 = R.pad2 = 0;
-      runs[(size_t)t * kMaxRuns + nr] = R;
-      total += R.count;
-      ++nr;
-      if (a < np) { ca = pa[a]; cb = pb[a]; }
+  // bitonic sort of the pieces by (first cell, lane) across the wavefront
+  int key = pa, val = pb, tie = lane;
+  for (int size = 2; size <= 64; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int ok = __shfl_xor(key, stride, 64), ov = __shfl_xor(val, stride, 64), ot = __shfl_xor(tie, stride, 64);
+      const bool up = (lane & size) == 0;
+      const bool lower = (lane & stride) == 0;
+      const bool other_less = ok < key || (ok == key && ot < tie);
+      // the lower lane of an ascending pair keeps the smaller element
+      if (other_less == (lower == up)) { key = ok; val = ov; tie = ot; }
     }
   }
-  if (bad || total > budget || total > 65535) {
-    ctl->plan_fail = 1;
-    nr = 0;
+  const bool valid = key != kNone;
+  int incl = valid ? val : -2;                   // inclusive prefix maximum of the piece ends
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl = max(incl, y);
   }
-  nruns[t] = nr;
-  atomicMax(&ctl->max_region, total);
+  int excl = __shfl_up(incl, 1, 64);
+  if (lane == 0) excl = -2;
+  const bool start = valid && key > excl + 1;
+  const unsigned long long starts = __ballot(start);
+  const int id = __popcll(starts & ((2ull << lane) - 1ull)) - 1;   // run of this piece
+  const bool next_start = lane < 63 && ((starts >> (lane + 1)) & 1ull);
+  const bool next_valid = __shfl_down(valid ? 1 : 0, 1, 64) != 0 && lane < 63;
+  const bool last = valid && (!next_valid || next_start);
+  if (start) run_a[wv][id] = key;
+  if (last) run_b[wv][id] = incl;
+  __syncthreads();
+  const int nr = __popcll(starts);
+  int slot_start = 0, count = 0, ca = 0, cb = 0;
+  if (live && !bad && lane < nr) {
+    ca = run_a[wv][lane];
+    cb = run_b[wv][lane];
+    slot_start = cstart[ca];
+    count = cstart[cb + 1] - slot_start;
+  }
+  int off = count;                               // inclusive prefix sum of the counts
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(off, o, 64);
+    if (lane >= o) off += y;
+  }
+  const int total = __shfl(off, 63, 64);
+  const bool fail = bad || nr > kMaxRuns || total > budget || total > 65535;
+  if (lane == 0) region_max[wv] = (live && !fail) ? total : 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = 0;
+    for (int w = 0; w < kPlanTilesPerBlock; ++w) m = max(m, region_max[w]);
+    atomicMax(&ctl->max_region, m);
+  }
+  if (!live) return;
+  if (!fail && lane < nr) {
+    TileRun R;
+    R.slot_start = slot_start;
+    R.count = count;
+    R.lds_off = off - count;
+    R.cell_a = ca;
+    R.cell_b = cb;
+    R.pad0 = R.pad1 = R.pad2 = 0;
+    runs[(size_t)t * kMaxRuns + lane] = R;
+  }
+  if (lane == 0) {
+    if (fail) ctl->plan_fail = 1;
+    nruns[t] = fail ? 0 : nr;
+  }
 }
 
 // run holding cell c (runs sorted by cell_a, c guaranteed inside one of them)
@@ -2038,8 +2078,8 @@ void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum,
 void launch_tile_plan(hipStream_t s, int n, int T, int ntiles, const int* cell_of_slot, const int* cstart,
                       const Geom& g, int budget, TileRun* runs, int* nruns, Ctl* ctl) {
   if (ntiles > 0)
-    hipLaunchKernelGGL(k_tile_plan, dim3(nblocks(ntiles, kPlanThreads)), dim3(kPlanThreads), 0, s, n, T, ntiles,
-                       cell_of_slot, cstart, g, budget, runs, nruns, ctl);
+    hipLaunchKernelGGL(k_tile_plan, dim3(nblocks(ntiles, kPlanTilesPerBlock)), dim3(64 * kPlanTilesPerBlock), 0, s, n,
+                       T, ntiles, cell_of_slot, cstart, g, budget, runs, nruns, ctl);
 }
 
 void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Geom& g) {
