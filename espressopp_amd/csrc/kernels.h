@@ -83,9 +83,6 @@ void launch_thermo_integrate2(hipStream_t s, int n, const int* id, double4* vel,
                               double* fy, double* fz, int lang, double pref1, double pref2,
                               unsigned long long seed, unsigned long long call,
                               const unsigned char* excl, int nid, int do_int2, double half_dt);
-void launch_integrate1(hipStream_t s, int n, double4* pos, double4* vel, const double* fx,
-                       const double* fy, const double* fz, double dt, uint4* posq, const Geom& g,
-                       Ctl* ctl);
 int kinetic_blocks(int n);
 void launch_kinetic(hipStream_t s, int n, const double4* vel, const double4* pos,
                     unsigned long long type_mask, double* partial);

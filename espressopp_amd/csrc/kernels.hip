@@ -1,16 +1,19 @@
 // HIP kernels of the MI355X short-range MD step (gfx950, wave64).
 //
 //   binning          k_fold_cell, k_scan_cells, k_scatter, k_sort_cells, k_permute
+//   tiles            k_tile_plan (staged LDS region of each tile of particles)
 //   neighbour build  k_nbr_build / k_nbr_build_tiled (full list, wavefront-padded
 //                                   ragged SoA; 16-bit LDS offsets when tiled)
 //   pair forces      k_lj_tiled_mixed / k_lj_mixed (32-bit fixed-point positions,
 //                                   exact FP64 r^2, FP32-seeded reciprocal,
-//                                   FP64 LJ factor and force accumulation)
-//                    k_lj_fp64     (all-FP64 comparison path)
-//   bonds            k_fene        (per-particle bond adjacency, no atomics)
+//                                   FP64 LJ factor and force accumulation; the
+//                                   step variant also finishes the step)
+//                    k_lj_tiled_fp64 / k_lj_fp64 (all-FP64 comparison path)
+//                    k_tab_tiled / k_tab (tabulated potentials)
+//   bonds            k_fene, k_bond_tab (per-particle bond adjacency, no atomics)
 //   integrator       k_step_fused (thermostat + integrate2 + integrate1),
-//                    k_integrate1, k_thermo_integrate2, k_finish_max
-//   reductions       k_kinetic, k_reduce_partials
+//                    k_thermo_integrate2, k_finish_max
+//   observables      k_kinetic, k_rdf, k_reduce_partials
 //
 // Launch wrappers at the bottom are the only entry points used by the host
 // runtime (sim.hip).
@@ -417,6 +420,22 @@ __device__ bool is_excluded(int a, int b, const int* __restrict__ ex_start,
   return false;
 }
 
+// The workgroup's maximum of v into *dst with one atomic per workgroup (every
+// thread calls it): same-address atomics from each wavefront of a large grid
+// would queue behind each other.
+__device__ void block_max_to(int v, int* dst) {
+  __shared__ int wmax[16];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = wmax[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = max(m, wmax[w]);
+    atomicMax(dst, m);
+  }
+}
+
 // Full (both-direction) Verlet list.  Membership reproduces the reference's
 // half-list test exactly (src/VerletList.cpp:247-276): a pair is kept iff
 // d.sqr() <= (rc+skin)^2 in FP64 with no contraction, where d is evaluated in
@@ -486,10 +505,7 @@ __global__ __launch_bounds__(256) void k_nbr_build(int n, const double4* __restr
     cnt[i] = k;
     if (k > cap) ctl->overflow = 1;
   }
-  int m = k;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(&ctl->maxcount, m);
+  block_max_to(k, &ctl->maxcount);
 }
 
 // ------------------------------------------------------------------ LJ forces
@@ -1105,10 +1121,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     pred[i] = k;
     lane_slot[L] = i;
   }
-  int m = k;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(&ctl->maxcount, m);
+  block_max_to(k, &ctl->maxcount);
 }
 
 // ------------------------------------------------------------------ step helpers
@@ -1855,43 +1868,6 @@ __global__ __launch_bounds__(256) void k_thermo_integrate2(
   }
 }
 
-// VelocityVerlet::integrate1 (src/integrator/VelocityVerlet.cpp:236-287): half
-// kick, drift, global max |dx|^2 (atomicMax on the bits of a non-negative
-// double), and the fixed-point mirror for the mixed kernels.
-__global__ __launch_bounds__(256) void k_integrate1(
-    int n, double4* __restrict__ pos, double4* __restrict__ vel, const double* __restrict__ fx,
-    const double* __restrict__ fy, const double* __restrict__ fz, double dt,
-    uint4* __restrict__ posq, Geom g, Ctl* ctl) {
-#pragma clang fp contract(off)
-  __shared__ double red[4];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double sq = 0.0;
-  if (i < n) {
-    double4 v = vel[i];
-    double4 p = pos[i];
-    const double dtfm = 0.5 * dt / v.w;
-    v.x += dtfm * fx[i];
-    v.y += dtfm * fy[i];
-    v.z += dtfm * fz[i];
-    const double d0 = v.x * dt, d1 = v.y * dt, d2 = v.z * dt;
-    p.x += d0; p.y += d1; p.z += d2;
-    sq = d0 * d0 + d1 * d1 + d2 * d2;
-    vel[i] = v;
-    pos[i] = p;
-    if (posq)
-      posq[i] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]),
-                           (uint32_t)(int)p.w);
-  }
-  sq = wave_max(sq);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
-    atomicMax(&ctl->maxsq_bits, (unsigned long long)__double_as_longlong(m));
-  }
-}
-
 // ------------------------------------------------------------------ fused step
 //
 // Thermostat and integrate2 of step s-1 followed by integrate1 of step s on the
@@ -2238,12 +2214,6 @@ void launch_thermo_integrate2(hipStream_t s, int n, const int* id, double4* vel,
                        pref2, seed, call, excl, nid, do_int2, half_dt);
 }
 
-void launch_integrate1(hipStream_t s, int n, double4* pos, double4* vel, const double* fx,
-                       const double* fy, const double* fz, double dt, uint4* posq, const Geom& g,
-                       Ctl* ctl) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_integrate1, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, vel, fx, fy, fz, dt, posq, g, ctl);
-}
 
 int kinetic_blocks(int n) { return nblocks(n, 256); }
 
