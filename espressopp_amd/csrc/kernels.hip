@@ -806,19 +806,6 @@ struct RunRef {
   int slot_start, lds_off;
 };
 
-__device__ __forceinline__ int load_run_refs(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
-                                             RunRef* rr, int t) {
-  const int nr = nruns[t];
-  if ((int)threadIdx.x < nr) {
-    const TileRun R = runs[(size_t)t * kMaxRuns + threadIdx.x];
-    rr[threadIdx.x] = RunRef{R.slot_start, R.lds_off};
-    if ((int)threadIdx.x == nr - 1) rr[nr] = RunRef{0, R.lds_off + R.count};
-  }
-  if (nr == 0 && threadIdx.x == 0) rr[0] = RunRef{0, 0};
-  __syncthreads();
-  return nr;
-}
-
 // slot of LDS entry e (runs sorted by lds_off)
 __device__ __forceinline__ int slot_of_lds(const RunRef* rr, int nr, int e) {
   int lo = 0, hi = nr - 1;
@@ -829,31 +816,115 @@ __device__ __forceinline__ int slot_of_lds(const RunRef* rr, int nr, int e) {
   return rr[lo].slot_start + (e - rr[lo].lds_off);
 }
 
-template <typename V, typename S, typename Conv>
-__device__ __forceinline__ int stage_region(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
-                                            const V* __restrict__ src, S* lds, RunRef* rr, int t, Conv conv) {
-  const int nr = load_run_refs(runs, nruns, rr, t);
-  const int total = rr[nr].lds_off;
+// Stages tile t's region into LDS (entry e <- src of its slot, through conv) and
+// leaves the run table in rr for slot_of_lds.  Every wavefront holds the table
+// in its lanes (run k in lane k, kMaxRuns == 64) and stages 64 consecutive
+// entries at a time; the run of each entry follows from two ballots over the
+// runs' LDS offsets and scalar reads of the runs that start inside the 64, so
+// the loads of all of a thread's entries go out back to back, with no LDS
+// round trip before them.
+// A tile's run table as each wavefront holds it: run k (slot_start, count,
+// lds_off) in lane k.  The plan fills the first nr entries of the tile's table;
+// the rest are loaded but never used.
+struct RunTable {
+  int4 R;
+  int nr;
+};
+
+__device__ __forceinline__ RunTable load_run_table(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
+                                                   int t) {
+  static_assert(kMaxRuns == 64, "one run per lane");
+  RunTable rt;
+  rt.R = *reinterpret_cast<const int4*>(runs + (size_t)t * kMaxRuns + ((int)threadIdx.x & 63));
+  rt.nr = nruns[t];
+  return rt;
+}
+
+// slot - entry for entry e = e0 + lane of a wavefront: the run of e0 is the last
+// one starting at or before it (off holds run lane's LDS offset, rising with the
+// lane, with the region size past the table); later runs starting inside the
+// 64 entries take over from their first entry on.  Up to three of them are
+// resolved without a loop.
+__device__ __forceinline__ int stage_delta(int e0, int e, int off, int delta) {
+  const int r0 = __popcll(__ballot(off <= e0)) - 1;
+  int d = __builtin_amdgcn_readlane(delta, r0);
+#pragma unroll
+  for (int b = 1; b <= 3; ++b) {
+    const int k = min(r0 + b, 63);
+    if (e >= __builtin_amdgcn_readlane(off, k)) d = __builtin_amdgcn_readlane(delta, k);
+  }
+  const int k4 = min(r0 + 4, 63);
+  if (k4 > r0 + 3 || __builtin_amdgcn_readlane(off, k4) > e0 + 63) return d;
+  // more than three runs start inside the 64 entries (short runs)
+  unsigned long long inside = __ballot(off > e0 && off <= e0 + 63);
+  while (inside) {
+    const int k = __ffsll((long long)inside) - 1;
+    inside &= inside - 1ull;
+    if (e >= __builtin_amdgcn_readlane(off, k)) d = __builtin_amdgcn_readlane(delta, k);
+  }
+  return d;
+}
+
+// One pass of stage_region over entries [base, base + kB * blockDim.x): source
+// slots first (no memory access), then all loads, then all stores.  The loads
+// are unconditional (entries past the region read slot 0 and are dropped), so
+// the waits before the stores can count them; after_loads() issues the
+// caller's own loads behind them.
+template <int kB, typename V, typename S, typename Conv, typename After>
+__device__ __forceinline__ void stage_pass(int base, int total, int off, int delta, const V* __restrict__ src,
+                                           S* lds, Conv conv, After after_loads) {
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int bs = blockDim.x;
-  // eight independent loads in flight per thread, then eight LDS stores; a
-  // thread's entries rise by bs, so its run index only ever moves forward
-  int r = 0;
-  for (int base = 0; base < total; base += 8 * bs) {
-    V v[8];
+  int js[kB];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int e = base + u * bs + (int)threadIdx.x;
-      if (e < total) {
-        while (rr[r + 1].lds_off <= e) ++r;
-        v[u] = src[rr[r].slot_start + (e - rr[r].lds_off)];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int e = base + u * bs + (int)threadIdx.x;
-      if (e < total) lds[e] = conv(v[u]);
+  for (int u = 0; u < kB; ++u) {
+    const int e0 = base + u * bs + wv * 64;                         // first entry of the wavefront
+    const int e = e0 + lane;
+    js[u] = 0;
+    if (e0 < total) {
+      const int d = stage_delta(e0, e, off, delta);
+      if (e < total) js[u] = e + d;
     }
   }
+  V v[kB];
+#pragma unroll
+  for (int u = 0; u < kB; ++u) v[u] = src[js[u]];
+  after_loads();
+#pragma unroll
+  for (int u = 0; u < kB; ++u) {
+    const int e = base + u * bs + wv * 64 + lane;
+    if (e < total) lds[e] = conv(v[u]);
+  }
+}
+
+// Stages a tile's region into LDS (entry e <- src of its slot, through conv) and
+// leaves the run table in rr for slot_of_lds.  Every wavefront holds the table
+// in its lanes and stages 64 consecutive entries at a time, finding their runs
+// from ballots and scalar reads, so the loads of all of a thread's entries go
+// out back to back with no LDS round trip before them.  after_loads() runs once,
+// after the first pass's loads are issued.
+template <typename V, typename S, typename Conv, typename After>
+__device__ __forceinline__ int stage_region(const RunTable& rt, const V* __restrict__ src, S* lds, RunRef* rr,
+                                            Conv conv, After after_loads) {
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int4 R = rt.R;
+  const int nr = rt.nr;
+  // region size: end of the last run
+  const int total = nr > 0 ? __builtin_amdgcn_readlane(R.z + R.y, nr - 1) : 0;
+  // lanes past the table start at the region's end, so the ballots never take them
+  const int off = lane < nr ? R.z : total;
+  const int delta = lane < nr ? R.x - R.z : 0;                      // slot = entry + delta
+  if (wv == 0) {
+    if (lane < nr) rr[lane] = RunRef{R.x, R.z};
+    if (lane == 0) rr[nr] = RunRef{0, total};
+  }
+  constexpr int kB = sizeof(V) <= 16 ? 16 : 8;                     // entries in flight per thread
+  const int step = kB * (int)blockDim.x;
+  stage_pass<kB>(0, total, off, delta, src, lds, conv, after_loads);
+  for (int base = step; base < total; base += step)
+    stage_pass<kB>(base, total, off, delta, src, lds, conv, [] {});
   __syncthreads();
   return nr;
 }
@@ -861,7 +932,7 @@ __device__ __forceinline__ int stage_region(const TileRun* __restrict__ runs, co
 template <typename V>
 __device__ __forceinline__ int stage_tile(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
                                           const V* __restrict__ src, V* lds, RunRef* rr, int t) {
-  return stage_region(runs, nruns, src, lds, rr, t, [](const V& v) { return v; });
+  return stage_region(load_run_table(runs, nruns, t), src, lds, rr, [](const V& v) { return v; }, [] {});
 }
 
 template <typename S>
@@ -872,10 +943,16 @@ template <>
 __device__ __forceinline__ uint3 narrow_slot<uint3>(const uint4& v) { return make_uint3(v.x, v.y, v.z); }
 
 // stage_tile for the fixed-point mirror, storing S (uint3 or uint4) per particle
+template <typename S, typename After>
+__device__ __forceinline__ int stage_tile_q(const RunTable& rt, const uint4* __restrict__ src, S* lds, RunRef* rr,
+                                            After after_loads) {
+  return stage_region(rt, src, lds, rr, [](const uint4& v) { return narrow_slot<S>(v); }, after_loads);
+}
+
 template <typename S>
 __device__ __forceinline__ int stage_tile_q(const TileRun* __restrict__ runs, const int* __restrict__ nruns,
                                             const uint4* __restrict__ src, S* lds, RunRef* rr, int t) {
-  return stage_region(runs, nruns, src, lds, rr, t, [](const uint4& v) { return narrow_slot<S>(v); });
+  return stage_tile_q<S>(load_run_table(runs, nruns, t), src, lds, rr, [] {});
 }
 
 // d.sqr() <= cutsq of Real3D (src/Real3D.hpp), evaluated left to right without contraction
@@ -1226,19 +1303,38 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
   Slot* lds = reinterpret_cast<Slot*>(lds_raw);
   __shared__ RunRef sr[kMaxRuns + 1];
   const int tile = xcd_tile(blockIdx.x, ntiles);
-  const int nr = stage_tile_q<Slot>(runs, nruns, posq, lds, sr, tile);
+  const RunTable rt = load_run_table(runs, nruns, tile);
   const int L = tile * blockDim.x + threadIdx.x;   // list lane
+  const bool live = L < n;
+  // kGroups 16-byte groups (8 entries each) per iteration, the next ones loaded
+  // during it; with several types one group, so the per-entry type-pair
+  // coefficients of a batch stay within the register budget
+  constexpr int kGroups = ONETYPE ? 2 : 1;
+  constexpr int kBatch = 8 * kGroups;              // entries per iteration
+  // The lane's own data and its first list groups do not depend on the staged
+  // region: their loads go out before the staging, unconditionally (lanes past
+  // n read lane n-1's and drop it), so that later waits can count them.
+  const int Lc = live ? L : n - 1;
+  const int i = lane_slot[Lc];                     // particle slot
+  const int clist = cnt[Lc];                       // list length
+  const uint4* np = nbr + tiled_group(Lc, capg, 0);
+  uint4 wn[kGroups];
+#pragma unroll
+  for (int q = 0; q < kGroups; ++q) wn[q] = np[(size_t)min(q, capg - 1) * 64];
+  // the particle's own mirror, velocity and position are loaded behind the
+  // staging loads, so the staging does not wait for them
+  uint4 qi;
+  double4 step_v, step_p;
+  const int nr = stage_tile_q<Slot>(rt, posq, lds, sr, [&] {
+    qi = posq[i];
+    if (STEP) { step_v = sa.vel[i]; step_p = sa.pos[i]; }
+  });
   double stepsq = 0.0;                             // |dx|^2 of the drift (STEP)
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
-  if (L < n) {
-    const int i = lane_slot[L];                      // particle slot
-    const uint4 qi = posq[i];
-    double4 step_v, step_p;
-    if (STEP) { step_v = sa.vel[i]; step_p = sa.pos[i]; }
-    const uint4* np = nbr + tiled_group(L, capg, 0);
-    const int c = nr > 0 ? min(cnt[L], 8 * capg) : 0;
+  if (live) {
+    const int c = nr > 0 ? min(clist, 8 * capg) : 0;
     const int rowi = (int)qi.w * ntypes;
     const LJParam& p0 = prm[0];
     double ff1 = CUBIC ? p0.uff1 : p0.dff1, ff2 = CUBIC ? p0.uff2 : p0.dff2;
@@ -1247,27 +1343,36 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     // FP64 positions; r^2 above hi is outside
     double lo = CUBIC ? p0.ulo : p0.dlo, hi = CUBIC ? p0.uhi : p0.dhi;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
-    // kGroups 16-byte groups (8 entries each) per iteration, the next ones loaded
-    // during it; with several types one group, so the per-entry type-pair
-    // coefficients of a batch stay within the register budget
-    constexpr int kGroups = ONETYPE ? 2 : 1;
     const int ng = (c + 7) >> 3;
-    uint4 wn[kGroups];
+    const int nit = (ng + kGroups - 1) / kGroups;    // iterations
 #pragma unroll
-    for (int q = 0; q < kGroups; ++q) wn[q] = q < ng ? np[(size_t)q * 64] : make_uint4(0u, 0u, 0u, 0u);
-    for (int kg = 0; kg < ng; kg += kGroups) {
+    for (int q = 0; q < kGroups; ++q)
+      if (q >= ng) wn[q] = make_uint4(0u, 0u, 0u, 0u);
+    // One iteration over entries [kBatch * it, kBatch * it + kBatch).  Only the
+    // last iteration can hold entries past the list length c (zero offsets, which
+    // stage a valid slot), so only CHECKED tests each entry against c.  The
+    // positions of all the batch's entries are read from LDS before the first is
+    // used, so the gathers overlap the arithmetic of the batch.
+    auto batch = [&](int it, auto checked) {
+      constexpr bool CHECKED = decltype(checked)::value;
+      const int kg = it * kGroups;
       uint32_t w[4 * kGroups];
 #pragma unroll
       for (int q = 0; q < kGroups; ++q) {
         w[4 * q] = wn[q].x; w[4 * q + 1] = wn[q].y; w[4 * q + 2] = wn[q].z; w[4 * q + 3] = wn[q].w;
         wn[q] = (kg + kGroups + q < ng) ? np[(size_t)(kg + kGroups + q) * 64] : make_uint4(0u, 0u, 0u, 0u);
       }
+      Slot qa[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
+        qa[u] = lds[e];
+      }
       bool anynear = false;
 #pragma unroll
-      for (int u = 0; u < 8 * kGroups; ++u) {
-        const bool valid = 8 * kg + u < c;
-        const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
-        const Slot qj = lds[e];
+      for (int u = 0; u < kBatch; ++u) {
+        const bool valid = !CHECKED || kBatch * it + u < c;
+        const Slot qj = qa[u];
         if (!ONETYPE) {
           int tj = 0;
           if constexpr (!ONETYPE) tj = (int)qj.w;
@@ -1332,7 +1437,9 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
           }
         }
       }
-    }
+    };
+    for (int it = 0; it + 1 < nit; ++it) batch(it, std::false_type{});
+    if (nit > 0) batch(nit - 1, std::true_type{});
     if (CUBIC) {
       // back from fixed-point units: F = d_units * s * ff_units; w = r2_units * s^2 * ff_units
       Ax *= g.qsd[0]; Ay *= g.qsd[0]; Az *= g.qsd[0];
