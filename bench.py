@@ -226,6 +226,10 @@ def main():
     tile, region = ctypes.c_int32(0), ctypes.c_int32(0)
     espressopp._lib.check(h.lib.espp_tile_stats(h.h, ctypes.byref(tile), ctypes.byref(region)))
     log("tile plan: %d particles per tile, largest staged region %d particles" % (tile.value, region.value))
+    lb, wb = ctypes.c_int64(0), ctypes.c_int64(0)
+    espressopp._lib.check(h.lib.espp_vl_batch_stats(h.h, vl._id, 16, ctypes.byref(lb), ctypes.byref(wb)))
+    log("16-entry batches of the pair loop: %.3f per lane, %.3f per wavefront (its longest lane)" %
+        (lb.value / n, wb.value * 64.0 / n))
     log("mixed: %.3f ms/step, %.4g particle-steps/s, pair+step kernel %.3f ms avg (%d launches), nbar %.2f, "
         "cap %d, resorts %d" % (dt_run / args.steps * 1e3, value, avg_s * 1e3, launches, nbar, cap, resorts))
     for k, (c, ms) in stats.items():

@@ -75,6 +75,7 @@ PROF_SIGNATURES = {
     "espp_prof_reset": [_vp],
     "espp_prof_stats": [_vp, _c_int, _P(_c_i64), _P(_c_dbl)],
     "espp_vl_stats": [_vp, _c_int, _P(_c_i64), _P(_c_int), _P(_c_int)],
+    "espp_vl_batch_stats": [_vp, _c_int, _c_int, _P(_c_i64), _P(_c_i64)],
     "espp_tile_stats": [_vp, _P(_c_int), _P(_c_int)],
 }
 KERNEL = {"pair": 0, "nbr_build": 1, "integrate1": 2, "integrate2": 3, "binning": 4, "bond": 5, "pair_step": 6}

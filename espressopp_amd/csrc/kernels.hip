@@ -17,6 +17,7 @@
 //
 // Launch wrappers at the bottom are the only entry points used by the host
 // runtime (sim.hip).
+#include <climits>
 #include <type_traits>
 
 #include "espp_dev.h"
@@ -842,7 +843,7 @@ __device__ __forceinline__ RunTable load_run_table(const TileRun* __restrict__ r
 
 // slot - entry for entry e = e0 + lane of a wavefront: the run of e0 is the last
 // one starting at or before it (off holds run lane's LDS offset, rising with the
-// lane, with the region size past the table); later runs starting inside the
+// lane, and INT_MAX past the table); later runs starting inside the
 // 64 entries take over from their first entry on.  Up to three of them are
 // resolved without a loop.
 __device__ __forceinline__ int stage_delta(int e0, int e, int off, int delta) {
@@ -853,8 +854,8 @@ __device__ __forceinline__ int stage_delta(int e0, int e, int off, int delta) {
     const int k = min(r0 + b, 63);
     if (e >= __builtin_amdgcn_readlane(off, k)) d = __builtin_amdgcn_readlane(delta, k);
   }
-  const int k4 = min(r0 + 4, 63);
-  if (k4 > r0 + 3 || __builtin_amdgcn_readlane(off, k4) > e0 + 63) return d;
+  // done unless a fourth run starts inside the 64 entries
+  if (r0 + 4 > 63 || __builtin_amdgcn_readlane(off, r0 + 4) > e0 + 63) return d;
   // more than three runs start inside the 64 entries (short runs)
   unsigned long long inside = __ballot(off > e0 && off <= e0 + 63);
   while (inside) {
@@ -913,8 +914,8 @@ __device__ __forceinline__ int stage_region(const RunTable& rt, const V* __restr
   const int nr = rt.nr;
   // region size: end of the last run
   const int total = nr > 0 ? __builtin_amdgcn_readlane(R.z + R.y, nr - 1) : 0;
-  // lanes past the table start at the region's end, so the ballots never take them
-  const int off = lane < nr ? R.z : total;
+  // lanes past the table start beyond every entry, so no ballot or test takes them
+  const int off = lane < nr ? R.z : INT_MAX;
   const int delta = lane < nr ? R.x - R.z : 0;                      // slot = entry + delta
   if (wv == 0) {
     if (lane < nr) rr[lane] = RunRef{R.x, R.z};

@@ -1846,4 +1846,26 @@ int espp_vl_stats(espp_sim* s, int32_t vl, int64_t* entries, int32_t* cap, int32
   });
 }
 
+int espp_vl_batch_stats(espp_sim* s, int32_t vl, int32_t batch, int64_t* lane_batches, int64_t* wave_batches) {
+  return guarded([&] {
+    VerletListState& v = get_vl(s, vl);
+    if (batch < 1) fail(ESPP_ERR_ARG, "batch must be positive");
+    s->ensure_lists();
+    s->sync();
+    std::vector<int> c = download(v.cnt, (size_t)s->n, s->stream);
+    int64_t lanes = 0, waves = 0;
+    for (size_t w0 = 0; w0 < c.size(); w0 += 64) {
+      int64_t mx = 0;
+      for (size_t l = w0; l < std::min(c.size(), w0 + 64); ++l) {
+        const int64_t b = (c[l] + batch - 1) / batch;
+        lanes += b;
+        mx = std::max(mx, b);
+      }
+      waves += mx;
+    }
+    if (lane_batches) *lane_batches = lanes;
+    if (wave_batches) *wave_batches = waves;
+  });
+}
+
 }  // extern "C"

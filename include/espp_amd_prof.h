@@ -46,6 +46,12 @@ int espp_prof_stats(espp_sim *sim, int32_t kernel, int64_t *launches, double *to
  * the padded row capacity and the largest row of the last build. */
 int espp_vl_stats(espp_sim *sim, int32_t vl, int64_t *entries, int32_t *cap, int32_t *maxcount);
 
+/* Batches of `batch` list entries in the list's lane order (lanes of a tile
+ * ordered as the pair kernels take them): summed over lanes, and summed over
+ * wavefronts of 64 consecutive lanes with each wavefront counted at its longest
+ * lane, which is what a wavefront runs. */
+int espp_vl_batch_stats(espp_sim *sim, int32_t vl, int32_t batch, int64_t *lane_batches, int64_t *wave_batches);
+
 /* Particles per tile of the current tile plan (0: untiled kernels) and the
  * largest staged region in particles. */
 int espp_tile_stats(espp_sim *sim, int32_t *tile, int32_t *max_region);

@@ -136,6 +136,41 @@ def test_forces_mixed(gpu, n, seed):
     assert ia.computeVirial() == pytest.approx(ref.virial(ria)[0], rel=1e-6)
 
 
+def sparse_positions(n, box, min_dist, seed):
+    """n points in the box, each at least min_dist from the others (periodic), by rejection."""
+    rng = np.random.default_rng(seed)
+    L = np.asarray(box)
+    pos = np.zeros((0, 3))
+    while len(pos) < n:
+        p = rng.uniform(0, 1, 3) * L
+        d = pos - p
+        d -= L * np.round(d / L)
+        if len(pos) == 0 or np.min(np.einsum("ij,ij->i", d, d)) >= min_dist * min_dist:
+            pos = np.vstack([pos, p])
+    return pos
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_forces_sparse_box(gpu, precision):
+    # A dilute, elongated box: a cell holds about one particle, so a tile's staged
+    # region is made of many short runs, several of them starting within one
+    # wavefront's 64 entries.
+    box = (20.0, 7.5, 33.0)
+    pos = sparse_positions(1200, box, 0.9, 21)
+    rc, skin = 1.5, 0.2
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells_for(box, rc, skin), rc, skin, precision)
+    assert pair_set(vl.pairs_array()) == pair_set(ref.vl_pairs(rvl))
+    espressopp.integrator.VelocityVerlet(system).run(0)
+    ref.calc_forces()
+    f = system.storage.getParticleArrays()["f"]
+    if precision == "fp64":
+        assert force_rel_err(f, ref.get("f"), 1e-3).max() < 1e-12
+        assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=1e-12)
+    else:
+        assert force_rel_err(f, ref.get("f"), 0.1).max() <= 1e-5
+        assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=1e-6)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_multitype_forces(gpu, precision):
     pid, pos, _, box = read_fixture()
