@@ -12,7 +12,6 @@ int scan_blocks(int ncell);
 // blockmax of a needs nblocks(n, 256) entries
 void launch_step_fused(hipStream_t s, int n, double* fx, double* fy, double* fz, const StepArgs& a, const Geom& g,
                        Ctl* ctl, unsigned long long* host_out);
-void launch_permute_int(hipStream_t s, int n, const int* order, const int* src, int* dst);
 void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start,
                     int* order);
 void launch_sort_cells(hipStream_t s, int ncell, const int* start, int* order, const double4* pos, const Ctl* ctl);
@@ -39,7 +38,7 @@ void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Ge
 void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
-                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot,
+                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* lane_slot,
                             float clip_margin, Ctl* ctl);
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,

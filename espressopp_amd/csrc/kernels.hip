@@ -265,13 +265,6 @@ __global__ __launch_bounds__(1024) void k_scan_down(int ncell, const int* __rest
 }
 
 
-// dst[s] = src[order[s]]: a per-particle int array into the new slot order
-__global__ void k_permute_int(int n, const int* __restrict__ order, const int* __restrict__ src,
-                              int* __restrict__ dst) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n) dst[s] = src[order[s]];
-}
-
 __global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __restrict__ rank,
                           const int* __restrict__ start, int* __restrict__ order) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1043,25 +1036,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g, double cutsq, float cut_lo,
     float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,
     const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint4* __restrict__ nbr,
-    int* __restrict__ cnt, int capg, int* __restrict__ pred, int* __restrict__ lane_slot, float clip_margin,
+    int* __restrict__ cnt, int capg, int* __restrict__ lane_slot, float clip_margin,
     Ctl* ctl) {
   __shared__ TileRun sr[kMaxRuns];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = nruns[tile];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)tile * kMaxRuns + threadIdx.x];
-  // Lanes take the tile's particles in order of their list length at the last
-  // build (carried with the particles through binning), rounded up to the pair
-  // kernels' 16-entry iterations, and in slot order within one length class.
-  // The lanes of a wavefront then run the same number of iterations, and
-  // neighbouring particles, whose lists are alike, share a wavefront, so their
-  // gathers of the staged positions fall on nearby LDS addresses.
+  __syncthreads();
+  // threads in slot order (lanes of one cell read the same candidates); each
+  // particle's list is stored at its slot
   const int s0 = tile * blockDim.x;
-  const int cls = (s0 + (int)threadIdx.x < n) ? min((pred[s0 + threadIdx.x] + 15) >> 4, 62) : 63;
-  const int rank = block_class_rank(cls);
-  // threads stay in slot order (lanes of one cell read the same candidates);
-  // each particle's list goes to the lane of its rank
   const int i = s0 + threadIdx.x;        // particle slot
-  const int L = s0 + rank;               // list lane (slots past n rank last, with lanes past n)
   int k = 0;
   if (i < n && nr > 0) {
     const double4 pi = pos[i];
@@ -1069,7 +1054,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
     const int myid = HAS_EXCL ? id[i] : 0;
-    uint4* out = nbr + tiled_group(L, capg, 0);
+    uint4* out = nbr + tiled_group(i, capg, 0);
     const int cap = 8 * capg;
     // entries not yet stored: (w3:w2:w1:w0) is a 128-bit shift register of 16-bit
     // entries, each new entry entering at the top; after 8 entries it holds the
@@ -1190,15 +1175,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       }
       out[(size_t)(k >> 3) * 64] = make_uint4(w0, w1, w2, w3);
     }
-    cnt[L] = k;
+    cnt[i] = k;
     if (k > cap) ctl->overflow = 1;
   } else if (i < n) {
-    cnt[L] = 0;
+    cnt[i] = 0;
   }
-  if (i < n) {
-    pred[i] = k;
-    lane_slot[L] = i;
-  }
+  // The pair kernels take the tile's particles in order of this list length,
+  // rounded up to their 16-entry iterations, and in slot order within one
+  // length class (lane_slot[s0 + r] is the slot of rank r).  The lanes of a
+  // wavefront then run the same number of iterations, and neighbouring
+  // particles, whose lists are alike, share a wavefront, so their gathers of
+  // the staged positions fall on nearby LDS addresses.  Slots past n rank last.
+  const int rank = block_class_rank(i < n ? min((k + 15) >> 4, 62) : 63);
+  if (i < n) lane_slot[s0 + rank] = i;
   block_max_to(k, &ctl->maxcount);
 }
 
@@ -1316,9 +1305,9 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
   // region: their loads go out before the staging, unconditionally (lanes past
   // n read lane n-1's and drop it), so that later waits can count them.
   const int Lc = live ? L : n - 1;
-  const int i = lane_slot[Lc];                     // particle slot
-  const int clist = cnt[Lc];                       // list length
-  const uint4* np = nbr + tiled_group(Lc, capg, 0);
+  const int i = lane_slot[Lc];                     // particle slot, which also indexes its list
+  const int clist = cnt[i];                        // list length
+  const uint4* np = nbr + tiled_group(i, capg, 0);
   uint4 wn[kGroups];
 #pragma unroll
   for (int q = 0; q < kGroups; ++q) wn[q] = np[(size_t)min(q, capg - 1) * 64];
@@ -1486,8 +1475,8 @@ __global__ __launch_bounds__(256) void k_lj_tiled_fp64(
   if (L < n) {
     const int i = lane_slot[L];                      // particle slot
     const double4 pi = pos[i];
-    const uint4* np = nbr + tiled_group(L, capg, 0);
-    const int c = nr > 0 ? min(cnt[L], 8 * capg) : 0;
+    const uint4* np = nbr + tiled_group(i, capg, 0);
+    const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
     const int rowi = (int)pi.w * ntypes;
     double ff1 = prm[0].dff1, ff2 = prm[0].dff2, rc2 = prm[0].drc2, shift = prm[0].dshift;
     double sig2 = prm[0].sig2, eps4 = prm[0].eps4;
@@ -1647,8 +1636,8 @@ __global__ __launch_bounds__(256) void k_tab_tiled(
     const int i = lane_slot[L];                      // particle slot
     const double4 pi = pos[i];
     const uint4 qi = MIXED ? posq[i] : make_uint4(0u, 0u, 0u, 0u);
-    const uint32_t* np = reinterpret_cast<const uint32_t*>(nbr + tiled_group(L, capg, 0));
-    const int c = nr > 0 ? min(cnt[L], 8 * capg) : 0;
+    const uint32_t* np = reinterpret_cast<const uint32_t*>(nbr + tiled_group(i, capg, 0));
+    const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
     const int rowi = (int)pi.w * ntypes;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     for (int k = 0; k < c; ++k) {
@@ -2087,9 +2076,6 @@ void launch_step_fused(hipStream_t s, int n, double* fx, double* fy, double* fz,
   hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, a.blockmax, ctl, host_out);
 }
 
-void launch_permute_int(hipStream_t s, int n, const int* order, const int* src, int* dst) {
-  if (n > 0) hipLaunchKernelGGL(k_permute_int, dim3(nblocks(n, 256)), dim3(256), 0, s, n, order, src, dst);
-}
 
 void launch_scatter(hipStream_t s, int n, const int* cell_of, const int* rank, const int* start, int* order) {
   if (n > 0) hipLaunchKernelGGL(k_scatter, dim3(nblocks(n, 256)), dim3(256), 0, s, n, cell_of, rank, start, order);
@@ -2175,12 +2161,12 @@ static bool cubic_box(const Geom& g) { return g.qsd[0] == g.qsd[1] && g.qsd[1] =
 void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
-                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* pred, int* lane_slot,
+                            const int* ex_ids, uint4* nbr, int* cnt, int capg, int* lane_slot,
                             float clip_margin, Ctl* ctl) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
-#define NB(EX) hipLaunchKernelGGL((k_nbr_build_tiled<EX>), grid, block, 0, s, n, ntiles, posf, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, pred, lane_slot, clip_margin, ctl)
+#define NB(EX) hipLaunchKernelGGL((k_nbr_build_tiled<EX>), grid, block, 0, s, n, ntiles, posf, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, lane_slot, clip_margin, ctl)
   if (ex_start) NB(true);
   else NB(false);
 #undef NB

@@ -90,10 +90,8 @@ struct VerletListState {
   double cut = 0.0;
   int cap = 0;          // entries per particle row (untiled) or 2 x words per row (tiled)
   bool tiled = false;   // entries are 16-bit offsets into the tile's staged region
-  DBuf<int> nbr, cnt;   // tiled: indexed by list lane, lane_slot maps a lane to its particle slot
-  DBuf<int> lane_slot;
-  DBuf<int> pred, pred2;   // list length of each slot's particle at the last build (lane order of the next)
-  int64_t pred_n = -1;     // particle count pred is valid for
+  DBuf<int> nbr, cnt;   // indexed by particle slot
+  DBuf<int> lane_slot;  // tiled: the slot each pair-kernel lane works on (lanes ordered by list length)
   int64_t builds = 0;
   int maxcount = 0;
   bool stale = true;   // slot order changed since the last build
@@ -483,13 +481,6 @@ struct espp_sim {
       pos.swap(pos2); vel.swap(vel2);
       if (carry_forces) { fx.swap(fx2); fy.swap(fy2); fz.swap(fz2); }
       id.swap(id2); img.swap(img2);
-      for (auto* v : vls) {
-        if (v->pred_n == n && v->pred.n >= (size_t)n) {
-          v->pred2.resize((size_t)n);
-          launch_permute_int(stream, ni, order.p, v->pred.p, v->pred2.p);
-          v->pred.swap(v->pred2);
-        }
-      }
     }
     prof_end(4, pe);
     disp_since_bin = 0.0;
@@ -589,15 +580,9 @@ struct espp_sim {
         // 16-bit entries, eight per 16-byte group: cap / 2 ints per particle
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)(v.cap / 2) * 64);
         v.lane_slot.resize((size_t)std::max<int64_t>(n, 1));
-        if (v.pred_n != n) {
-          // no list lengths yet for this particle set: lanes in slot order
-          v.pred.resize((size_t)std::max<int64_t>(n, 1));
-          launch_fill_int(stream, (int)n, v.pred.p, 0);
-          v.pred_n = n;
-        }
         launch_nbr_build_tiled(stream, (int)n, T, posf.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
                                cutsq, cut_lo_q, cut_hi_q, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
-                               v.has_excl ? v.ex_ids.p : nullptr, (uint4*)v.nbr.p, v.cnt.p, v.cap / 8, v.pred.p,
+                               v.has_excl ? v.ex_ids.p : nullptr, (uint4*)v.nbr.p, v.cnt.p, v.cap / 8,
                                v.lane_slot.p, clip_margin, ctl.p);
       } else {
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
@@ -1414,12 +1399,11 @@ int espp_vl_pairs(espp_sim* s, int32_t vl, int32_t* out, int64_t cap, int64_t* t
     const size_t rowlen = v.tiled ? (size_t)v.cap / 2 : (size_t)v.cap;
     std::vector<int> nb = download(v.nbr, (size_t)((n + 63) / 64) * rowlen * 64, s->stream);
     std::vector<TileRun> runs;
-    std::vector<int> nruns, lane_slot;
+    std::vector<int> nruns;
     if (v.tiled) {
       size_t ntiles = (n + (size_t)s->T - 1) / (size_t)s->T;
       runs = download(s->runs, ntiles * kMaxRuns, s->stream);
       nruns = download(s->nruns, ntiles, s->stream);
-      lane_slot = download(v.lane_slot, n, s->stream);
     }
     auto entry_slot = [&](size_t i, int q) -> int {
       if (!v.tiled) return nb[(i / 64) * rowlen * 64 + (size_t)q * 64 + (i % 64)];
@@ -1434,10 +1418,9 @@ int espp_vl_pairs(espp_sim* s, int32_t vl, int32_t* out, int64_t cap, int64_t* t
       return sr[r].slot_start + (e - sr[r].lds_off);
     };
     int64_t k = 0;
-    for (size_t l = 0; l < n; ++l) {         // list lane l holds the list of slot i
-      const size_t i = v.tiled ? (size_t)lane_slot[l] : l;
-      for (int q = 0; q < c[l]; ++q) {
-        int j = entry_slot(l, q);
+    for (size_t i = 0; i < n; ++i) {         // the list of slot i
+      for (int q = 0; q < c[i]; ++q) {
+        int j = entry_slot(i, q);
         if (ids[i] < ids[(size_t)j]) {
           if (k < cap) { out[2 * k] = ids[i]; out[2 * k + 1] = ids[(size_t)j]; }
           ++k;
@@ -1853,6 +1836,12 @@ int espp_vl_batch_stats(espp_sim* s, int32_t vl, int32_t batch, int64_t* lane_ba
     s->ensure_lists();
     s->sync();
     std::vector<int> c = download(v.cnt, (size_t)s->n, s->stream);
+    if (v.tiled) {
+      // the pair kernels take the lists in lane order: lane l works on slot lane_slot[l]
+      std::vector<int> ls = download(v.lane_slot, (size_t)s->n, s->stream), cl(c.size());
+      for (size_t l = 0; l < c.size(); ++l) cl[l] = c[(size_t)ls[l]];
+      c.swap(cl);
+    }
     int64_t lanes = 0, waves = 0;
     for (size_t w0 = 0; w0 < c.size(); w0 += 64) {
       int64_t mx = 0;
