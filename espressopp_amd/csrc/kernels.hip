@@ -1990,7 +1990,17 @@ __global__ __launch_bounds__(1024) void k_finish_max(int nblk, const double* __r
                                                      unsigned long long* host_out) {
   __shared__ double red[16];
   double m = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) m = fmax(m, blockmax[b]);
+  // sixteen independent loads in flight per thread, not one round trip per load
+  for (int b0 = 0; b0 < nblk; b0 += 16 * (int)blockDim.x) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int b = b0 + u * (int)blockDim.x + (int)threadIdx.x;
+      v[u] = b < nblk ? blockmax[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) m = fmax(m, v[u]);
+  }
   m = wave_max(m);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
