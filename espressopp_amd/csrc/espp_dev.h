@@ -80,6 +80,11 @@ struct StepArgs {
   double* blockmax;                // per-block max |dx|^2
 };
 
+// Field groups of Ctl that launch_reset_ctl clears: the binning's (max |dx|^2,
+// nanpos), the list build's (overflow, maxcount) and the tile plan's
+// (plan_fail, max_region).
+constexpr int kCtlBin = 1, kCtlBuild = 2, kCtlPlan = 4, kCtlAll = 7;
+
 // Per-call control block kept on the device.
 struct Ctl {
   unsigned long long maxsq_bits;   // max |dx|^2 of the last integrate1 (non-negative double bits)

@@ -22,7 +22,7 @@ void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, 
                     int* slot_of_id, uint4* posq, float4* posf, const Geom& g);
 void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf);
 void launch_fill_int(hipStream_t s, int n, int* a, int v);
-void launch_reset_ctl(hipStream_t s, Ctl* ctl);
+void launch_reset_ctl(hipStream_t s, Ctl* ctl, int fields = kCtlAll);
 void launch_nbr_build(hipStream_t s, int n, const double4* pos, const int* id,
                       const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                       const int* ex_start, const int* ex_ids, int* nbr, int* cnt, int cap,

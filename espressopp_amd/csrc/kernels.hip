@@ -389,14 +389,20 @@ __global__ void k_fill_int(int n, int* __restrict__ a, int v) {
   if (i < n) a[i] = v;
 }
 
-__global__ void k_reset_ctl(Ctl* ctl) {
+__global__ void k_reset_ctl(Ctl* ctl, int fields) {
   if (threadIdx.x == 0) {
-    ctl->maxsq_bits = 0ull;
-    ctl->overflow = 0;
-    ctl->maxcount = 0;
-    ctl->nanpos = 0;
-    ctl->plan_fail = 0;
-    ctl->max_region = 0;
+    if (fields & kCtlBin) {
+      ctl->maxsq_bits = 0ull;
+      ctl->nanpos = 0;
+    }
+    if (fields & kCtlBuild) {
+      ctl->overflow = 0;
+      ctl->maxcount = 0;
+    }
+    if (fields & kCtlPlan) {
+      ctl->plan_fail = 0;
+      ctl->max_region = 0;
+    }
   }
 }
 
@@ -450,6 +456,9 @@ __global__ __launch_bounds__(256) void k_nbr_build(int n, const double4* __restr
                                                    int* __restrict__ nbr, int* __restrict__ cnt,
                                                    int cap, Ctl* ctl) {
 #pragma clang fp contract(off)
+  // a lost particle: the binning put it in cell 0 and the run ends in an error;
+  // do not scan what may be one huge cell
+  if (ctl->nanpos) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int k = 0;
   if (i < n) {
@@ -680,6 +689,8 @@ constexpr int kPlanTilesPerBlock = 16;
 __global__ __launch_bounds__(64 * kPlanTilesPerBlock) void k_tile_plan(
     int n, int T, int ntiles, const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g,
     int budget, TileRun* __restrict__ runs, int* __restrict__ nruns, Ctl* ctl) {
+  // a lost particle: the run ends in an error (no plan is needed)
+  if (ctl->nanpos) return;
   __shared__ int run_a[kPlanTilesPerBlock][64], run_b[kPlanTilesPerBlock][64];
   __shared__ int region_max[kPlanTilesPerBlock];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1038,6 +1049,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint4* __restrict__ nbr,
     int* __restrict__ cnt, int capg, int* __restrict__ lane_slot, float clip_margin,
     Ctl* ctl) {
+  // a lost particle: the binning put it in cell 0 and the run ends in an error;
+  // do not scan what may be one huge cell.  A failed tile plan: the host plans
+  // smaller tiles and builds again.
+  if (ctl->nanpos || ctl->plan_fail) return;
   __shared__ TileRun sr[kMaxRuns];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = nruns[tile];
@@ -2115,8 +2130,8 @@ void launch_fill_int(hipStream_t s, int n, int* a, int v) {
   if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblocks(n, 256)), dim3(256), 0, s, n, a, v);
 }
 
-void launch_reset_ctl(hipStream_t s, Ctl* ctl) {
-  hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, s, ctl);
+void launch_reset_ctl(hipStream_t s, Ctl* ctl, int fields) {
+  hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, s, ctl, fields);
 }
 
 void launch_nbr_build(hipStream_t s, int n, const double4* pos, const int* id, const int* cell_of_slot,

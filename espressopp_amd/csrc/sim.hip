@@ -311,6 +311,13 @@ struct espp_sim {
   int lds_cap = 65536;          // dynamic+static LDS bytes one workgroup may use
   int T = 0;                    // particles per tile; 0 when no plan fits
   bool plan_valid = false;
+  bool plan_pending = false;    // a tentative plan was launched; its result is not read yet
+  int plan_tried = 0;           // tile size of the pending tentative plan
+  int plan_failed = 0;          // tile size whose plan failed for the current slot order
+  int plan_hint = 0;            // tile size of the last plan that fitted
+  int large_fail_streak = 0;    // tentative plans of the largest size that failed in a row
+  int64_t plan_count = 0;       // tentative plans launched
+  bool nan_pending = false;     // the binning's lost-particle flag is not read yet
   int max_region = 0;           // largest staged region (particles)
   DBuf<TileRun> runs;
   DBuf<int> nruns;
@@ -460,7 +467,9 @@ struct espp_sim {
 
   // Fold (optionally), bin into cells, sort each cell by id and permute every
   // per-particle array into cell order.
-  void bin(bool do_fold, bool carry_forces = true) {
+  // With defer_check the lost-particle flag is read with the list build's
+  // control block (one read per rebuild) instead of here.
+  void bin(bool do_fold, bool carry_forces = true, bool defer_check = false) {
     int ni = (int)n;
     hipEvent_t pe = prof_begin();
     launch_reset_ctl(stream, ctl.p);
@@ -487,12 +496,19 @@ struct espp_sim {
     plan_valid = false;
     for (auto* v : vls) v->stale = true;
     if (do_fold) coord_bound = std::max(g.L[0], std::max(g.L[1], g.L[2]));
-    {
-      read_ctl();
-      if (hctl->nanpos)
-        fail(ESPP_ERR_NAN, "particle has moved to outer space (one or more coordinates are nan or infinite)");
-    }
     sorted = true;
+    if (defer_check) {
+      nan_pending = true;
+    } else {
+      read_ctl();
+      check_nanpos();
+    }
+  }
+
+  void check_nanpos() {
+    nan_pending = false;
+    if (hctl->nanpos)
+      fail(ESPP_ERR_NAN, "particle has moved to outer space (one or more coordinates are nan or infinite)");
   }
 
   void ensure_sorted() {
@@ -501,11 +517,24 @@ struct espp_sim {
 
   // Tile plan for the current slot order: the largest tile size whose staged
   // regions fit the LDS budget; T = 0 selects the untiled kernels.
-  void ensure_plan() {
+  // With tentative, one tile size is launched without waiting for its result:
+  // the list build that follows (which returns at once when the plan failed)
+  // reads plan_fail and max_region with its own control block, and only when
+  // the plan failed are the other sizes searched, largest first.  The tentative
+  // size is the largest, or, after the largest failed twice in a row, the last
+  // size that fitted (the largest is tried again every 16th plan).
+  void ensure_plan(bool tentative = false) {
+    if (plan_valid && plan_pending && !tentative) {
+      // read the tentative plan's result now
+      read_ctl();
+      if (nan_pending) check_nanpos();
+      if (tentative_plan_done(!hctl->plan_fail)) return;
+    }
     if (plan_valid) return;
     T = 0;
     max_region = 0;
     plan_valid = true;
+    plan_pending = false;
     if (n == 0) return;
     // the tiled list build assumes at least 3 cells per dimension
     if (g.n[0] < 3 || g.n[1] < 3 || g.n[2] < 3) return;
@@ -514,20 +543,55 @@ struct espp_sim {
     const int sizes_mixed[3] = {256, 128, 64}, sizes_fp64[2] = {128, 64};
     const int* sizes = mixed() ? sizes_mixed : sizes_fp64;
     const int nsz = mixed() ? 3 : 2;
-    for (int q = 0; q < nsz; ++q) {
-      const int tt = sizes[q];
+    auto launch_plan = [&](int tt) {
       const int ntiles = (int)((n + tt - 1) / tt);
       runs.resize((size_t)ntiles * kMaxRuns);
       nruns.resize((size_t)ntiles);
-      launch_reset_ctl(stream, ctl.p);
+      launch_reset_ctl(stream, ctl.p, kCtlPlan);
       launch_tile_plan(stream, (int)n, tt, ntiles, cell_of_slot.p, cstart.p, g, budget, runs.p, nruns.p, ctl.p);
+    };
+    if (tentative) {
+      int tt = sizes[0];
+      if (large_fail_streak >= 2 && plan_hint > 0 && plan_count % 16 != 0) tt = plan_hint;
+      ++plan_count;
+      launch_plan(tt);
+      T = tt;
+      plan_tried = tt;
+      plan_pending = true;
+      return;
+    }
+    const int skip = plan_failed;   // a size whose plan just failed for this slot order
+    plan_failed = 0;
+    for (int q = 0; q < nsz; ++q) {
+      const int tt = sizes[q];
+      if (tt == skip) continue;
+      launch_plan(tt);
       read_ctl();
+      if (nan_pending) check_nanpos();
       if (!hctl->plan_fail) {
         T = tt;
         max_region = hctl->max_region;
+        plan_hint = tt;
         return;
       }
     }
+  }
+
+  // Bookkeeping once a tentative plan's result is read.  Returns true when it
+  // fitted (T and max_region are then final); otherwise the plan is invalid and
+  // the next ensure_plan() searches the other sizes.
+  bool tentative_plan_done(bool fitted) {
+    plan_pending = false;
+    const int largest = mixed() ? 256 : 128;
+    if (plan_tried == largest) large_fail_streak = fitted ? 0 : large_fail_streak + 1;
+    if (fitted) {
+      max_region = hctl->max_region;
+      plan_hint = T;
+      return true;
+    }
+    plan_valid = false;
+    plan_failed = plan_tried;
+    return false;
   }
 
   // neighbour lists hold slot indices: rebuild every list the last binning invalidated
@@ -548,7 +612,7 @@ struct espp_sim {
                                " is smaller than cutoff+skin " + std::to_string(cutVerlet));
     }
     if (v.has_excl && v.ex_nid != nid) upload_exclusions(v);
-    ensure_plan();
+    ensure_plan(true);
     const int ngroups = (int)((n + 63) / 64);
     if (v.cap == 0) {
       double vol = g.L[0] * g.L[1] * g.L[2];
@@ -574,7 +638,7 @@ struct espp_sim {
     // particles moved since the binning sorted the cells by x: widen the x clip
     const float clip_margin = disp_since_bin > 0.0 ? (float)(2.0 * disp_since_bin * (1.0 + 1e-6) + 4.0 * ulp) : 0.0f;
     for (int attempt = 0; attempt < 4; ++attempt) {
-      launch_reset_ctl(stream, ctl.p);
+      launch_reset_ctl(stream, ctl.p, kCtlBuild);
       hipEvent_t pe = prof_begin();
       if (v.tiled) {
         // 16-bit entries, eight per 16-byte group: cap / 2 ints per particle
@@ -591,7 +655,17 @@ struct espp_sim {
                          v.cnt.p, v.cap, ctl.p);
       }
       prof_end(1, pe);
+      // one control-block read per rebuild: the binning's lost-particle flag,
+      // the tentative tile plan and this build's capacity
       read_ctl();
+      if (nan_pending) check_nanpos();
+      if (plan_pending && !tentative_plan_done(!hctl->plan_fail)) {
+        // the tentative tile size did not fit: search the others, then build again
+        ensure_plan();
+        v.tiled = T > 0;
+        --attempt;
+        continue;
+      }
       v.maxcount = hctl->maxcount;
       if (!hctl->overflow) break;
       v.cap = ((int)std::ceil(v.maxcount * 1.2) + 8 + 7) & ~7;
@@ -648,7 +722,7 @@ struct espp_sim {
   }
 
   void decompose(bool carry_forces = true) {
-    bin(true, carry_forces);
+    bin(true, carry_forces, !vls.empty());
     for (auto* v : vls) build_vl(*v);
   }
 
