@@ -1109,20 +1109,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
           const int base = R.lds_off + (jb - R.slot_start);
           // the piece is sorted by x, so x_i - x_j falls along it: keep the stretch
           // with |x_i - x_j| <= reach
-          int ja = jb, je = je0;
+          // ja: first j with x_i - x_j <= reach; je: first j with x_i - x_j < -reach.
+          // Both searches run in lockstep, so their loads are in flight together
+          // (a finished search reloads its last probe, which the padding of posf
+          // keeps inside the buffer).
+          int ja, je;
           {
-            int lo = jb, hi = je0;                       // first j with x_i - x_j <= reach
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (xs - posf[mid].x > reach) lo = mid + 1; else hi = mid;
+            int lo1 = jb, hi1 = je0, lo2 = jb, hi2 = je0;
+            while (lo1 < hi1 || lo2 < hi2) {
+              const int m1 = (lo1 + hi1) >> 1, m2 = (lo2 + hi2) >> 1;
+              const float x1 = posf[m1].x, x2 = posf[m2].x;
+              if (lo1 < hi1) { if (xs - x1 > reach) lo1 = m1 + 1; else hi1 = m1; }
+              if (lo2 < hi2) { if (xs - x2 >= -reach) lo2 = m2 + 1; else hi2 = m2; }
             }
-            ja = lo;
-            hi = je0;                                    // first j with x_i - x_j < -reach
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (xs - posf[mid].x >= -reach) lo = mid + 1; else hi = mid;
-            }
-            je = lo;
+            ja = lo1;
+            je = lo2;
           }
           for (int j0 = ja; j0 < je; j0 += kCandChunk) {
             const int jn = je - j0;
