@@ -325,7 +325,7 @@ def _lattice_box(box, rho, seed, jitter=0.05):
     return pos + rng.uniform(-jitter, jitter, pos.shape)
 
 
-@pytest.mark.parametrize("case", ["two_types_orthorhombic", "fene_then_wca", "wca_then_fene"])
+@pytest.mark.parametrize("case", ["two_types_orthorhombic", "fene_then_wca", "wca_then_fene", "short_rows"])
 def test_step_variants_match_oracle(gpu, case):
     # Multi-step trajectories through the force kernel that also finishes the step: several
     # types in an orthorhombic box, an LJ interaction that accumulates onto FENE forces, and
@@ -337,6 +337,17 @@ def test_step_variants_match_oracle(gpu, case):
         types = (np.arange(n) % 3 == 0).astype(np.int32)
         rc, skin = 2.5, 0.3
         lj = ((0, 0, 1.0, 1.0), (0, 1, 1.2, 1.05), (1, 1, 0.8, 0.95))
+        bonds = None
+        dt = 0.005
+    elif case == "short_rows":
+        # rows of three cells: the largest tiles span more than three rows, so every
+        # rebuild's tentative tile plan fails first and the smaller tiles are used
+        box = (8.5, 30.0, 30.0)
+        pos = _lattice_box(box, 0.8442, seed=44)
+        n = len(pos)
+        types = None
+        rc, skin = 2.5, 0.3
+        lj = ((0, 0, 1.0, 1.0),)
         bonds = None
         dt = 0.005
     else:
