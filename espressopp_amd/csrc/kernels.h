@@ -6,7 +6,6 @@ namespace espp {
 
 void launch_fold_cell(hipStream_t s, int n, double4* pos, int4* img, int* cell_of, int* rank,
                       int* cell_count, const Geom& g, Ctl* ctl, int do_fold);
-void launch_scan_cells(hipStream_t s, int ncell, const int* count, int* start);
 void launch_scan_multi(hipStream_t s, int ncell, const int* count, int* start, int* blocksum, int* blockoff);
 int scan_blocks(int ncell);
 // blockmax of a needs nblocks(n, 256) entries
@@ -34,7 +33,6 @@ void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum,
                double* partial);
 void launch_tile_plan(hipStream_t s, int n, int T, int ntiles, const int* cell_of_slot, const int* cstart,
                       const Geom& g, int budget, TileRun* runs, int* nruns, Ctl* ctl);
-void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Geom& g);
 void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,

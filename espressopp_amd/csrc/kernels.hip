@@ -1826,14 +1826,6 @@ __global__ __launch_bounds__(256) void k_rdf(int n, const double4* __restrict__ 
   }
 }
 
-// fixed-point mirror of the FP64 positions (refresh before a tiled build of an FP64 system)
-__global__ void k_posq(int n, const double4* __restrict__ pos, uint4* __restrict__ posq, Geom g) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double4 p = pos[i];
-  posq[i] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]), (uint32_t)(int)p.w);
-}
-
 // ------------------------------------------------------------------ FENE bonds
 
 // Bonds are stored per particle id in both directions (bstart/bpartner), so
@@ -2080,10 +2072,6 @@ void launch_fold_cell(hipStream_t s, int n, double4* pos, int4* img, int* cell_o
   if (n > 0) hipLaunchKernelGGL(k_fold_cell, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, img, cell_of, rank, cell_count, g, ctl, do_fold);
 }
 
-void launch_scan_cells(hipStream_t s, int ncell, const int* count, int* start) {
-  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, ncell, count, start);
-}
-
 
 void launch_scan_multi(hipStream_t s, int ncell, const int* count, int* start, int* blocksum, int* blockoff) {
   const int nb = (ncell + kScanChunk - 1) / kScanChunk;
@@ -2178,9 +2166,6 @@ void launch_tile_plan(hipStream_t s, int n, int T, int ntiles, const int* cell_o
                        T, ntiles, cell_of_slot, cstart, g, budget, runs, nruns, ctl);
 }
 
-void launch_posq(hipStream_t s, int n, const double4* pos, uint4* posq, const Geom& g) {
-  if (n > 0) hipLaunchKernelGGL(k_posq, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, posq, g);
-}
 
 static bool cubic_box(const Geom& g) { return g.qsd[0] == g.qsd[1] && g.qsd[1] == g.qsd[2]; }
 
