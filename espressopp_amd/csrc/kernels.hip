@@ -126,7 +126,7 @@ __global__ void k_fold_cell(int n, double4* __restrict__ pos, int4* __restrict__
   double x[3] = {p.x, p.y, p.z};
   int iv[3] = {im.x, im.y, im.z};
   int c[3];
-  bool bad = false;
+  bool bad = false, moved = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     // NaN, infinite, or so far outside that the image count would overflow
@@ -135,6 +135,7 @@ __global__ void k_fold_cell(int n, double4* __restrict__ pos, int4* __restrict__
       int tmp = (int)floor(x[d] * g.invL[d]);
       iv[d] += tmp;
       x[d] -= (double)tmp * g.L[d];
+      moved |= tmp != 0;
     }
     int cc = (int)((x[d] - 0.0) * g.ics[d]) + 1;
     if (cc < 1) cc = 1;
@@ -145,9 +146,12 @@ __global__ void k_fold_cell(int n, double4* __restrict__ pos, int4* __restrict__
   const int r = wave_aggregated_rank(cell, live, cell_count);
   if (!live) return;
   if (bad) ctl->nanpos = 1;
-  p.x = x[0]; p.y = x[1]; p.z = x[2];
-  pos[i] = p;
-  img[i] = make_int4(iv[0], iv[1], iv[2], 0);
+  // a particle that stayed inside the box keeps its position and image bits
+  if (moved) {
+    p.x = x[0]; p.y = x[1]; p.z = x[2];
+    pos[i] = p;
+    img[i] = make_int4(iv[0], iv[1], iv[2], 0);
+  }
   cell_of[i] = cell;
   rank[i] = r;
 }
