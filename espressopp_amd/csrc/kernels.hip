@@ -1857,12 +1857,13 @@ __global__ __launch_bounds__(256) void k_fene(
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
   if (i < n) {
     const int pid = id[i];
+    // the step's own loads go out before the bond chain (id -> bonds -> partner slot)
+    double4 step_v, step_p;
+    if (STEP) { step_v = sa.vel[i]; step_p = sa.pos[i]; }
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     const int b0 = bstart[pid], b1 = bstart[pid + 1];
-    for (int b = b0; b < b1; ++b) {
-      const int q = bpartner[b];
-      const int j = (q >= 0 && q < nid) ? slot_of_id[q] : -1;
-      if (j < 0) continue;
+    auto bond = [&](int j) {
+#pragma clang fp contract(off)
       double dx, dy, dz;
       if (MIXED) {
         const uint4 qi = posq[i], qj = posq[j];
@@ -1911,13 +1912,22 @@ __global__ __launch_bounds__(256) void k_fene(
         acc[2] += dx * dx * ff; acc[3] += dy * dy * ff; acc[4] += dz * dz * ff;
         acc[5] += dx * dy * ff; acc[6] += dx * dz * ff; acc[7] += dy * dz * ff;
       }
+    };
+    // bonds two at a time: both partners' lookups are in flight together (a chain
+    // bead has two bonds), and the bonds are summed in their stored order
+    for (int b = b0; b < b1; b += 2) {
+      const int q0 = bpartner[b];
+      const int q1 = b + 1 < b1 ? bpartner[b + 1] : -1;
+      const int j0 = (q0 >= 0 && q0 < nid) ? slot_of_id[q0] : -1;
+      const int j1 = (q1 >= 0 && q1 < nid) ? slot_of_id[q1] : -1;
+      if (j0 >= 0) bond(j0);
+      if (j1 >= 0) bond(j1);
     }
     if (STEP) {
       double f0 = Ax, f1 = Ay, f2 = Az;
       if (ACCUM) { f0 = fx[i] + Ax; f1 = fy[i] + Ay; f2 = fz[i] + Az; }
-      const double4 v = sa.vel[i];
-      step_thermostat(i, v, f0, f1, f2, sa);
-      stepsq = step_kick_drift(i, v, sa.pos[i], f0, f1, f2, sa, g);
+      step_thermostat(i, step_v, f0, f1, f2, sa);
+      stepsq = step_kick_drift(i, step_v, step_p, f0, f1, f2, sa, g);
     } else if (!ENERGY) {
       if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
       else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
