@@ -1,11 +1,14 @@
 #!/usr/bin/env python3
 """Benchmark of the short-range MD step on one MI355X.
 
-Workload: Lennard-Jones fluid, N = 4,194,304 (first N sites of a 162^3
-simple-cubic lattice, jittered U(-0.05, 0.05)), density 0.8442, eps = sigma = 1, rc = 2.5 (shift auto), skin 0.3,
-dt = 0.005.  Warm-up: W Langevin steps at T = 0.72, gamma = 1 (untimed), then
-K timed NVE steps.  A step is one velocity-Verlet step: integrate1, skin check,
-rebuild when triggered, pair forces, integrate2.
+Workload: Lennard-Jones fluid, N = 4,194,304 (a 128^3 simple-cubic lattice,
+jittered U(-0.05, 0.05)), density 0.8442, eps = sigma = 1, rc = 2.5 (shift auto),
+skin 0.3, dt = 0.005.  Equilibration: 1000 Langevin steps at T = 0.72, gamma = 1
+(untimed, always), then W untimed NVE steps (--warmup), then K timed NVE steps.
+A step is one velocity-Verlet step: integrate1, skin check, rebuild when
+triggered, pair forces, integrate2.  NVE energy drift is the fitted slope of the
+total energy over a separate run of --drift-steps NVE steps (default 10^4), for
+both the mixed and the FP64 path.
 
 Prints one JSON line: particle-steps/s of the mixed-precision path (value),
 the all-FP64 GPU path on the same state, force accuracy of the mixed path
@@ -98,6 +101,25 @@ def vl_entries(h, vl):
     return e.value, cap.value, mx.value
 
 
+def drift_run(system, integ, lj, n, steps, chunk=500):
+    """NVE energy drift: total energy every `chunk` steps over `steps` steps, least-squares
+    slope, per particle per 1e5 steps (and the RMS of the residuals, per particle)."""
+    t, e = [0], [total_energy(system, lj, n)]
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        integ.run(k)
+        done += k
+        t.append(done)
+        e.append(total_energy(system, lj, n))
+    t, e = np.asarray(t, dtype=np.float64), np.asarray(e)
+    slope, icpt = np.polyfit(t, e, 1)
+    resid = e - (slope * t + icpt)
+    return dict(steps=steps, samples=len(t), per_particle_per_1e5_steps=slope / n * 1e5,
+                fluctuation_rms_per_particle=float(np.sqrt(np.mean(resid ** 2)) / n),
+                e_first_per_particle=float(e[0] / n), e_last_per_particle=float(e[-1] / n))
+
+
 def cpu_baseline(pos, vel, box, cells, rc, skin, steps, threads):
     from oracle.oracle import OracleSim
     n = len(pos)
@@ -131,7 +153,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed NVE steps after the 1000-step Langevin equilibration")
+    ap.add_argument("--equil", type=int, default=1000,
+                    help="Langevin equilibration steps at T=0.72 (the workload's definition: 1000)")
+    ap.add_argument("--drift-steps", type=int, default=10000,
+                    help="NVE steps of the separate energy-drift run of each path (0: skip)")
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=100)
@@ -173,9 +200,11 @@ def main():
     lang.temperature = 0.72
     integ.addExtension(lang)
     t = time.perf_counter()
-    integ.run(args.warmup)
-    log("warmup %d Langevin steps: %.2fs" % (args.warmup, time.perf_counter() - t))
+    integ.run(args.equil)
+    log("equilibration %d Langevin steps: %.2fs" % (args.equil, time.perf_counter() - t))
     lang.disconnect()
+    if args.warmup:
+        integ.run(args.warmup)
     h = system.storage._h
     state = system.storage.getParticleArrays()
 
@@ -200,9 +229,7 @@ def main():
         log("accuracy:", err)
         del f32, f64
 
-    e0 = total_energy(system, lj, n)
     dt_run = timed_run(espressopp, system, integ, args.steps)
-    e1 = total_energy(system, lj, n)
     value = n * args.steps / dt_run
     # per-kernel durations from HIP events on the library's stream, in a separate NVE run
     # (events are kept out of the timed run)
@@ -234,6 +261,11 @@ def main():
         "cap %d, resorts %d" % (dt_run / args.steps * 1e3, value, avg_s * 1e3, launches, nbar, cap, resorts))
     for k, (c, ms) in stats.items():
         log("  %-11s %5d launches, %.3f ms avg" % (k, c, ms / max(c, 1)))
+    drift = None
+    if args.drift_steps:
+        t = time.perf_counter()
+        drift = drift_run(system, integ, lj, n, args.drift_steps)
+        log("mixed drift over %d NVE steps (%.1fs): %s" % (args.drift_steps, time.perf_counter() - t, drift))
 
     if not args.no_fp64:
         k64 = args.fp64_steps or args.steps
@@ -247,19 +279,29 @@ def main():
                     steps=k64)
         log("fp64: %.3f ms/step, %.4g particle-steps/s, pair kernel %.3f ms" %
             (fp64["ms_per_step"], fp64["value"], fp64["pair_kernel_ms"]))
+        if args.drift_steps:
+            t = time.perf_counter()
+            fp64["drift"] = drift_run(sys64, integ64, lj64, n, args.drift_steps)
+            log("fp64 drift over %d NVE steps (%.1fs): %s" % (args.drift_steps, time.perf_counter() - t,
+                                                              fp64["drift"]))
         del sys64, vl64, lj64, integ64
 
     cpu = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        # every host core this process may use: the affinity set, capped by the OpenMP thread
+        # count the host grants one GPU's job (OMP_NUM_THREADS, when set)
+        affinity = len(os.sched_getaffinity(0))
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = min(affinity, omp) if omp > 0 else affinity
         cval, ctime = cpu_baseline(state["pos"] + state["image"] * np.asarray(box), state["v"], box, cells, rc,
                                    skin, args.cpu_steps, threads)
         cpu = {"value": cval, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": "same 4,194,304-particle LJ state after warm-up, %d NVE steps of the FP64 "
-                         "oracle (half Verlet list, Newton-3, OpenMP) = %.1f s" % (args.cpu_steps, ctime)}
+               "host_nproc": os.cpu_count(), "affinity_cpus": affinity, "omp_num_threads": omp or None,
+               "sample": "same 4,194,304-particle LJ state after equilibration, %d NVE steps of the FP64 "
+                         "oracle (half Verlet list, Newton-3, OpenMP on %d threads) = %.1f s"
+                         % (args.cpu_steps, threads, ctime)}
         log("cpu:", cpu)
 
-    drift = (e1 - e0) / n / args.steps * 1e5
     out = {
         "metric": "particle-steps/s, LJ 4M: mixed vs FP64 GPU path at force rel err <=1e-5",
         "value": value,
@@ -267,14 +309,16 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "equilibration_steps": args.equil,
         "ms_per_step": dt_run / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32-fixed/fp64",
-        "data": "synthetic: jittered simple-cubic lattice, Maxwell-Boltzmann velocities, Langevin warm-up",
+        "data": "synthetic: jittered simple-cubic lattice, Maxwell-Boltzmann velocities, Langevin equilibration",
         "config": {"workload": "LJ fluid N=%d rho=0.8442 rc=2.5 skin=0.3 dt=0.005 NVE after %d-step "
-                               "Langevin warm-up at T=0.72" % (n, args.warmup),
+                               "Langevin equilibration at T=0.72 (+%d untimed NVE steps)"
+                               % (n, args.equil, args.warmup),
                    "n": n, "box": box[0], "cells": cells, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -283,7 +327,8 @@ def main():
         "fp64_path": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "accuracy": err,
-        "nve_drift_per_particle_per_1e5_steps": drift,
+        "nve_drift_per_particle_per_1e5_steps": drift["per_particle_per_1e5_steps"] if drift else None,
+        "nve_drift": drift,
         "resorts": resorts,
         "cpu_baseline": cpu,
     }

@@ -24,7 +24,7 @@ SIGNATURES = {
     "espp_last_error": [],
     "espp_version": [],
     "espp_device_count": [_P(_c_int)],
-    "espp_create": [_P(_c_dbl), _P(_c_int), _c_dbl, _c_u64, _c_int, _P(_vp)],
+    "espp_create": [_P(_c_dbl), _P(_c_int), _c_int, _c_dbl, _c_u64, _c_int, _P(_vp)],
     "espp_destroy": [_vp],
     "espp_set_skin": [_vp, _c_dbl],
     "espp_add_particles": [_vp, _c_i64, _vp, _vp, _vp, _vp, _vp],
@@ -33,6 +33,7 @@ SIGNATURES = {
     "espp_get_ids": [_vp, _vp],
     "espp_get_particles": [_vp, _c_int, _vp],
     "espp_set_particles": [_vp, _c_int, _c_i64, _vp, _vp],
+    "espp_get_particles_by_id": [_vp, _c_int, _c_i64, _vp, _vp],
     "espp_vl_create": [_vp, _c_dbl, _P(_c_int)],
     "espp_vl_exclude": [_vp, _c_int, _c_i64, _vp],
     "espp_vl_rebuild": [_vp, _c_int],
@@ -50,6 +51,9 @@ SIGNATURES = {
     "espp_tab_set": [_vp, _c_int, _c_int, _c_int, _c_int, _c_i64, _vp, _vp, _vp, _c_dbl],
     "espp_tabbond_create": [_vp, _c_int, _c_int, _c_i64, _vp, _vp, _vp, _c_dbl, _P(_c_int)],
     "espp_add_interaction": [_vp, _c_int],
+    "espp_remove_interaction": [_vp, _c_int],
+    "espp_fene_set": [_vp, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl],
+    "espp_ia_set_fpl": [_vp, _c_int, _c_int],
     "espp_ia_energy": [_vp, _c_int, _P(_c_dbl)],
     "espp_ia_virial": [_vp, _c_int, _P(_c_dbl), _vp],
     "espp_compute_forces": [_vp],
@@ -127,12 +131,13 @@ def ptr(a):
 class Handle:
     """Owner of one espp_sim."""
 
-    def __init__(self, box, cells, skin, seed, precision):
+    def __init__(self, box, cells, skin, seed, precision, half_cell_int=1):
         lib = load()
         b = (_c_dbl * 3)(*[float(x) for x in box])
         c = (_c_int * 3)(*[int(x) for x in cells])
         h = _vp()
-        check(lib.espp_create(b, c, float(skin), int(seed) & 0xFFFFFFFFFFFFFFFF, int(precision), ctypes.byref(h)))
+        check(lib.espp_create(b, c, int(half_cell_int), float(skin), int(seed) & 0xFFFFFFFFFFFFFFFF, int(precision),
+                              ctypes.byref(h)))
         self.h = h
         self.lib = lib
 
@@ -180,6 +185,13 @@ class Handle:
         width = 1 if field in ("type", "mass") else 3
         out = np.zeros((n, width) if width == 3 else n, dtype=np.float64)
         self.call("espp_get_particles", FIELD[field], ptr(out))
+        return out
+
+    def get_by_id(self, field, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        width = 1 if field in ("type", "mass") else 3
+        out = np.zeros((len(ids), width) if width == 3 else len(ids), dtype=np.float64)
+        self.call("espp_get_particles_by_id", FIELD[field], len(ids), ptr(ids), ptr(out))
         return out
 
     def set(self, field, ids, values):

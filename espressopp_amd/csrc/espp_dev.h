@@ -24,6 +24,15 @@ struct Geom {
   double qsd[3];                   // L / 2^32 in FP64
   double qs2;                      // (L / 2^32)^2 for cubic boxes (tiled mixed kernel)
   float qsf;                       // L / 2^32 as FP32 (cubic boxes)
+  // The caller's cell grid with halfCellInt h (src/storage/DomainDecomposition.cpp:325-374):
+  // nh = h * n cells of size >= (rc+skin)/h, stencil +-h.  The binning above uses the
+  // grid n = nh / h; the caller's grid only fixes which of a pair's two particles the
+  // reference takes as the first, i.e. the orientation of d for the list test.
+  int hci;                         // halfCellInt (1: the two grids coincide)
+  int nh[3];                       // the caller's cells per dimension
+  double hics[3];                  // 1 / (L / nh)
+  int* hcell;                      // per slot: linear cell of the caller's grid at the last binning (hci > 1)
+  const int* cslot;                // per slot: binning cell (cell_of_slot), for the reference's orientation
 };
 
 // Lennard-Jones coefficients of one type pair (src/interaction/LennardJones.hpp:79-87).

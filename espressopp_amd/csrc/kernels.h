@@ -76,6 +76,12 @@ constexpr int kRdfLdsBins = 12288;   // histograms up to this many bins are kept
 void launch_rdf(hipStream_t s, int n, const double4* pos, const Geom& g, double dr, int nbins,
                 unsigned long long* counts);
 void launch_zero3(hipStream_t s, int n, double* a, double* b, double* c);
+// fields of m particles given by id (3 doubles each) and their slots (-1: no such id)
+void launch_gather_ids(hipStream_t s, int m, const int* ids, const int* slot_of_id, int nid, int field,
+                       const double4* pos, const double4* vel, const double* fx, const double* fy, const double* fz,
+                       const int4* img, double* out, int* slots_out);
+void launch_scatter_slots(hipStream_t s, int m, const int* slots, int field, const double* val, double4* pos,
+                          double4* vel);
 void launch_thermo_integrate2(hipStream_t s, int n, const int* id, double4* vel, double* fx,
                               double* fy, double* fz, int lang, double pref1, double pref2,
                               unsigned long long seed, unsigned long long call,

@@ -27,6 +27,12 @@ namespace espp {
 
 // ------------------------------------------------------------------ helpers
 
+// |d|^2 summed left to right without contraction (Real3D::sqr, src/Real3D.hpp)
+__device__ __forceinline__ double sqr_nc(double x, double y, double z) {
+#pragma clang fp contract(off)
+  return x * x + y * y + z * z;
+}
+
 __device__ __forceinline__ uint32_t fixq(double x, double qinv) {
   // position -> 32-bit fixed point with L <-> 2^32; the int64 detour makes the
   // mapping modular, so unfolded positions wrap onto the same periodic image.
@@ -348,6 +354,64 @@ __global__ void k_sort_cells_big(int ncell, const int* __restrict__ start, int* 
   }
 }
 
+// Cell of the caller's grid (halfCellInt > 1) holding a folded position:
+// CellGrid::mapPositionToCellClipped for one node (src/storage/CellGrid.cpp:84-97),
+// truncation toward zero, then clipped to the inner cells.
+__device__ __forceinline__ int caller_cell(const double4& p, const Geom& g) {
+  const double x[3] = {p.x, p.y, p.z};
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int cc = (int)(x[d] * g.hics[d]);
+    if (cc < 0) cc = 0;
+    else if (cc >= g.nh[d]) cc = g.nh[d] - 1;
+    c[d] = cc;
+  }
+  return c[0] + g.nh[0] * (c[1] + g.nh[1] * c[2]);
+}
+
+// Orientation of the reference's list test for halfCellInt > 1: the pair is
+// visited from the cell with the smaller framed index (neighbours within +-h
+// cells, src/storage/DomainDecomposition.cpp:342-358), so d = xi - (xj + s L)
+// iff the offset of j's ghost cell (its cell shifted by s grid lengths) from
+// i's cell is lexicographically positive, z most significant.
+__device__ __forceinline__ bool caller_forward(int ci, int cj, int sx, int sy, int sz, const Geom& g) {
+  const int nx = g.nh[0], ny = g.nh[1], nz = g.nh[2];
+  const int ox = cj % nx + sx * nx - ci % nx;
+  const int oy = (cj / nx) % ny + sy * ny - (ci / nx) % ny;
+  const int oz = cj / (nx * ny) + sz * nz - ci / (nx * ny);
+  return (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
+}
+
+// |d|^2 of a listed pair (slots i, j) as the reference's list and force loops
+// evaluate it: d in the orientation of the half-shell visit, with the second
+// particle's ghost at x + s L (s: j's periodic image nearest to i; see k_nbr_build),
+// summed left to right without contraction.  The force kernels form d their own
+// way and use this only for pairs within a few ulp of a cutoff, so that such a
+// pair falls on the reference's side of it.
+__device__ double ref_pair_r2(int i, int j, const double4& pi, const double4& pj, const Geom& g) {
+#pragma clang fp contract(off)
+  const int sx = (int)__builtin_rint((pi.x - pj.x) * g.invL[0]);
+  const int sy = (int)__builtin_rint((pi.y - pj.y) * g.invL[1]);
+  const int sz = (int)__builtin_rint((pi.z - pj.z) * g.invL[2]);
+  bool fwd;
+  if (g.hci > 1) {
+    fwd = caller_forward(g.hcell[i], g.hcell[j], sx, sy, sz, g);
+  } else {
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const int ci = g.cslot[i], cj = g.cslot[j];
+    const int ox = cj % nx + sx * nx - ci % nx;
+    const int oy = (cj / nx) % ny + sy * ny - (ci / nx) % ny;
+    const int oz = cj / (nx * ny) + sz * nz - ci / (nx * ny);
+    fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
+  }
+  const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
+  double ex, ey, ez;
+  if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
+  else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
+  return ex * ex + ey * ey + ez * ez;
+}
+
 __global__ void k_permute(int n, const int* __restrict__ order, const int* __restrict__ cell_of,
                           const double4* __restrict__ pos, const double4* __restrict__ vel,
                           const double* __restrict__ fx, const double* __restrict__ fy,
@@ -378,6 +442,7 @@ __global__ void k_permute(int n, const int* __restrict__ order, const int* __res
     posq[s] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]),
                          (uint32_t)(int)p.w);
   posf[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+  if (g.hci > 1) g.hcell[s] = caller_cell(p, g);
 }
 
 // FP32 copy of the positions, the candidates of the tiled list build.
@@ -490,7 +555,7 @@ __global__ __launch_bounds__(256) void k_nbr_build(int n, const double4* __restr
             if (j == i && unshifted) continue;
             const double4 pj = pos[j];
             double dx, dy, dz;
-            if (fwd) {
+            if (g.hci > 1 ? caller_forward(g.hcell[i], g.hcell[j], sx, sy, sz, g) : fwd) {
               dx = pi.x - (pj.x + shx);
               dy = pi.y - (pj.y + shy);
               dz = pi.z - (pj.z + shz);
@@ -570,13 +635,8 @@ __global__ __launch_bounds__(256) void k_lj_mixed(
       const double r2 = pair_r2<false>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
       bool in = p.set && r2 <= p.drc2;
       if (p.set && r2 >= p.dlo && r2 <= p.dhi) {
-        // fixed-point quantisation could flip the cutoff test: decide from FP64 positions
-        const double4 pi = pos[i], pj = pos[j];
-        double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
-        hx = __builtin_fma(-g.L[0], __builtin_rint(hx * g.invL[0]), hx);
-        hy = __builtin_fma(-g.L[1], __builtin_rint(hy * g.invL[1]), hy);
-        hz = __builtin_fma(-g.L[2], __builtin_rint(hz * g.invL[2]), hz);
-        in = (hx * hx + hy * hy + hz * hz) <= p.drc2;
+        // fixed-point quantisation could flip the cutoff test: decide it as the reference does
+        in = ref_pair_r2(i, j, pos[i], pos[j], g) <= p.drc2;
       }
       double inv;
       double ff = lj_ff_seeded(r2, p.dff1, p.dff2, inv);
@@ -622,7 +682,7 @@ __global__ __launch_bounds__(256) void k_lj_fp64(
     const int c = min(cnt[i], cap);
     const int rowi = (int)pi.w * ntypes;
     double ff1 = prm[0].dff1, ff2 = prm[0].dff2, rc2 = prm[0].drc2, shift = prm[0].dshift;
-    double sig2 = prm[0].sig2, eps4 = prm[0].eps4;
+    double sig2 = prm[0].sig2, eps4 = prm[0].eps4, lo = prm[0].dlo, hi = prm[0].dhi;
     int set = prm[0].set;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     for (int k = 0; k < c; ++k) {
@@ -635,10 +695,11 @@ __global__ __launch_bounds__(256) void k_lj_fp64(
       if (!ONETYPE) {
         const LJParam& p = prm[rowi + (int)pj.w];
         ff1 = p.dff1; ff2 = p.dff2; rc2 = p.drc2; shift = p.dshift; sig2 = p.sig2; eps4 = p.eps4;
-        set = p.set;
+        set = p.set; lo = p.dlo; hi = p.dhi;
       }
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      const bool in = set && r2 <= rc2;
+      const double r2 = sqr_nc(dx, dy, dz);
+      bool in = set && r2 <= rc2;
+      if (set && r2 >= lo && r2 <= hi) in = ref_pair_r2(i, j, pi, pj, g) <= rc2;   // at the cutoff (rare)
       const double inv = rcp_nr(r2);
       const double r6 = inv * inv * inv;
       double ff = r6 * (ff1 * r6 - ff2) * inv;
@@ -1160,7 +1221,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
               const int cjx = cell_of_slot[j] % nx;
               int ox = cjx - cx, sx = 0;
               if (ox > 1) { ox = -1; sx = -1; } else if (ox < -1) { ox = 1; sx = 1; }
-              const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
+              const bool fwd = g.hci > 1 ? caller_forward(g.hcell[i], g.hcell[j], sx, sy, sz, g)
+                                         : (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
               const double4 pj = pos[j];
               const double shx = (double)sx * g.L[0], shy = (double)sy * g.L[1], shz = (double)sz * g.L[2];
               double ex, ey, ez;
@@ -1426,12 +1488,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
           const double r2 = pair_r2<CUBIC>(qi, (int)(qi.x - qj.x), (int)(qi.y - qj.y), (int)(qi.z - qj.z), g, ex, ey, ez);
           if (r2 < (CUBIC ? p.ulo : p.dlo) || !(r2 <= (CUBIC ? p.uhi : p.dhi))) continue;
           const int j = slot_of_lds(sr, nr, e);
-          const double4 pi = pos[i], pj = pos[j];
-          double hx = pi.x - pj.x, hy = pi.y - pj.y, hz = pi.z - pj.z;
-          hx = __builtin_fma(-g.L[0], __builtin_rint(hx * g.invL[0]), hx);
-          hy = __builtin_fma(-g.L[1], __builtin_rint(hy * g.invL[1]), hy);
-          hz = __builtin_fma(-g.L[2], __builtin_rint(hz * g.invL[2]), hz);
-          if (!(hx * hx + hy * hy + hz * hz <= p.drc2)) continue;
+          if (!(ref_pair_r2(i, j, pos[i], pos[j], g) <= p.drc2)) continue;
           double inv;
           const double ff = lj_ff_seeded(r2, CUBIC ? p.uff1 : p.dff1, CUBIC ? p.uff2 : p.dff2, inv);
           Ax = __builtin_fma(ex, ff, Ax);
@@ -1499,7 +1556,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_fp64(
     const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
     const int rowi = (int)pi.w * ntypes;
     double ff1 = prm[0].dff1, ff2 = prm[0].dff2, rc2 = prm[0].drc2, shift = prm[0].dshift;
-    double sig2 = prm[0].sig2, eps4 = prm[0].eps4;
+    double sig2 = prm[0].sig2, eps4 = prm[0].eps4, lo = prm[0].dlo, hi = prm[0].dhi;
     int set = prm[0].set;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     const int ng = (c + 7) >> 3;
@@ -1520,10 +1577,12 @@ __global__ __launch_bounds__(256) void k_lj_tiled_fp64(
         if (!ONETYPE) {
           const LJParam& p = prm[rowi + (int)pj.w];
           ff1 = p.dff1; ff2 = p.dff2; rc2 = p.drc2; shift = p.dshift; sig2 = p.sig2; eps4 = p.eps4;
-          set = p.set;
+          set = p.set; lo = p.dlo; hi = p.dhi;
         }
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const bool in = set && r2 <= rc2;
+        const double r2 = sqr_nc(dx, dy, dz);
+        bool in = set && r2 <= rc2;
+        if (set && r2 >= lo && r2 <= hi)   // at the cutoff (rare)
+          in = ref_pair_r2(i, slot_of_lds(sr, nr, e), pi, pos[slot_of_lds(sr, nr, e)], g) <= rc2;
         const double inv = rcp_nr(r2);
         const double r6 = inv * inv * inv;
         double ff = r6 * (ff1 * r6 - ff2) * inv;
@@ -1605,12 +1664,13 @@ __device__ __forceinline__ double tab_energy(const TabParam& t, const double* __
 
 // Adds the contribution of one pair (distance vector d, r2 = |d|^2) to the
 // particle's force or to its energy/virial partials.
+// r2c is the |d|^2 the cut test takes (the reference's own near the cutoff).
 template <bool ENERGY>
 __device__ __forceinline__ void tab_pair(const TabParam& t, const double* __restrict__ pool, double dx, double dy,
-                                         double dz, double r2, double& Ax, double& Ay, double& Az,
+                                         double dz, double r2, double r2c, double& Ax, double& Ay, double& Az,
                                          double (&acc)[kNPart]) {
 #pragma clang fp contract(off)
-  if (!t.set || r2 > t.rc2) return;
+  if (!t.set || r2c > t.rc2) return;
   const double ff = tab_ffactor(t, pool, r2);
   const double Fx = dx * ff, Fy = dy * ff, Fz = dz * ff;
   if (!ENERGY) {
@@ -1626,6 +1686,7 @@ __device__ __forceinline__ void tab_pair(const TabParam& t, const double* __rest
 // FP64 minimum-image distance of slots i and j from the FP64 positions
 __device__ __forceinline__ double min_image_fp64(const double4& pi, const double4& pj, const Geom& g, double& dx,
                                                  double& dy, double& dz) {
+#pragma clang fp contract(off)
   dx = pi.x - pj.x; dy = pi.y - pj.y; dz = pi.z - pj.z;
   dx = __builtin_fma(-g.L[0], __builtin_rint(dx * g.invL[0]), dx);
   dy = __builtin_fma(-g.L[1], __builtin_rint(dy * g.invL[1]), dy);
@@ -1663,7 +1724,7 @@ __global__ __launch_bounds__(256) void k_tab_tiled(
     for (int k = 0; k < c; ++k) {
       const uint32_t w = np[(size_t)(k >> 3) * 256 + ((k >> 1) & 3)];
       const int e = (k & 1) ? (int)(w >> 16) : (int)(w & 0xFFFFu);
-      double dx, dy, dz, r2;
+      double dx, dy, dz, r2, r2c;
       int tj;
       if (MIXED) {
         const uint4 qj = lds_raw[e];
@@ -1672,17 +1733,22 @@ __global__ __launch_bounds__(256) void k_tab_tiled(
         dy = (double)(int)(qi.y - qj.y) * g.qsd[1];
         dz = (double)(int)(qi.z - qj.z) * g.qsd[2];
         r2 = dx * dx + dy * dy + dz * dz;
-        const TabParam& t = prm[rowi + tj];
-        if (r2 >= t.dlo && r2 <= t.dhi) {
-          // fixed-point quantisation could flip the cut test: the pair from the FP64 positions
-          r2 = min_image_fp64(pi, pos[slot_of_lds(sr, nr, e)], g, dx, dy, dz);
-        }
       } else {
         const double4 pj = reinterpret_cast<const double4*>(lds_raw)[e];
         tj = (int)pj.w;
         r2 = min_image_fp64(pi, pj, g, dx, dy, dz);
       }
-      tab_pair<ENERGY>(prm[rowi + tj], pool, dx, dy, dz, r2, Ax, Ay, Az, acc);
+      r2c = r2;
+      const TabParam& t = prm[rowi + tj];
+      if (r2 >= t.dlo && r2 <= t.dhi) {
+        // at the cutoff, where quantisation or the rounding of d could flip the cut test:
+        // the pair from the FP64 positions, and the reference's own |d|^2 for the test
+        const int j = slot_of_lds(sr, nr, e);
+        const double4 pj = pos[j];
+        r2 = min_image_fp64(pi, pj, g, dx, dy, dz);
+        r2c = ref_pair_r2(i, j, pi, pj, g);
+      }
+      tab_pair<ENERGY>(t, pool, dx, dy, dz, r2, r2c, Ax, Ay, Az, acc);
     }
     if (!ENERGY) {
       if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
@@ -1713,10 +1779,13 @@ __global__ __launch_bounds__(256) void k_tab(
     const int rowi = (int)pi.w * ntypes;
     double Ax = 0.0, Ay = 0.0, Az = 0.0;
     for (int k = 0; k < c; ++k) {
-      const double4 pj = pos[np[(size_t)k * 64]];
+      const int j = np[(size_t)k * 64];
+      const double4 pj = pos[j];
       double dx, dy, dz;
       const double r2 = min_image_fp64(pi, pj, g, dx, dy, dz);
-      tab_pair<ENERGY>(prm[rowi + (int)pj.w], pool, dx, dy, dz, r2, Ax, Ay, Az, acc);
+      const TabParam& t = prm[rowi + (int)pj.w];
+      const double r2c = (r2 >= t.dlo && r2 <= t.dhi) ? ref_pair_r2(i, j, pi, pj, g) : r2;
+      tab_pair<ENERGY>(t, pool, dx, dy, dz, r2, r2c, Ax, Ay, Az, acc);
     }
     if (!ENERGY) {
       if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
@@ -1759,7 +1828,8 @@ __global__ __launch_bounds__(256) void k_bond_tab(
       if (dx < -g.halfL[0]) dx += g.L[0]; else if (dx > g.halfL[0]) dx -= g.L[0];
       if (dy < -g.halfL[1]) dy += g.L[1]; else if (dy > g.halfL[1]) dy -= g.L[1];
       if (dz < -g.halfL[2]) dz += g.L[2]; else if (dz > g.halfL[2]) dz -= g.L[2];
-      tab_pair<ENERGY>(t, pool, dx, dy, dz, dx * dx + dy * dy + dz * dz, Ax, Ay, Az, acc);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      tab_pair<ENERGY>(t, pool, dx, dy, dz, r2, r2, Ax, Ay, Az, acc);
     }
     if (!ENERGY) {
       if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
@@ -1938,6 +2008,60 @@ __global__ __launch_bounds__(256) void k_fene(
 #pragma unroll
     for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
     block_reduce_store<kNPart>(acc, partial);
+  }
+}
+
+// Properties of m particles given by id (field numbering of espp_amd.h: 0 pos,
+// 1 vel, 2 force, 3 image, 4 type, 5 mass); out holds 3 doubles per particle.
+// slots_out[k] is the particle's slot, or -1 for an id that does not exist.
+__global__ void k_gather_ids(int m, const int* __restrict__ ids, const int* __restrict__ slot_of_id, int nid,
+                             int field, const double4* __restrict__ pos, const double4* __restrict__ vel,
+                             const double* __restrict__ fx, const double* __restrict__ fy,
+                             const double* __restrict__ fz, const int4* __restrict__ img,
+                             double* __restrict__ out, int* __restrict__ slots_out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int pid = ids[k];
+  const int s = (pid >= 0 && pid < nid) ? slot_of_id[pid] : -1;
+  slots_out[k] = s;
+  if (s < 0) return;
+  double a = 0.0, b = 0.0, c = 0.0;
+  if (field == 0 || field == 4) {
+    const double4 p = pos[s];
+    if (field == 0) { a = p.x; b = p.y; c = p.z; } else { a = p.w; }
+  } else if (field == 1 || field == 5) {
+    const double4 v = vel[s];
+    if (field == 1) { a = v.x; b = v.y; c = v.z; } else { a = v.w; }
+  } else if (field == 2) {
+    a = fx[s]; b = fy[s]; c = fz[s];
+  } else {
+    const int4 q = img[s];
+    a = (double)q.x; b = (double)q.y; c = (double)q.z;
+  }
+  out[3 * k] = a;
+  out[3 * k + 1] = b;
+  out[3 * k + 2] = c;
+}
+
+// Set a property of m particles at the given slots (pos / vel: 3 values per
+// particle; type / mass: 1).
+__global__ void k_scatter_slots(int m, const int* __restrict__ slots, int field, const double* __restrict__ val,
+                                double4* __restrict__ pos, double4* __restrict__ vel) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int s = slots[k];
+  if (field == 0) {
+    double4 p = pos[s];
+    p.x = val[3 * k]; p.y = val[3 * k + 1]; p.z = val[3 * k + 2];
+    pos[s] = p;
+  } else if (field == 4) {
+    pos[s].w = val[k];
+  } else if (field == 1) {
+    double4 v = vel[s];
+    v.x = val[3 * k]; v.y = val[3 * k + 1]; v.z = val[3 * k + 2];
+    vel[s] = v;
+  } else {
+    vel[s].w = val[k];
   }
 }
 
@@ -2318,6 +2442,20 @@ void launch_rdf(hipStream_t s, int n, const double4* pos, const Geom& g, double 
                        counts);
   else
     hipLaunchKernelGGL(k_rdf<false>, grid, block, 0, s, n, pos, g, dr, nbins, jchunk, counts);
+}
+
+void launch_gather_ids(hipStream_t s, int m, const int* ids, const int* slot_of_id, int nid, int field,
+                       const double4* pos, const double4* vel, const double* fx, const double* fy, const double* fz,
+                       const int4* img, double* out, int* slots_out) {
+  if (m > 0)
+    hipLaunchKernelGGL(k_gather_ids, dim3(nblocks(m, 256)), dim3(256), 0, s, m, ids, slot_of_id, nid, field, pos, vel,
+                       fx, fy, fz, img, out, slots_out);
+}
+
+void launch_scatter_slots(hipStream_t s, int m, const int* slots, int field, const double* val, double4* pos,
+                          double4* vel) {
+  if (m > 0)
+    hipLaunchKernelGGL(k_scatter_slots, dim3(nblocks(m, 256)), dim3(256), 0, s, m, slots, field, val, pos, vel);
 }
 
 void launch_zero3(hipStream_t s, int n, double* a, double* b, double* c) {

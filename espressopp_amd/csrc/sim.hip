@@ -305,6 +305,7 @@ struct espp_sim {
   DBuf<int> scan_sum, scan_off;   // per-chunk totals and offsets of the cell scan
   DBuf<double> blockmax;          // per-block max |dx|^2 of the fused step
   DBuf<int> slot_of_id;
+  DBuf<int> hcell;                // per slot: cell of the caller's grid (halfCellInt > 1)
   int nid = 0;
 
   // tile plan of the current slot order (tiled list build and force kernels)
@@ -447,9 +448,14 @@ struct espp_sim {
     pos.grow_keep(m, keep, stream); vel.grow_keep(m, keep, stream);
     fx.grow_keep(m, keep, stream); fy.grow_keep(m, keep, stream); fz.grow_keep(m, keep, stream);
     id.grow_keep(m, keep, stream); img.grow_keep(m, keep, stream);
+    if (g.hci > 1) {
+      hcell.grow_keep(m, keep, stream);
+      g.hcell = hcell.p;
+    }
     pos2.resize(m); vel2.resize(m); fx2.resize(m); fy2.resize(m); fz2.resize(m);
     id2.resize(m); img2.resize(m);
     cell_of.resize(m); rank.resize(m); order.resize(m); cell_of_slot.grow_keep(m, keep, stream);
+    g.cslot = cell_of_slot.p;
     // the tiled list build reads up to 31 entries past a candidate range
     posq.grow_keep(m + kPosqPad, keep, stream);
     posq2.resize(m + kPosqPad);
@@ -1109,16 +1115,25 @@ void need(bool ok, const char* msg) {
 // two mirrors are exact int32 and equal the minimum image modulo P = L / 2^k;
 // k is the largest value keeping P >= 6 cell sizes, so every candidate of the
 // 27 neighbour cells (|dx| < 2 cells plus drift) is represented unaliased.
-void set_geom(Geom& g, const double box[3], const int32_t cells[3]) {
+// The binning grid is the caller's grid coarsened by halfCellInt h: floor(nh / h)
+// cells, each at least h times the caller's cell, i.e. at least rc + skin for a
+// grid from tools.decomp.cellGrid(..., h).
+void set_geom(Geom& g, const double box[3], const int32_t cells[3], int hci) {
+  g.hci = hci;
+  g.hcell = nullptr;
+  g.cslot = nullptr;
   for (int d = 0; d < 3; ++d) {
     g.L[d] = box[d];
     g.invL[d] = 1.0 / box[d];
     g.halfL[d] = 0.5 * box[d];
-    g.n[d] = cells[d];
-    double cs = (box[d] - 0.0) / (double)cells[d];
+    g.nh[d] = cells[d];
+    // CellGrid: cellSize = (myRight - myLeft) / n, invCellSize = 1 / cellSize (src/storage/CellGrid.cpp:52-56)
+    g.hics[d] = 1.0 / ((box[d] - 0.0) / (double)cells[d]);
+    g.n[d] = cells[d] / hci;
+    double cs = (box[d] - 0.0) / (double)g.n[d];
     g.ics[d] = 1.0 / cs;
     int kbits = 0;
-    while (kbits < 16 && (double)cells[d] / (double)(1 << (kbits + 1)) >= 6.0) ++kbits;
+    while (kbits < 16 && (double)g.n[d] / (double)(1 << (kbits + 1)) >= 6.0) ++kbits;
     const double scale = std::ldexp(1.0, 32 + kbits);
     g.qinv[d] = scale / box[d];
     g.qs[d] = (float)(box[d] / scale);
@@ -1159,14 +1174,22 @@ int espp_device_count(int32_t* count) {
   });
 }
 
-int espp_create(const double box[3], const int32_t cell_grid[3], double skin, uint64_t seed, int32_t precision,
-                espp_sim** out) {
+int espp_create(const double box[3], const int32_t cell_grid[3], int32_t half_cell_int, double skin, uint64_t seed,
+                int32_t precision, espp_sim** out) {
   return guarded([&] {
     need(out != nullptr && box != nullptr && cell_grid != nullptr, "NULL argument");
     *out = nullptr;
+    need(half_cell_int >= 1 && half_cell_int <= 8, "halfCellInt must be between 1 and 8");
     for (int d = 0; d < 3; ++d) {
       need(box[d] > 0.0 && std::isfinite(box[d]), "box lengths must be positive and finite");
       need(cell_grid[d] > 0, "cell grid must be positive");
+      // one cell (or fewer than 2 h half-cells) in a dimension would meet a particle's own
+      // periodic images in the cell stencil: the reference's DomainDecomposition raises the
+      // grid to 2 there (src/storage/DomainDecomposition.py:106-111)
+      if (cell_grid[d] / half_cell_int < 2)
+        fail(ESPP_ERR_ARG, "cell grid " + std::to_string(cell_grid[d]) + " in direction " + std::to_string(d) +
+                               " is below 2 x halfCellInt (" + std::to_string(half_cell_int) +
+                               "): the box must hold two cells of at least cutoff + skin");
     }
     need(skin >= 0.0, "skin must be non-negative");
     need(precision == ESPP_FP64 || precision == ESPP_MIXED, "precision must be ESPP_FP64 or ESPP_MIXED");
@@ -1177,7 +1200,7 @@ int espp_create(const double box[3], const int32_t cell_grid[3], double skin, ui
     }
     auto* s = new espp_sim();
     try {
-      set_geom(s->g, box, cell_grid);
+      set_geom(s->g, box, cell_grid, half_cell_int);
       s->skin = skin;
       s->seed = seed;
       s->precision = precision;
@@ -1349,49 +1372,78 @@ int espp_get_particles(espp_sim* s, int32_t field, double* out) {
   });
 }
 
+// Slots of m particles given by id (the cell order must be current: slot_of_id is
+// written by the binning).  Fails with the reference's message for an id that
+// does not exist.
+static void slots_of_ids(espp_sim* s, int32_t field, int64_t m, const int32_t* ids, double* out_values,
+                         DBuf<int>& dslots) {
+  s->ensure_sorted();
+  DBuf<int> dids;
+  DBuf<double> dout;
+  dids.resize((size_t)m);
+  dslots.resize((size_t)m);
+  dout.resize((size_t)m * 3);
+  hip_check(hipMemcpyAsync(dids.p, ids, (size_t)m * sizeof(int), hipMemcpyHostToDevice, s->stream), "id upload");
+  launch_gather_ids(s->stream, (int)m, dids.p, s->slot_of_id.p, s->nid, field < 0 ? 0 : field, s->pos.p, s->vel.p,
+                    s->fx.p, s->fy.p, s->fz.p, s->img.p, dout.p, dslots.p);
+  std::vector<int> slots = download(dslots, (size_t)m, s->stream);
+  for (int64_t k = 0; k < m; ++k)
+    if (slots[(size_t)k] < 0) fail(ESPP_ERR_ARG, "particle " + std::to_string(ids[k]) + " does not exist");
+  if (out_values) {
+    std::vector<double> v = download(dout, (size_t)m * 3, s->stream);
+    std::memcpy(out_values, v.data(), v.size() * sizeof(double));
+  }
+}
+
+int espp_get_particles_by_id(espp_sim* s, int32_t field, int64_t m, const int32_t* ids, double* out) {
+  return guarded([&] {
+    need(s && (m == 0 || (ids && out)), "NULL argument");
+    need(m >= 0, "negative count");
+    need(field >= 0 && field <= ESPP_FIELD_MASS, "unknown particle field");
+    if (m == 0) return;
+    DBuf<int> slots;
+    std::vector<double> v((size_t)m * 3);
+    slots_of_ids(s, field, m, ids, v.data(), slots);
+    const int width = (field == ESPP_FIELD_TYPE || field == ESPP_FIELD_MASS) ? 1 : 3;
+    for (int64_t k = 0; k < m; ++k)
+      for (int c = 0; c < width; ++c) out[(size_t)k * width + c] = v[(size_t)k * 3 + c];
+  });
+}
+
 int espp_set_particles(espp_sim* s, int32_t field, int64_t m, const int32_t* ids, const double* values) {
   return guarded([&] {
     need(s && (m == 0 || (ids && values)), "NULL argument");
     need(m >= 0, "negative count");
+    need(field == ESPP_FIELD_POS || field == ESPP_FIELD_VEL || field == ESPP_FIELD_TYPE || field == ESPP_FIELD_MASS,
+         "field cannot be set");
     if (m == 0) return;
-    size_t n = (size_t)s->n;
-    s->sync();
-    std::vector<int> dev_ids = download(s->id, n, s->stream);
-    std::vector<int64_t> slot_of((size_t)std::max(s->nid, 1), -1);
-    for (size_t k = 0; k < n; ++k) slot_of[(size_t)dev_ids[k]] = (int64_t)k;
-    auto slot = [&](int32_t pid) {
-      if (pid < 0 || pid >= s->nid || slot_of[(size_t)pid] < 0)
-        fail(ESPP_ERR_ARG, "particle " + std::to_string(pid) + " does not exist");
-      return (size_t)slot_of[(size_t)pid];
-    };
-    if (field == ESPP_FIELD_POS || field == ESPP_FIELD_TYPE) {
-      std::vector<double4> p = download(s->pos, n, s->stream);
-      for (int64_t k = 0; k < m; ++k) {
-        double4& q = p[slot(ids[k])];
-        if (field == ESPP_FIELD_POS) {
-          q.x = values[3 * k]; q.y = values[3 * k + 1]; q.z = values[3 * k + 2];
-          s->coord_bound = std::max(s->coord_bound, std::max(std::fabs(q.x), std::max(std::fabs(q.y), std::fabs(q.z))));
-        }
-        else {
-          need(values[k] >= 0.0, "particle types must be non-negative");
-          q.w = std::floor(values[k]);
-          s->maxtype = std::max(s->maxtype, (int)q.w);
+    const int width = (field == ESPP_FIELD_TYPE || field == ESPP_FIELD_MASS) ? 1 : 3;
+    std::vector<double> v(values, values + (size_t)m * width);
+    for (int64_t k = 0; k < m; ++k) {
+      if (field == ESPP_FIELD_TYPE) {
+        need(v[(size_t)k] >= 0.0, "particle types must be non-negative");
+        v[(size_t)k] = std::floor(v[(size_t)k]);
+        s->maxtype = std::max(s->maxtype, (int)v[(size_t)k]);
+      } else if (field == ESPP_FIELD_MASS) {
+        need(v[(size_t)k] > 0.0, "particle mass must be positive");
+      } else if (field == ESPP_FIELD_POS) {
+        for (int c = 0; c < 3; ++c) {
+          const double x = v[(size_t)k * 3 + c];
+          if (!(x == x)) fail(ESPP_ERR_NAN, "particle position is nan");
+          s->coord_bound = std::max(s->coord_bound, std::fabs(x));
         }
       }
-      upload_bytes(s->pos.p, p.data(), n * sizeof(double4), s->stream, "upload");
-      // re-binning refreshes the cell order, the fixed-point mirror and the lists
-      s->sorted = false;
-    } else if (field == ESPP_FIELD_VEL || field == ESPP_FIELD_MASS) {
-      std::vector<double4> v = download(s->vel, n, s->stream);
-      for (int64_t k = 0; k < m; ++k) {
-        double4& q = v[slot(ids[k])];
-        if (field == ESPP_FIELD_VEL) { q.x = values[3 * k]; q.y = values[3 * k + 1]; q.z = values[3 * k + 2]; }
-        else { need(values[k] > 0.0, "particle mass must be positive"); q.w = values[k]; }
-      }
-      upload_bytes(s->vel.p, v.data(), n * sizeof(double4), s->stream, "upload");
-    } else {
-      fail(ESPP_ERR_ARG, "field cannot be set");
     }
+    DBuf<int> slots;
+    slots_of_ids(s, -1, m, ids, nullptr, slots);
+    DBuf<double> dv;
+    dv.resize(v.size());
+    upload_bytes(dv.p, v.data(), v.size() * sizeof(double), s->stream, "upload");
+    launch_scatter_slots(s->stream, (int)m, slots.p, field, dv.p, s->pos.p, s->vel.p);
+    s->sync();
+    // a new position or type: re-binning refreshes the cell order, the fixed-point
+    // mirror and the lists
+    if (field == ESPP_FIELD_POS || field == ESPP_FIELD_TYPE) s->sorted = false;
   });
 }
 
@@ -1757,6 +1809,38 @@ int espp_add_interaction(espp_sim* s, int32_t ia) {
     get_ia(s, ia);
     for (int k : s->active) need(k != ia, "interaction already added");
     s->active.push_back(ia);
+  });
+}
+
+int espp_remove_interaction(espp_sim* s, int32_t position) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    if (position < 0 || position >= (int32_t)s->active.size())
+      fail(ESPP_ERR_ARG, "Probably you are trying to remove the interaction " + std::to_string(position) +
+                             " which does not exist. Check your script!");
+    s->active.erase(s->active.begin() + position);
+  });
+}
+
+int espp_fene_set(espp_sim* s, int32_t ia, double K, double r0, double rMax, double cutoff, double shift) {
+  return guarded([&] {
+    InteractionState& a = get_ia(s, ia);
+    need(a.kind == 1, "not a FixedPairList FENE interaction");
+    a.fp.K = K;
+    a.fp.r0 = r0;
+    a.fp.rMax = rMax;
+    a.fp.rMaxSqr = rMax * rMax;
+    a.fp.cutoffSqr = cutoff * cutoff;
+    a.fp.shift = shift;
+  });
+}
+
+int espp_ia_set_fpl(espp_sim* s, int32_t ia, int32_t fpl) {
+  return guarded([&] {
+    InteractionState& a = get_ia(s, ia);
+    need(a.kind == 1 || a.kind == 3, "not a FixedPairList interaction");
+    need(fpl >= 0 && fpl < (int32_t)s->fpls.size(), "no such fixed pair list");
+    a.list = fpl;
   });
 }
 

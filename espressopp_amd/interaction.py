@@ -243,14 +243,28 @@ class FixedPairListFENE(Interaction):
     _bond_type = 2   # Pair
 
     def __init__(self, system, bondlist, potential):
+        if not isinstance(potential, FENE):
+            raise TypeError("FixedPairListFENE needs a FENE potential")
         self._fpl = bondlist
         self._h = bondlist._h
         self._potential = potential
         p = potential
         self._id = self._h.out_int("espp_fene_create", bondlist._id, p.K, p.r0, p.rMax, p.cutoff, p.shift)
+        potential._owners.append(self)
 
     def setPotential(self, potential):
-        raise NotImplementedError("FENE parameters are fixed at construction on the GPU storage")
+        """FixedPairListInteractionTemplate::setPotential (src/interaction/FENE.py:114-116)."""
+        if not isinstance(potential, FENE):
+            raise TypeError("FixedPairListFENE needs a FENE potential")
+        if self in self._potential._owners:
+            self._potential._owners.remove(self)
+        self._potential = potential
+        potential._owners.append(self)
+        self._potential_changed(potential)
+
+    def _potential_changed(self, p):
+        if p is self._potential:
+            self._h.call("espp_fene_set", self._id, p.K, p.r0, p.rMax, p.cutoff, p.shift)
 
     def getPotential(self):
         return self._potential
@@ -259,7 +273,11 @@ class FixedPairListFENE(Interaction):
         return self._fpl
 
     def setFixedPairList(self, fpl):
-        raise NotImplementedError("the bond list is fixed at construction on the GPU storage")
+        """FixedPairListInteractionTemplate::setFixedPairList (src/interaction/FENE.py:122-124)."""
+        if fpl._h is not self._h:
+            raise RuntimeError("the bond list belongs to another system")
+        self._h.call("espp_ia_set_fpl", self._id, fpl._id)
+        self._fpl = fpl
 
     def getMaxCutoff(self):
         return self._potential.cutoff if not math.isinf(self._potential.cutoff) else 0.0
@@ -398,6 +416,12 @@ class FixedPairListTabulated(Interaction):
 
     def getFixedPairList(self):
         return self._fpl
+
+    def setFixedPairList(self, fpl):
+        if fpl._h is not self._h:
+            raise RuntimeError("the bond list belongs to another system")
+        self._h.call("espp_ia_set_fpl", self._id, fpl._id)
+        self._fpl = fpl
 
     def getMaxCutoff(self):
         return self._potential.cutoff if not math.isinf(self._potential.cutoff) else 0.0

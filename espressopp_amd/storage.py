@@ -12,6 +12,10 @@ from ._vec3 import Int3D, Real3D, toInt3DFromVector
 
 _PROPS = ("id", "pos", "type", "mass", "v", "f", "q", "modepos", "modemom", "fm", "radius",
           "fradius", "vradius", "varmass", "adrat", "lambda_adr", "lambda_adrd", "state", "pib")
+# the properties the GPU storage keeps; the others of the reference's Particle
+# (charge, radius, AdResS and path-integral fields, ...) belong to no part of this
+# path and are refused rather than dropped
+_STORED = ("id", "pos", "type", "mass", "v")
 
 
 class Particle:
@@ -22,12 +26,11 @@ class Particle:
         self.storage = storage
 
     def _row(self, field):
-        h = self.storage._h
-        ids = h.ids()
-        k = int(np.searchsorted(ids, self.pid))
-        if k >= len(ids) or ids[k] != self.pid:
-            raise RuntimeError("particle %d does not exist" % self.pid)
-        return h.get(field)[k]
+        # one particle by id through the device's id -> slot map: O(1) in the particle count
+        try:
+            return self.storage._h.get_by_id(field, [self.pid])[0]
+        except ValueError:
+            raise RuntimeError("particle %d does not exist" % self.pid) from None
 
     @property
     def id(self):
@@ -83,8 +86,7 @@ class DomainDecomposition:
             # NodeGridMismatch (src/storage/DomainDecomposition.cpp:94-97): one GPU owns the box
             raise RuntimeError("node grid %s does not match the number of GPU ranks (1)" % (tuple(nodeGrid),))
         cellGrid = Int3D(2, 2, 2) if cellGrid == "auto" else toInt3DFromVector(cellGrid)
-        if halfCellInt not in ("auto", 1):
-            raise NotImplementedError("only halfCellInt=1 is supported")
+        halfCellInt = 1 if halfCellInt == "auto" else int(halfCellInt)
         if not nocheck:
             for k in range(3):
                 if nodeGrid[k] * cellGrid[k] == 1:
@@ -93,9 +95,10 @@ class DomainDecomposition:
         self.system = system
         self.nodeGrid = nodeGrid
         self.cellGrid = cellGrid
+        self.halfCellInt = halfCellInt
         seed = system.rng.get_seed() if system.rng is not None else 12345
         precision = {"fp64": _lib.ESPP_FP64, "mixed": _lib.ESPP_MIXED}[system.precision]
-        self._h = _lib.Handle(system.bc.boxL, cellGrid, system.skin, seed, precision)
+        self._h = _lib.Handle(system.bc.boxL, cellGrid, system.skin, seed, precision, halfCellInt)
 
     def getCellGrid(self):
         return Int3D(self.cellGrid)
@@ -118,6 +121,10 @@ class DomainDecomposition:
             if key not in _PROPS:
                 raise SyntaxError("unknown particle property: %s" % p)
             idx[key] = k
+        unstored = [p for p in idx if p not in _STORED]
+        if unstored:
+            raise NotImplementedError("particle properties %s are not stored by the GPU storage (it keeps %s)"
+                                      % (", ".join(unstored), ", ".join(_STORED)))
         if "id" not in idx:
             raise Exception("particle property id is mandatory")
         if "pos" not in idx:

@@ -92,7 +92,35 @@ class System:
         return {k: self.getInteraction(v) for k, v in self._interaction2id.items()}
 
     def removeInteraction(self, number):
-        raise NotImplementedError("removing interactions is not supported by the GPU storage")
+        """System::removeInteraction (src/System.cpp:108-135): drop the interaction at
+        `number` of the short-range list and recompute maxCutoff if it held the largest."""
+        if not 0 <= number < len(self._interactions):
+            raise RuntimeError("Probably you are trying to remove the interaction %d which does not exist. "
+                               "Check your script!" % number)
+        ia = self._interactions[number]
+        self._handle().call("espp_remove_interaction", int(number))
+        del self._interactions[number]
+        ia._system_attached = None
+        try:
+            cut = ia.getMaxCutoff()
+        except AttributeError:
+            cut = 0.0
+        if cut >= self.maxCutoff:
+            self.maxCutoff = 0.0
+            for other in self._interactions:
+                try:
+                    self.maxCutoff = max(self.maxCutoff, other.getMaxCutoff())
+                except AttributeError:
+                    pass
+
+    def removeInteractionByName(self, name):
+        """src/System.py:150-161: remove by name and renumber the later names."""
+        if name not in self._interaction2id:
+            raise RuntimeError("Interaction {} not found".format(name))
+        iid = self._interaction2id[name]
+        self.removeInteraction(iid)
+        self._interaction2id = {k: v if v < iid else v - 1 for k, v in self._interaction2id.items() if v != iid}
+        self._interaction_pid = max(self._interaction2id.values()) + 1 if self._interaction2id else 0
 
     def setTrace(self, switch):
         pass

@@ -57,12 +57,14 @@ int espp_device_count(int32_t *count);
 
 /* System + OrthorhombicBC + storage::DomainDecomposition for one node.
  * Replaces System() (src/System.cpp:206-226), bc::OrthorhombicBC (src/bc/OrthorhombicBC.cpp:35-50)
- * and DomainDecomposition::DomainDecomposition / createCellGrid / initCellInteractions
- * (src/storage/DomainDecomposition.cpp:70-170,325-374).  cell_grid is the inner
- * cell grid (tools.decomp.cellGrid), skin is System.skin, seed the RNG seed.
- * precision: ESPP_FP64 or ESPP_MIXED. */
-int espp_create(const double box[3], const int32_t cell_grid[3], double skin, uint64_t seed,
-                int32_t precision, espp_sim **out);
+ * and DomainDecomposition::DomainDecomposition(system, nodeGrid, cellGrid, halfCellInt) /
+ * createCellGrid / initCellInteractions (src/storage/DomainDecomposition.cpp:70-170,325-374).
+ * cell_grid is the inner cell grid (tools.decomp.cellGrid(box, nodeGrid, rc, skin,
+ * halfCellInt)), half_cell_int the halfCellInt (1 full cells, 2 half cells, ...; 1..8),
+ * skin is System.skin, seed the RNG seed.  cell_grid / half_cell_int must be at least 2 in
+ * every dimension (ESPP_ERR_ARG otherwise).  precision: ESPP_FP64 or ESPP_MIXED. */
+int espp_create(const double box[3], const int32_t cell_grid[3], int32_t half_cell_int, double skin,
+                uint64_t seed, int32_t precision, espp_sim **out);
 
 /* Frees the handle and all its device memory. */
 int espp_destroy(espp_sim *sim);
@@ -92,8 +94,14 @@ int espp_get_ids(espp_sim *sim, int32_t *out);
  * particle for POS/VEL/FORCE/IMAGE and 1 for TYPE/MASS. */
 int espp_get_particles(espp_sim *sim, int32_t field, double *out);
 
+/* Properties of n particles given by id (Storage::lookupRealParticle + the
+ * ParticleAccess getters, src/storage/Storage.cpp:1037-1050, src/Particle.py), in the
+ * order of ids: 3 doubles per particle for POS/VEL/FORCE/IMAGE, 1 for TYPE/MASS.  Costs
+ * O(n), not O(N).  Fails (ESPP_ERR_ARG) for an id that does not exist. */
+int espp_get_particles_by_id(espp_sim *sim, int32_t field, int64_t n, const int32_t *ids, double *out);
+
 /* Set a property of n particles given by id (Particle.pos/.v/.type/.mass setters,
- * src/Particle.py).  values: 3 per particle for POS/VEL, 1 for TYPE/MASS. */
+ * src/Particle.py).  values: 3 per particle for POS/VEL, 1 for TYPE/MASS.  Costs O(n). */
 int espp_set_particles(espp_sim *sim, int32_t field, int64_t n, const int32_t *ids,
                        const double *values);
 
@@ -163,6 +171,18 @@ int espp_tabbond_create(espp_sim *sim, int32_t fpl, int32_t itype, int64_t n, co
 /* System::addInteraction (src/System.cpp:75-85, bound at :221): appends the
  * interaction to the short-range list whose order is the force summation order. */
 int espp_add_interaction(espp_sim *sim, int32_t ia);
+
+/* System::removeInteraction (src/System.cpp:108-135, bound at :222): removes the
+ * interaction at `position` of the short-range list (0-based, addInteraction order). */
+int espp_remove_interaction(espp_sim *sim, int32_t position);
+
+/* FixedPairListFENE::setPotential(FENE(K, r0, rMax, cutoff, shift))
+ * (src/interaction/FENE.py:114-116, FixedPairListInteractionTemplate::setPotential). */
+int espp_fene_set(espp_sim *sim, int32_t ia, double K, double r0, double rMax, double cutoff, double shift);
+
+/* setFixedPairList of a FixedPairList interaction, FENE or Tabulated
+ * (src/interaction/FENE.py:122-124, FixedPairListInteractionTemplate::setFixedPairList). */
+int espp_ia_set_fpl(espp_sim *sim, int32_t ia, int32_t fpl);
 
 /* Interaction::computeEnergy (src/interaction/Interaction.cpp:46). */
 int espp_ia_energy(espp_sim *sim, int32_t ia, double *energy);

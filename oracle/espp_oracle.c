@@ -13,7 +13,7 @@
  *       src/esutil/Grid.hpp:68-71 (linear index x + nx*(y + ny*z))
  *   - folding: src/bc/OrthorhombicBC.cpp:144-161 (tmp=floor(x*invL); img+=tmp; x-=tmp*L)
  *   - minimum image for bonds: src/bc/OrthorhombicBC.cpp:86-105
- *   - half-shell cell neighbours: src/storage/DomainDecomposition.cpp:325-374
+ *   - half-shell cell neighbours (halfCellInt stencil): src/storage/DomainDecomposition.cpp:325-374
  *       (useForAllPairs = cell2Idx < cellIdx; the list uses the others)
  *   - ghost shift: src/storage/DomainDecomposition.cpp:1320-1343 with
  *       src/storage/NodeGrid.cpp:117-118 (right ghost frame = left cells + L)
@@ -116,6 +116,9 @@ typedef struct {
 typedef struct or_sim {
     double L[3], invL[3], halfL[3];
     int n[3];
+    int hci;                       /* halfCellInt: cell neighbours within +-hci cells */
+    int nstencil;                  /* neighbour offsets with a larger framed index */
+    int stencil[2456][3];
     double cs[3], ics[3];
     int64_t ncell;
     double skin;
@@ -200,6 +203,8 @@ void or_langevin_uniforms(uint64_t seed, uint64_t call, int32_t pid, double *u3)
     langevin_uniforms(seed, call, pid, u3);
 }
 
+static void make_stencil(or_sim *s);
+
 /* ---------------------------------------------------------------- lifecycle */
 or_sim *or_create(const double *L, const int *cells, double skin, uint64_t seed) {
     or_sim *s = (or_sim *)calloc(1, sizeof(or_sim));
@@ -214,6 +219,8 @@ or_sim *or_create(const double *L, const int *cells, double skin, uint64_t seed)
         s->ics[d] = 1.0 / s->cs[d];
     }
     s->ncell = (int64_t)cells[0] * cells[1] * cells[2];
+    s->hci = 1;
+    make_stencil(s);
     s->skin = skin;
     s->seed = seed;
     s->dt = 0.005;
@@ -221,6 +228,16 @@ or_sim *or_create(const double *L, const int *cells, double skin, uint64_t seed)
     s->nthreads = 1;
     s->cstart = (int64_t *)calloc((size_t)s->ncell + 1, sizeof(int64_t));
     return s;
+}
+
+/* DomainDecomposition's halfCellInt: the cells given to or_create are the fine
+ * grid (tools.decomp.cellGrid(..., halfCellInt)), neighbours within +-h cells. */
+int or_set_halfcell(or_sim *s, int h) {
+    if (h < 1 || h > 8) return -1;
+    for (int d = 0; d < 3; ++d) if (s->n[d] < 2 * h) return -1;
+    s->hci = h;
+    make_stencil(s);
+    return 0;
 }
 
 static void tab_free(or_tab *t) {
@@ -361,16 +378,24 @@ static void bin_particles(or_sim *s) {
     build_cell_lists(s);
 }
 
-/* half-shell offsets in the order DomainDecomposition::initCellInteractions
- * enumerates neighbours (z outer, then y, then x), keeping those whose framed
- * linear index is larger than the cell's own (useForAllPairs == false). */
-static const int HALF_SHELL[13][3] = {
-    {1, 0, 0},
-    {-1, 1, 0}, {0, 1, 0}, {1, 1, 0},
-    {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
-    {-1, 0, 1}, {0, 0, 1}, {1, 0, 1},
-    {-1, 1, 1}, {0, 1, 1}, {1, 1, 1},
-};
+/* Half-shell offsets in the order DomainDecomposition::initCellInteractions
+ * enumerates the neighbours within +-halfCellInt cells (z outer, then y, then x;
+ * src/storage/DomainDecomposition.cpp:342-358), keeping those whose framed
+ * linear index is larger than the cell's own (useForAllPairs == false): the
+ * lexicographically positive offsets.  For halfCellInt 1 these are 13 cells. */
+static void make_stencil(or_sim *s) {
+    const int h = s->hci;
+    s->nstencil = 0;
+    for (int p = -h; p <= h; ++p)
+        for (int q = -h; q <= h; ++q)
+            for (int r = -h; r <= h; ++r)
+                if (p > 0 || (p == 0 && (q > 0 || (q == 0 && r > 0)))) {
+                    s->stencil[s->nstencil][0] = r;
+                    s->stencil[s->nstencil][1] = q;
+                    s->stencil[s->nstencil][2] = p;
+                    s->nstencil++;
+                }
+}
 
 static int is_excluded(const or_vl *v, int32_t a, int32_t b) {
     if (!v->nexcl) return 0;
@@ -420,8 +445,8 @@ static void vl_build_cell(const or_sim *s, const or_vl *v, int64_t c, pairbuf *o
             if (t > out->max_type) out->max_type = t;
         }
         /* neighbour loop over the half-shell buffer (ghost copies carry p + shift) */
-        for (int k = 0; k < 13; ++k) {
-            int q[3] = {cx + HALF_SHELL[k][0], cy + HALF_SHELL[k][1], cz + HALF_SHELL[k][2]};
+        for (int k = 0; k < s->nstencil; ++k) {
+            int q[3] = {cx + s->stencil[k][0], cy + s->stencil[k][1], cz + s->stencil[k][2]};
             int sh[3];
             for (int d = 0; d < 3; ++d) {
                 sh[d] = 0;

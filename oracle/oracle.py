@@ -36,6 +36,7 @@ def lib():
             "or_destroy": (None, [vp]),
             "or_last_error": (ctypes.c_char_p, [vp]),
             "or_set_threads": (None, [vp, i32]),
+            "or_set_halfcell": (i32, [vp, i32]),
             "or_set_skin": (None, [vp, dbl]),
             "or_set_dt": (None, [vp, dbl]),
             "or_get_step": (i64, [vp]),
@@ -156,13 +157,15 @@ def tab_eval(itype, r, e, f, at, cutoff=float("inf")):
 class OracleSim:
     """One FP64 system: box, cell grid, skin, particles, lists, interactions."""
 
-    def __init__(self, box, cells, skin, seed=12345, threads=1):
+    def __init__(self, box, cells, skin, seed=12345, threads=1, half_cell_int=1):
         L = (ctypes.c_double * 3)(*[float(b) for b in box])
         C = (ctypes.c_int * 3)(*[int(c) for c in cells])
         self._h = lib().or_create(L, C, float(skin), int(seed))
         if not self._h:
             raise MemoryError("or_create failed")
         lib().or_set_threads(self._h, int(threads))
+        if half_cell_int != 1 and lib().or_set_halfcell(self._h, int(half_cell_int)) != 0:
+            raise ValueError("halfCellInt %r does not fit the cell grid %r" % (half_cell_int, tuple(cells)))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -42,7 +42,9 @@ def test_create_fails_loudly_without_device_or_with_bad_arguments():
     lib = _lib.load()
     h = ctypes.c_void_p()
     box = (ctypes.c_double * 3)(10.0, 10.0, 10.0)
-    cells = (ctypes.c_int32 * 3)(0, 3, 3)
-    rc = lib.espp_create(box, cells, 0.3, 1, 0, ctypes.byref(h))
-    assert rc != 0 and not h.value
-    assert b"cell grid" in lib.espp_last_error()
+    for grid, hci in (((0, 3, 3), 1), ((1, 3, 3), 1), ((3, 3, 3), 2), ((4, 4, 4), 0)):
+        cells = (ctypes.c_int32 * 3)(*grid)
+        rc = lib.espp_create(box, cells, hci, 0.3, 1, 0, ctypes.byref(h))
+        assert rc == -1 and not h.value
+        msg = lib.espp_last_error()
+        assert b"cell grid" in msg or b"halfCellInt" in msg

@@ -106,6 +106,73 @@ def test_pairs_borderline_distances(gpu):
     assert pair_set(vl.pairs_array()) == pair_set(ref.vl_pairs(rvl))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("cells", [(2, 2, 2), (3, 3, 3)])
+def test_forces_borderline_cutoff(gpu, precision, cells):
+    # pairs at rc +- a few ulp, inside the box and across its periodic boundary: each falls on
+    # the reference's side of the cut test (|d|^2 > rc^2, d rounded as the reference forms it),
+    # untiled (2 cells) and tiled (3 cells) kernels alike
+    box = (10.0, 10.0, 10.0)
+    rc, skin = 2.5, 0.3
+    pos = []
+    for k in range(30):
+        y = 0.2 + 0.32 * k
+        x0 = 0.05 + 0.001 * k
+        for m, dz in enumerate((-2, -1, 0, 1, 2)):
+            z = 0.3 + 2.0 * m
+            d = rc + dz * 4e-16
+            pos.append([x0, y, z])
+            pos.append([x0 - d + 10.0 if k % 2 == 0 else x0 + d, y, z])
+    pos = np.asarray(pos)
+    system, vl, ia, ref, rvl, ria = build_lj(pos, box, cells, rc, skin, precision, shift=0.0)
+    espressopp.integrator.VelocityVerlet(system).run(0)
+    ref.calc_forces()
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert force_rel_err(system.storage.getParticleArrays()["f"], ref.get("f"), 0.1).max() <= tol
+    assert ia.computeEnergy() == pytest.approx(ref.energy(ria), rel=1e-11 if precision == "fp64" else 1e-6)
+
+
+def _half_cell_border_positions():
+    # pairs at (rc+skin) +- a few ulp across the x boundary whose partners sit in the same
+    # full cell but in neighbouring half cells in y: the reference's orientation of d (and
+    # so the rounding of the list test) depends on halfCellInt for them
+    cut = 2.8
+    pos = []
+    for k in range(40):
+        z = 0.2 + 0.24 * k
+        x0 = 0.05 + 0.001 * k
+        for m, dz in enumerate(range(-6, 7)):
+            yi = 1.40 + 0.002 * m
+            yj = yi + 0.05 + 0.001 * m
+            d = cut + dz * 2e-16
+            dx = math.sqrt(d * d - (yj - yi) ** 2)
+            pos.append([x0, yi, z])
+            pos.append([x0 - dx + 10.0, yj, z])
+    return np.asarray(pos)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("hci,cells", [(1, (3, 3, 3)), (2, (7, 7, 7)), (3, (10, 10, 10))])
+def test_pairs_half_cells(gpu, precision, hci, cells):
+    # DomainDecomposition(..., halfCellInt): the GPU bins on cells of at least rc+skin and
+    # takes each pair's orientation from the caller's grid; pair sets equal the oracle's on
+    # the same half-cell grid, including borderline pairs whose orientation changes with it
+    box, rc, skin = (10.0, 10.0, 10.0), 2.5, 0.3
+    for pos in (_half_cell_border_positions(), np.random.default_rng(70 + hci).uniform(0, 10.0, (3000, 3))):
+        system = espressopp.System(precision=precision)
+        system.rng = espressopp.esutil.RNG(5)
+        system.bc = espressopp.bc.OrthorhombicBC(system.rng, box)
+        system.skin = skin
+        system.storage = espressopp.storage.DomainDecomposition(system, (1, 1, 1), cells, hci)
+        system.storage.addParticlesArray(np.arange(len(pos), dtype=np.int32), pos)
+        system.storage.decompose()
+        vl = espressopp.VerletList(system, cutoff=rc)
+        ref = OracleSim(box, cells, skin, half_cell_int=hci)
+        ref.add_particles(np.arange(len(pos)), pos)
+        ref.decompose()
+        assert pair_set(vl.pairs_array()) == pair_set(ref.vl_pairs(ref.vl_create(rc)))
+
+
 def test_forces_fp64_fixture(gpu):
     pid, pos, _, box = read_fixture()
     rc, skin = 2.5, 0.3
@@ -278,8 +345,10 @@ def test_langevin_excluded_particle_does_not_move(gpu):
     assert all(b != a for b, a in zip(before[3:], after[3:]))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_fene_wca_melt(gpu, precision):
+@pytest.mark.parametrize("precision,r0", [("fp64", 0.0), ("mixed", 0.0), ("fp64", 0.2), ("mixed", 0.2)])
+def test_fene_wca_melt(gpu, precision, r0):
+    # r0 > 0 takes FENE's other force branch (src/interaction/FENE.hpp:100-104) and arrives
+    # through FixedPairListFENE.setPotential (src/interaction/FENE.py:114-116)
     pos, box, bonds = random_walk_chains(40, 50, 0.85, seed=7)
     rc, skin = 2 ** (1 / 6), 0.3
     cells = cells_for(box, rc, skin)
@@ -295,6 +364,8 @@ def test_fene_wca_melt(gpu, precision):
     fpl.addBonds(bonds.tolist())
     fene = espressopp.interaction.FixedPairListFENE(system, fpl, espressopp.interaction.FENE(30.0, 0.0, 1.5))
     system.addInteraction(fene)
+    if r0 > 0.0:
+        fene.setPotential(espressopp.interaction.FENE(30.0, r0, 1.5))
     espressopp.integrator.VelocityVerlet(system).run(0)
     ref = OracleSim(box, cells, skin)
     ref.add_particles(np.arange(n), pos)
@@ -303,7 +374,7 @@ def test_fene_wca_melt(gpu, precision):
     ref.lj_set(ria, 0, 0, 1.0, 1.0, rc, "auto")
     rfpl = ref.fpl_create()
     ref.fpl_add(rfpl, bonds)
-    rfene = ref.fene_create(rfpl, 30.0, 0.0, 1.5)
+    rfene = ref.fene_create(rfpl, 30.0, r0, 1.5)
     ref.calc_forces()
     f = system.storage.getParticleArrays()["f"]
     tol = 1e-11 if precision == "fp64" else 1e-5
@@ -312,6 +383,67 @@ def test_fene_wca_melt(gpu, precision):
     assert fene.computeEnergy() == pytest.approx(ref.energy(rfene), rel=etol)
     assert wca.computeEnergy() == pytest.approx(ref.energy(ria), rel=etol)
     assert fene.computeVirial() == pytest.approx(ref.virial(rfene)[0], rel=etol)
+
+
+def test_remove_interaction_and_bond_list_swap(gpu):
+    # System.removeInteraction (src/System.cpp:108-135) and FixedPairListFENE.setFixedPairList
+    # (src/interaction/FENE.py:122-124): forces follow the remaining interactions and the new bonds
+    pos, box, bonds = random_walk_chains(10, 30, 0.85, seed=17)
+    rc, skin = 2 ** (1 / 6), 0.3
+    cells = cells_for(box, rc, skin)
+    n = len(pos)
+    system = gpu_system(box, cells, skin, "fp64")
+    system.storage.addParticlesArray(np.arange(n), pos)
+    system.storage.decompose()
+    vl = espressopp.VerletList(system, cutoff=rc)
+    wca = espressopp.interaction.VerletListLennardJones(vl)
+    wca.setPotential(0, 0, espressopp.interaction.LennardJones(1.0, 1.0, rc, "auto"))
+    fpl = espressopp.FixedPairList(system.storage)
+    fpl.addBonds(bonds.tolist())
+    half = espressopp.FixedPairList(system.storage)
+    half.addBonds(bonds[::2].tolist())
+    fene = espressopp.interaction.FixedPairListFENE(system, fpl, espressopp.interaction.FENE(30.0, 0.0, 1.5))
+    system.addInteraction(wca)
+    system.addInteraction(fene)
+    system.removeInteraction(0)
+    assert system.getNumberOfInteractions() == 1 and system.getInteraction(0) is fene
+    fene.setFixedPairList(half)
+    espressopp.integrator.VelocityVerlet(system).run(0)
+    ref = OracleSim(box, cells, skin)
+    ref.add_particles(np.arange(n), pos)
+    ref.decompose()
+    rfpl = ref.fpl_create()
+    ref.fpl_add(rfpl, bonds[::2])
+    rfene = ref.fene_create(rfpl, 30.0, 0.0, 1.5)
+    ref.calc_forces()
+    f = system.storage.getParticleArrays()["f"]
+    assert force_rel_err(f, ref.get("f"), 0.1).max() <= 1e-12
+    assert fene.computeEnergy() == pytest.approx(ref.energy(rfene), rel=1e-12)
+    with pytest.raises(RuntimeError):
+        system.removeInteraction(3)
+
+
+def test_particle_access_by_id(gpu):
+    # getParticle(pid) reads one particle through the device's id -> slot map; setters write one
+    pos, box = jittered_lattice(3000, 0.8, seed=3)
+    system = gpu_system(box, cells_for(box, 2.5, 0.3), 0.3, "fp64")
+    ids = np.arange(10, 3010, dtype=np.int32)[::-1].copy()
+    system.storage.addParticlesArray(ids, pos)
+    system.storage.decompose()
+    arrays = system.storage.getParticleArrays()
+    for k in (0, 17, 2999):
+        pid = int(arrays["id"][k])
+        p = system.storage.getParticle(pid)
+        assert list(p.pos) == list(arrays["pos"][k])
+        assert p.mass == 1.0 and p.type == 0
+    p = system.storage.getParticle(1234)
+    p.v = espressopp.Real3D(1.0, -2.0, 0.5)
+    p.pos = espressopp.Real3D(1.5, 2.5, 3.5)
+    assert list(p.v) == [1.0, -2.0, 0.5] and list(p.pos) == [1.5, 2.5, 3.5]
+    with pytest.raises(RuntimeError):
+        system.storage.getParticle(5).pos
+    with pytest.raises(NotImplementedError):
+        system.storage.addParticles([(9000, espressopp.Real3D(1.0, 1.0, 1.0), 0.5)], "id", "pos", "q")
 
 
 def _lattice_box(box, rho, seed, jitter=0.05):
