@@ -70,6 +70,31 @@ SIGNATURES = {
     "espp_get_run_state": [_vp, _P(_c_dbl), _P(_c_int)],
     "espp_set_run_state": [_vp, _c_dbl, _c_int],
     "espp_kinetic_tensor": [_vp, _c_u64, _vp, _vp, _vp],
+    # learned pair force
+    "espp_mlp_create": [_vp, _c_int, _vp, _P(_c_int)],
+    "espp_mlp_set_params": [_vp, _c_int, _vp, _c_i64],
+    "espp_mlp_get_params": [_vp, _c_int, _vp, _c_i64],
+    "espp_mlp_train_frame": [_vp, _c_int, _vp, _c_dbl, _P(_c_dbl)],
+    "espp_mlp_grad": [_vp, _c_int, _vp, _c_i64],
+    "espp_mlp_zero_grad": [_vp, _c_int],
+    "espp_mlp_adam": [_vp, _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl],
+    # batched replicas
+    "espp_batch_create": [_c_int, _vp, _vp, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _vp, _vp, _P(_vp)],
+    "espp_batch_destroy": [_vp],
+    "espp_batch_load": [_vp, _c_int, _vp, _vp, _vp, _vp],
+    "espp_batch_set_temperature": [_vp, _c_int, _c_dbl],
+    "espp_batch_get_temperature": [_vp, _c_int, _P(_c_dbl)],
+    "espp_batch_run": [_vp, _c_int, _c_dbl],
+    "espp_batch_energy": [_vp, _vp, _vp],
+    "espp_batch_kinetic": [_vp, _vp],
+    "espp_batch_get": [_vp, _c_int, _c_int, _vp],
+    "espp_batch_pairs": [_vp, _c_int, _vp, _c_i64, _P(_c_i64)],
+    "espp_batch_set_lpp": [_vp, _vp],
+    "espp_batch_get_lpp": [_vp, _vp],
+    "espp_batch_force_time": [_vp, _c_int, _P(_c_dbl)],
+    "espp_batch_autotune": [_vp, _c_int, _vp, _vp, _P(_c_dbl), _P(_c_dbl)],
+    "espp_batch_set_graph": [_vp, _c_int],
+    "espp_batch_stats": [_vp, _c_int, _P(_c_i64), _P(_c_i64), _P(_c_int)],
 }
 
 # instrumentation entry points (include/espp_amd_prof.h)

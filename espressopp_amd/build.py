@@ -12,7 +12,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libespp_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["kernels.hip", "sim.hip"]
+SOURCES = ["kernels.hip", "sim.hip", "batch.hip", "mlp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -21,50 +21,12 @@
 #include <type_traits>
 
 #include "espp_dev.h"
+#include "espp_devfn.h"
 #include "kernels.h"
 
 namespace espp {
 
 // ------------------------------------------------------------------ helpers
-
-// |d|^2 summed left to right without contraction (Real3D::sqr, src/Real3D.hpp)
-__device__ __forceinline__ double sqr_nc(double x, double y, double z) {
-#pragma clang fp contract(off)
-  return x * x + y * y + z * z;
-}
-
-__device__ __forceinline__ uint32_t fixq(double x, double qinv) {
-  // position -> 32-bit fixed point with L <-> 2^32; the int64 detour makes the
-  // mapping modular, so unfolded positions wrap onto the same periodic image.
-  return (uint32_t)(unsigned long long)(long long)__builtin_rint(x * qinv);
-}
-
-__device__ __forceinline__ float rcp_nr(float x) {
-  // v_rcp_f32 followed by one Newton step: ~0.5 ulp, three VALU ops
-  float r = __builtin_amdgcn_rcpf(x);
-  return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
-}
-
-__device__ __forceinline__ double rcp_nr(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-  return r;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
 
 template <bool CUBIC>
 __device__ __forceinline__ double pair_r2(const uint4& qi, int ix, int iy, int iz, const Geom& g, double& ex,
@@ -72,14 +34,6 @@ __device__ __forceinline__ double pair_r2(const uint4& qi, int ix, int iy, int i
   ex = (double)ix; ey = (double)iy; ez = (double)iz;
   if (!CUBIC) { ex *= g.qsd[0]; ey *= g.qsd[1]; ez *= g.qsd[2]; }
   return ex * ex + ey * ey + ez * ez;
-}
-
-__device__ __forceinline__ double lj_ff_seeded(double r2, double ff1, double ff2, double& inv) {
-  double u = (double)__builtin_amdgcn_rcpf((float)r2);
-  u = u * __builtin_fma(-r2, u, 2.0);          // one Newton step: ~1e-14 relative
-  inv = u;
-  const double r6 = u * u * u;
-  return r6 * __builtin_fma(ff1, r6, -ff2) * u;
 }
 
 // lj_ff_seeded for a pair that may be out of range: such a pair seeds the
@@ -1276,19 +1230,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // ------------------------------------------------------------------ step helpers
 
 // Philox4x32-10 (Salmon et al., SC'11); same stream as the oracle.
-__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
-                                       uint32_t k0, uint32_t k1, uint32_t out[4]) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
-    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
-  }
-  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
-}
-
 // Thermostat on particle i's force (LangevinThermostat::frictionThermo,
 // src/integrator/LangevinThermostat.cpp:141-177); false when it does not apply.
 __device__ __forceinline__ bool step_thermostat(int i, const double4& v, double& f0, double& f1, double& f2,

@@ -20,71 +20,18 @@
 
 #include "../../include/espp_amd.h"
 #include "espp_dev.h"
+#include "host_util.h"
 #include "kernels.h"
+#include "mlp.h"
 
 using namespace espp;
+using namespace espp_host;
+
+thread_local std::string espp_host::g_err;
 
 namespace {
 
-thread_local std::string g_err;
-
-struct Error {
-  int code;
-  std::string msg;
-};
-
-[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
-
-void hip_check(hipError_t e, const char* what) {
-  if (e != hipSuccess) fail(ESPP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-// Device buffer; resize() discards contents, grow_keep() preserves them.
 constexpr size_t kPosqPad = 64;   // spare fixed-point mirror entries after the last particle
-
-template <class T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  void resize(size_t m) {
-    if (m <= n && p) return;
-    release();
-    if (m == 0) m = 1;
-    if (hipMalloc(&p, m * sizeof(T)) != hipSuccess) {
-      p = nullptr;
-      (void)hipGetLastError();
-      fail(ESPP_ERR_OOM, "device allocation of " + std::to_string(m * sizeof(T)) + " bytes failed");
-    }
-    n = m;
-  }
-  void grow_keep(size_t m, size_t keep, hipStream_t s) {
-    if (m <= n && p) return;
-    T* q = nullptr;
-    if (m == 0) m = 1;
-    if (hipMalloc(&q, m * sizeof(T)) != hipSuccess) {
-      (void)hipGetLastError();
-      fail(ESPP_ERR_OOM, "device allocation of " + std::to_string(m * sizeof(T)) + " bytes failed");
-    }
-    if (p && keep) hip_check(hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s), "grow copy");
-    hip_check(hipStreamSynchronize(s), "grow sync");
-    release();
-    p = q;
-    n = m;
-  }
-  void swap(DBuf& o) {
-    std::swap(p, o.p);
-    std::swap(n, o.n);
-  }
-};
 
 struct VerletListState {
   double cut = 0.0;
@@ -99,6 +46,7 @@ struct VerletListState {
   DBuf<int> ex_start, ex_ids;
   int ex_nid = -1;
   bool has_excl = false;
+  bool untiled_only = false;   // a learned pair force reads this list: keep the untiled layout
 };
 
 struct FixedPairListState {
@@ -116,7 +64,8 @@ struct HostTable {
 };
 
 struct InteractionState {
-  int kind = 0;  // 0: VerletList LJ, 1: FixedPairList FENE, 2: VerletList Tabulated, 3: FixedPairList Tabulated
+  int kind = 0;  // 0: VerletList LJ, 1: FixedPairList FENE, 2: VerletList Tabulated, 3: FixedPairList Tabulated,
+                 // 4: VerletList learned pair force (MLP)
   int list = 0;
   std::vector<LJParam> tab = std::vector<LJParam>(kMaxTypes * kMaxTypes);
   int maxset = -1;       // largest type with a potential
@@ -128,6 +77,15 @@ struct InteractionState {
   std::vector<std::shared_ptr<HostTable>> tables = std::vector<std::shared_ptr<HostTable>>(kMaxTypes * kMaxTypes);
   DBuf<TabParam> ttab;
   DBuf<double> tpool;
+  // learned pair force: FP32 parameters, their bfloat16 operand matrices, the
+  // gradient and Adam moments, per-slot densities and forward forces
+  mlp::Spec spec{};
+  DBuf<float> theta, grad, m1, m2, fpred;
+  DBuf<unsigned short> wpack;
+  DBuf<float2> rho;
+  DBuf<double> loss, fref;
+  bool params_set = false;
+  int64_t adam_steps = 0;
 };
 
 // ------------------------------------------------------------------ tables
@@ -628,7 +586,7 @@ struct espp_sim {
       v.cap = (cap + 7) & ~7;
     }
     v.cnt.resize((size_t)std::max<int64_t>(n, 1));
-    v.tiled = T > 0;
+    v.tiled = T > 0 && !v.untiled_only;
     // the FP32 candidate positions are the binning's until a particle moves
     if (v.tiled && disp_since_bin > 0.0) launch_posf(stream, (int)n, pos.p, posf.p);
     // FP32 prefilter window of the tiled build.  Its coordinates (a position, or
@@ -668,7 +626,7 @@ struct espp_sim {
       if (plan_pending && !tentative_plan_done(!hctl->plan_fail)) {
         // the tentative tile size did not fit: search the others, then build again
         ensure_plan();
-        v.tiled = T > 0;
+        v.tiled = T > 0 && !v.untiled_only;
         --attempt;
         continue;
       }
@@ -817,6 +775,18 @@ struct espp_sim {
   }
 
   void run_interaction(InteractionState& a, bool energy, bool accum, double* partial_out) {
+    if (a.kind == 4) {
+      if (energy) fail(ESPP_ERR_STATE, "the learned pair force is not conservative: it has no energy or virial");
+      if (!a.params_set) fail(ESPP_ERR_STATE, "the learned pair force has no parameters (setParameters)");
+      VerletListState& v = *vls[(size_t)a.list];
+      a.rho.resize((size_t)std::max<int64_t>(n, 1));
+      hipEvent_t pe = prof_begin();
+      mlp::launch_density(stream, (int)n, pos.p, v.nbr.p, v.cnt.p, v.cap, g, a.spec, a.rho.p);
+      mlp::launch_forces(stream, (int)n, pos.p, v.nbr.p, v.cnt.p, v.cap, g, a.spec, a.rho.p, a.wpack.p, a.theta.p,
+                         accum, fx.p, fy.p, fz.p);
+      prof_end(0, pe);
+      return;
+    }
     if (a.kind == 2) {
       VerletListState& v = *vls[(size_t)a.list];
       upload_tab(a);
@@ -1088,28 +1058,6 @@ struct espp_sim {
 
 // ------------------------------------------------------------------ C ABI
 namespace {
-
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    g_err.clear();
-    return ESPP_OK;
-  } catch (const Error& e) {
-    g_err = e.msg;
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_err = "host out of memory";
-    return ESPP_ERR_OOM;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return ESPP_ERR_STATE;
-  }
-}
-
-void need(bool ok, const char* msg) {
-  if (!ok) fail(ESPP_ERR_ARG, msg);
-}
 
 // Fixed-point mirror: x -> low 32 bits of x * 2^(32+k) / L.  Differences of
 // two mirrors are exact int32 and equal the minimum image modulo P = L / 2^k;
@@ -1428,9 +1376,10 @@ int espp_set_particles(espp_sim* s, int32_t field, int64_t m, const int32_t* ids
         need(v[(size_t)k] > 0.0, "particle mass must be positive");
       } else if (field == ESPP_FIELD_POS) {
         for (int c = 0; c < 3; ++c) {
+          // a non-finite coordinate is stored as given; the next binning reports the
+          // lost particle as the reference's decompose does
           const double x = v[(size_t)k * 3 + c];
-          if (!(x == x)) fail(ESPP_ERR_NAN, "particle position is nan");
-          s->coord_bound = std::max(s->coord_bound, std::fabs(x));
+          if (std::isfinite(x)) s->coord_bound = std::max(s->coord_bound, std::fabs(x));
         }
       }
     }
@@ -2012,6 +1961,147 @@ int espp_vl_batch_stats(espp_sim* s, int32_t vl, int32_t batch, int64_t* lane_ba
     }
     if (lane_batches) *lane_batches = lanes;
     if (wave_batches) *wave_batches = waves;
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ learned pair force
+extern "C" {
+
+int espp_mlp_create(espp_sim* s, int32_t vl, const double spec[7], int32_t* ia) {
+  return guarded([&] {
+    VerletListState& v = get_vl(s, vl);
+    need(ia != nullptr && spec != nullptr, "NULL argument");
+    const double rc = spec[0], Ra = spec[1], Rb = spec[2];
+    need(rc > 0.0 && std::isfinite(rc), "the model cutoff must be positive and finite");
+    need(Ra > 0.0 && Rb > 0.0 && Ra <= v.cut && Rb <= v.cut, "density radii must be positive and within the list cutoff");
+    need(rc <= v.cut, "the model cutoff must not exceed the Verlet list cutoff");
+    need(spec[4] > 0.0 && spec[6] > 0.0, "density scales must be positive");
+    auto* a = new InteractionState();
+    a->kind = 4;
+    a->list = vl;
+    mlp::Spec& sp = a->spec;
+    sp.rc = (float)rc;
+    sp.mu0 = 0.0f;
+    sp.dmu = (float)(rc / (mlp::kRbf - 1));
+    sp.inv2s2 = (float)(1.0 / (2.0 * (rc / (mlp::kRbf - 1)) * (rc / (mlp::kRbf - 1))));
+    sp.R[0] = (float)Ra;
+    sp.R[1] = (float)Rb;
+    sp.mean[0] = (float)spec[3];
+    sp.inv_scale[0] = (float)(1.0 / spec[4]);
+    sp.mean[1] = (float)spec[5];
+    sp.inv_scale[1] = (float)(1.0 / spec[6]);
+    a->theta.resize(mlp::kParams);
+    a->grad.resize(mlp::kParams);
+    a->m1.resize(mlp::kParams);
+    a->m2.resize(mlp::kParams);
+    a->wpack.resize(mlp::kPackElems);
+    a->loss.resize(1);
+    mlp::launch_zero(s->stream, a->grad.p, mlp::kParams);
+    mlp::launch_zero(s->stream, a->m1.p, mlp::kParams);
+    mlp::launch_zero(s->stream, a->m2.p, mlp::kParams);
+    hip_check(hipGetLastError(), "mlp create");
+    v.untiled_only = true;
+    v.stale = true;
+    s->ias.push_back(a);
+    *ia = (int32_t)s->ias.size() - 1;
+  });
+}
+
+static InteractionState& get_mlp(espp_sim* s, int32_t ia) {
+  InteractionState& a = get_ia(s, ia);
+  if (a.kind != 4) fail(ESPP_ERR_ARG, "not a learned pair-force interaction");
+  return a;
+}
+
+int espp_mlp_set_params(espp_sim* s, int32_t ia, const float* theta, int64_t n) {
+  return guarded([&] {
+    InteractionState& a = get_mlp(s, ia);
+    need(theta != nullptr, "NULL argument");
+    need(n == mlp::kParams, "the model has 10625 parameters");
+    upload_bytes(a.theta.p, theta, (size_t)n * sizeof(float), s->stream, "mlp parameters");
+    mlp::launch_pack(s->stream, a.theta.p, a.wpack.p);
+    hip_check(hipGetLastError(), "mlp pack");
+    a.params_set = true;
+  });
+}
+
+int espp_mlp_get_params(espp_sim* s, int32_t ia, float* theta, int64_t n) {
+  return guarded([&] {
+    InteractionState& a = get_mlp(s, ia);
+    need(theta != nullptr && n == mlp::kParams, "bad argument");
+    auto h = download(a.theta, (size_t)n, s->stream);
+    std::memcpy(theta, h.data(), (size_t)n * sizeof(float));
+  });
+}
+
+int espp_mlp_grad(espp_sim* s, int32_t ia, float* out, int64_t n) {
+  return guarded([&] {
+    InteractionState& a = get_mlp(s, ia);
+    need(out != nullptr && n == mlp::kParams, "bad argument");
+    auto h = download(a.grad, (size_t)n, s->stream);
+    std::memcpy(out, h.data(), (size_t)n * sizeof(float));
+  });
+}
+
+int espp_mlp_zero_grad(espp_sim* s, int32_t ia) {
+  return guarded([&] {
+    InteractionState& a = get_mlp(s, ia);
+    mlp::launch_zero(s->stream, a.grad.p, mlp::kParams);
+    hip_check(hipGetLastError(), "mlp zero");
+  });
+}
+
+int espp_mlp_train_frame(espp_sim* s, int32_t ia, const double* fref, double weight, double* loss) {
+  return guarded([&] {
+    InteractionState& a = get_mlp(s, ia);
+    need(fref != nullptr, "NULL argument");
+    if (!a.params_set) fail(ESPP_ERR_STATE, "the learned pair force has no parameters (setParameters)");
+    int lds = 0;
+    hip_check(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device), "device attribute");
+    if ((size_t)lds < mlp::lds_bytes(true) + 2048)
+      fail(ESPP_ERR_DEVICE, "training needs " + std::to_string(mlp::lds_bytes(true) + 2048) +
+                                " bytes of LDS per workgroup; the device offers " + std::to_string(lds));
+    s->ensure_lists();
+    VerletListState& v = *s->vls[(size_t)a.list];
+    const int64_t n = s->n;
+    // reference forces by id (input rows follow ascending id order)
+    std::vector<int32_t> ids = s->host_ids;
+    std::sort(ids.begin(), ids.end());
+    std::vector<double> byid((size_t)std::max(s->nid, 1) * 3, 0.0);
+    for (size_t k = 0; k < ids.size(); ++k)
+      for (int c = 0; c < 3; ++c) byid[(size_t)ids[k] * 3 + c] = fref[3 * k + c];
+    a.fref.resize(byid.size());
+    upload_bytes(a.fref.p, byid.data(), byid.size() * sizeof(double), s->stream, "reference forces");
+    a.rho.resize((size_t)std::max<int64_t>(n, 1));
+    a.fpred.resize((size_t)std::max<int64_t>(n, 1) * 3);
+    mlp::launch_density(s->stream, (int)n, s->pos.p, v.nbr.p, v.cnt.p, v.cap, s->g, a.spec, a.rho.p);
+    mlp::launch_forces(s->stream, (int)n, s->pos.p, v.nbr.p, v.cnt.p, v.cap, s->g, a.spec, a.rho.p, a.wpack.p,
+                       a.theta.p, false, s->fx.p, s->fy.p, s->fz.p);
+    mlp::launch_zero_d(s->stream, a.loss.p);
+    const double scale = n > 0 ? weight / (3.0 * (double)n) : 0.0;
+    mlp::launch_train(s->stream, (int)n, s->pos.p, s->id.p, v.nbr.p, v.cnt.p, v.cap, s->g, a.spec, a.rho.p, a.wpack.p,
+                      a.theta.p, s->fx.p, s->fy.p, s->fz.p, a.fpred.p, a.fref.p, scale, a.grad.p, a.loss.p);
+    hip_check(hipGetLastError(), "mlp train");
+    auto h = download(a.loss, 1, s->stream);
+    if (loss) *loss = h[0];
+  });
+}
+
+int espp_mlp_adam(espp_sim* s, int32_t ia, double lr, double beta1, double beta2, double eps) {
+  return guarded([&] {
+    InteractionState& a = get_mlp(s, ia);
+    need(lr > 0.0 && beta1 >= 0.0 && beta1 < 1.0 && beta2 >= 0.0 && beta2 < 1.0 && eps > 0.0,
+         "Adam needs lr > 0, 0 <= beta < 1, eps > 0");
+    if (!a.params_set) fail(ESPP_ERR_STATE, "the learned pair force has no parameters (setParameters)");
+    a.adam_steps++;
+    const double c1 = 1.0 / (1.0 - std::pow(beta1, (double)a.adam_steps));
+    const double c2 = 1.0 / (1.0 - std::pow(beta2, (double)a.adam_steps));
+    mlp::launch_adam(s->stream, a.theta.p, a.grad.p, a.m1.p, a.m2.p, (float)lr, (float)beta1, (float)beta2,
+                     (float)eps, (float)c1, (float)c2);
+    mlp::launch_pack(s->stream, a.theta.p, a.wpack.p);
+    hip_check(hipGetLastError(), "mlp adam");
   });
 }
 

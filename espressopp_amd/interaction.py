@@ -425,3 +425,63 @@ class FixedPairListTabulated(Interaction):
 
     def getMaxCutoff(self):
         return self._potential.cutoff if not math.isinf(self._potential.cutoff) else 0.0
+
+
+class VerletListPairMLP(Interaction):
+    """Learned coarse-grained pair force over a Verlet list (no reference counterpart;
+    the learned alternative to VerletListTabulated, src/interaction/Tabulated.hpp:43-115).
+
+    For a listed pair at r < cutoff: inputs are 32 Gaussian RBFs (centres k rc/31,
+    width rc/31) and the pair means of two smooth local densities (radii
+    densityRadii, cosine weights) normalised by densityNorm = ((mean_a, scale_a),
+    (mean_b, scale_b)); three 64-wide SiLU layers and a linear output give s; the
+    pair force is s (1 + cos(pi r / rc)) / 2 along d / r.  Evaluation and training
+    run on the GPU's BF16 matrix cores with FP32 accumulation.  The force is not
+    conservative, so computeEnergy / computeVirial raise."""
+
+    _bond_type = 1   # Nonbonded
+    N_PARAMS = 10625
+
+    def __init__(self, vl, cutoff=3.0, densityRadii=(1.5, 3.0), densityNorm=((0.0, 1.0), (0.0, 1.0))):
+        self._vl = vl
+        self._h = vl._h
+        spec = np.array([cutoff, densityRadii[0], densityRadii[1], densityNorm[0][0], densityNorm[0][1],
+                         densityNorm[1][0], densityNorm[1][1]], dtype=np.float64)
+        self._id = self._h.out_int("espp_mlp_create", vl._id, _ptr(spec))
+        self.cutoff = float(cutoff)
+
+    def setParameters(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        if t.size != self.N_PARAMS:
+            raise ValueError("the model has %d parameters" % self.N_PARAMS)
+        self._h.call("espp_mlp_set_params", self._id, _ptr(t), t.size)
+
+    def getParameters(self):
+        t = np.zeros(self.N_PARAMS, dtype=np.float32)
+        self._h.call("espp_mlp_get_params", self._id, _ptr(t), t.size)
+        return t
+
+    def trainFrame(self, fref, weight=1.0):
+        """Gradient of weight * sum_i |F_i - fref_i|^2 / (3N) on the current configuration
+        (fref in ascending id order) added to the accumulated gradient; returns the loss."""
+        f = np.ascontiguousarray(fref, dtype=np.float64)
+        loss = ctypes.c_double(0.0)
+        self._h.call("espp_mlp_train_frame", self._id, _ptr(f), float(weight), ctypes.byref(loss))
+        return loss.value
+
+    def gradient(self):
+        g = np.zeros(self.N_PARAMS, dtype=np.float32)
+        self._h.call("espp_mlp_grad", self._id, _ptr(g), g.size)
+        return g
+
+    def zeroGradient(self):
+        self._h.call("espp_mlp_zero_grad", self._id)
+
+    def adamStep(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._h.call("espp_mlp_adam", self._id, float(lr), float(beta1), float(beta2), float(eps))
+
+    def getVerletList(self):
+        return self._vl
+
+    def getMaxCutoff(self):
+        return self.cutoff

@@ -241,6 +241,126 @@ int espp_kinetic(espp_sim *sim, double *mv2);
 int espp_kinetic_tensor(espp_sim *sim, uint64_t type_mask, double *mv2, int64_t *count,
                         double *tensor6);
 
+
+/* ------------------------------------------------------------------------
+ * Learned coarse-grained pair force.
+ *
+ * No counterpart in the reference (ESPResSo++ has no learned potentials): a
+ * density-dependent pair MLP over a Verlet list, the learned alternative to
+ * interaction.VerletListTabulated (src/interaction/Tabulated.hpp:43-115), which
+ * stays the plain fallback.  For a listed pair at distance r < rc the inputs are
+ * 32 Gaussian RBFs (centres k rc / 31, width rc / 31) and the pair means of two
+ * smooth local densities (radii R_a, R_b; cosine weights), normalised by
+ * (x - mean) / scale; three 64-wide SiLU layers and a linear output give s, the
+ * pair force is g = s (1 + cos(pi r / rc)) / 2 along d / r.  Parameters: 10625
+ * FP32 values W1 [34][64], b1, W2 [64][64], b2, W3 [64][64], b3, w4 [64], b4.
+ * Evaluation and training run on BF16 matrix cores with FP32 accumulation.
+ * ------------------------------------------------------------------------ */
+
+/* Learned pair force over Verlet list vl: spec = {rc, R_a, R_b, mean_a, scale_a,
+ * mean_b, scale_b} (rc, R_a, R_b within the list cutoff).  Added to the force
+ * loop with espp_add_interaction like any interaction; it has no energy. */
+int espp_mlp_create(espp_sim *sim, int32_t vl, const double spec[7], int32_t *ia);
+
+/* Parameters (n = 10625 FP32 values) in and out. */
+int espp_mlp_set_params(espp_sim *sim, int32_t ia, const float *theta, int64_t n);
+int espp_mlp_get_params(espp_sim *sim, int32_t ia, float *theta, int64_t n);
+
+/* Force matching on the current configuration: adds the gradient of
+ * weight * sum_i |F_i - fref_i|^2 / (3 N) to the accumulated gradient and
+ * returns that loss.  fref: 3 doubles per particle in ascending id order.  The
+ * particles' forces are left holding the model's forces. */
+int espp_mlp_train_frame(espp_sim *sim, int32_t ia, const double *fref, double weight, double *loss);
+
+/* Accumulated gradient (n = 10625) and its reset. */
+int espp_mlp_grad(espp_sim *sim, int32_t ia, float *grad, int64_t n);
+int espp_mlp_zero_grad(espp_sim *sim, int32_t ia);
+
+/* One Adam step (Kingma & Ba) with the accumulated gradient, which is then cleared. */
+int espp_mlp_adam(espp_sim *sim, int32_t ia, double lr, double beta1, double beta2, double eps);
+
+/* ------------------------------------------------------------------------
+ * Batched replicas (parallel tempering on one GPU).
+ *
+ * One espp_batch holds many periodic cubic Lennard-Jones systems of uneven size
+ * (one particle type, LJ(eps, sigma, rc) with the automatic shift, Verlet skin,
+ * optional Langevin thermostat) packed into shared device arrays.  Each step of
+ * every replica is one fixed sequence of launches over all replicas (integrate,
+ * per-replica skin check and binning, list build, forces) with the rebuild
+ * decisions kept on the device; the sequence is captured once into a HIP graph
+ * and replayed, so a run makes no host round trip per step.  Replaces the
+ * replica loop of ParallelTempering.run / MultiSystem.runIntegrator
+ * (src/ParallelTempering.py:150-162, src/MultiSystem.py:34-81) over systems built
+ * as System + VerletList + VerletListLennardJones + VelocityVerlet +
+ * LangevinThermostat (src/VerletList.cpp:44-67, src/interaction/LennardJones.hpp,
+ * src/integrator/VelocityVerlet.cpp:63-169, src/integrator/LangevinThermostat.cpp).
+ * ------------------------------------------------------------------------ */
+typedef struct espp_batch espp_batch;
+
+/* nrep replicas: n[r] particles in a cubic box of edge box[r], cell grid
+ * int(box / (rc + skin)) per dimension (at least 3, at most 8192 cells);
+ * thermostat temperature[r] with friction gamma (gamma = 0: NVE) and Philox seed
+ * seed[r].  Replaces building nrep Systems as in src/ParallelTempering.py:114-148. */
+int espp_batch_create(int32_t nrep, const int32_t *n, const double *box, double rc, double skin, double eps,
+                      double sigma, double gamma, const double *temperature, const uint64_t *seed,
+                      espp_batch **out);
+
+int espp_batch_destroy(espp_batch *b);
+
+/* Particles of replica rep (Storage::addParticles, src/storage/Storage.py:233-330):
+ * n[rep] unique non-negative ids, positions (3 per particle), velocities and
+ * masses (may be NULL: 0 and 1). */
+int espp_batch_load(espp_batch *b, int32_t rep, const int32_t *ids, const double *pos, const double *vel,
+                    const double *mass);
+
+/* LangevinThermostat.temperature of one replica (the swap of
+ * src/ParallelTempering.py:194-196); takes effect at the next run. */
+int espp_batch_set_temperature(espp_batch *b, int32_t rep, double temperature);
+int espp_batch_get_temperature(espp_batch *b, int32_t rep, double *temperature);
+
+/* VelocityVerlet::run(nsteps) of every replica (src/integrator/VelocityVerlet.cpp:63-169). */
+int espp_batch_run(espp_batch *b, int32_t nsteps, double dt);
+
+/* Interaction::computeEnergy / computeVirial of each replica's LJ interaction
+ * (src/interaction/VerletListInteractionTemplate.hpp:116-198,274-331): nrep values
+ * each; virial may be NULL. */
+int espp_batch_energy(espp_batch *b, double *energy, double *virial);
+
+/* Sum of m v^2 per replica (src/analysis/Temperature.hpp:112-122): nrep values. */
+int espp_batch_kinetic(espp_batch *b, double *mv2);
+
+/* Particle fields of one replica in ascending id order, 3 doubles per particle:
+ * ESPP_FIELD_POS, _VEL, _FORCE or _IMAGE (src/Particle.py getters). */
+int espp_batch_get(espp_batch *b, int32_t rep, int32_t field, double *out);
+
+/* Pairs (id1 < id2) of one replica's Verlet list (VerletList::getAllPairs,
+ * src/VerletList.cpp:314-325); min(cap, total) pairs are written, total returned. */
+int espp_batch_pairs(espp_batch *b, int32_t rep, int32_t *out, int64_t cap, int64_t *total);
+
+/* Lanes per particle (1, 2, 4 or 8) of the force launch, per replica.  No
+ * reference counterpart: the reference evaluates one pair at a time per rank. */
+int espp_batch_set_lpp(espp_batch *b, const int32_t *lpp);
+int espp_batch_get_lpp(espp_batch *b, int32_t *lpp);
+
+/* Average time (ms) of one force launch over all replicas, over reps launches. */
+int espp_batch_force_time(espp_batch *b, int32_t reps, double *ms);
+
+/* Autotuner of the force launch: replicas in four size classes (quartiles of
+ * N), each class trying 1, 2, 4 and 8 lanes per particle with HIP-event timing;
+ * the fastest shape is kept.  lpp_out (nrep), table (16 times, class-major),
+ * before and after (ms) may be NULL.  No reference counterpart. */
+int espp_batch_autotune(espp_batch *b, int32_t reps, int32_t *lpp_out, double *table, double *before,
+                        double *after);
+
+/* Replay each run's steps from captured HIP graphs (on = 1, the default) or launch
+ * the same kernels directly (on = 0); both give identical results. */
+int espp_batch_set_graph(espp_batch *b, int32_t on);
+
+/* Per-replica counters: skin-triggered resorts (VelocityVerlet::getNumResorts,
+ * src/integrator/VelocityVerlet.cpp:434, summed over runs), list builds, and the
+ * longest neighbour list built so far. */
+int espp_batch_stats(espp_batch *b, int32_t rep, int64_t *resorts, int64_t *builds, int32_t *maxcount);
+
 #ifdef __cplusplus
 }
 #endif
