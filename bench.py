@@ -149,6 +149,144 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}), flush=True)
 
 
+def c4_batched_pt(steps=500, every=100, seed=2026):
+    """64 parallel-tempering replicas, N uniform in [2048, 65536] (seeded), density 0.8442,
+    temperatures geometric 0.7-1.5, rc 2.5, skin 0.3, dt 0.005, Langevin gamma 1: every
+    replica advances in the same graph-replayed launches; an exchange every `every` steps."""
+    import random
+    from espressopp_amd.batch import BatchedParallelTempering, ReplicaBatch
+    rng = np.random.default_rng(seed)
+    ns = [int(x) for x in rng.integers(2048, 65536, size=64, endpoint=True)]
+    temps = 0.7 * (1.5 / 0.7) ** (np.arange(64) / 63.0)
+    states = []
+    for k, n in enumerate(ns):
+        pos, box = lattice(n, 0.8442, seed=100 + k)
+        states.append((pos, velocities(n, temps[k], seed=200 + k), box[0]))
+    b = ReplicaBatch(ns, [s_[2] for s_ in states], rc=2.5, skin=0.3, gamma=1.0, temperature=temps,
+                     seed=1000 + np.arange(64))
+    for r, (pos, vel, _) in enumerate(states):
+        b.load(r, np.arange(len(pos), dtype=np.int32), pos, vel)
+    tune = b.autotune(reps=10)
+    b.run(1000, 0.005)                       # equilibration (untimed)
+    pt = BatchedParallelTempering(b, random.Random(seed), dt=0.005)
+    total = sum(ns)
+    lib = __import__("espressopp_amd")._lib.load()
+    lib.espp_device_synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        b.run(k, 0.005)
+        done += k
+        pt.exchange()
+    lib.espp_device_synchronize()
+    dt = time.perf_counter() - t0
+    # the same steps with the kernels launched directly instead of replayed from graphs
+    b.use_graph(False)
+    lib.espp_device_synchronize()
+    t1 = time.perf_counter()
+    b.run(every, 0.005)
+    lib.espp_device_synchronize()
+    direct_ms = (time.perf_counter() - t1) / every * 1e3
+    b.use_graph(True)
+    t1 = time.perf_counter()
+    b.run(every, 0.005)
+    lib.espp_device_synchronize()
+    graph_ms = (time.perf_counter() - t1) / every * 1e3
+    force_ms = b.force_time(20)
+    accepted = sum(1 for e in pt.log if e[4])
+    resorts = sum(b.stats(r)["resorts"] for r in range(64))
+    return {"workload": "64 LJ replicas, N uniform in [2048, 65536] (seed %d), %d particles, rho 0.8442, "
+                        "T geometric 0.7-1.5, Langevin gamma 1, dt 0.005, exchange every %d steps"
+                        % (seed, total, every),
+            "value": total * steps / dt, "unit": "particle-steps/s (all replicas)", "steps": steps,
+            "ms_per_step": dt / steps * 1e3, "ms_per_step_graph": graph_ms, "ms_per_step_direct_launch": direct_ms,
+            "force_launch_ms": force_ms, "autotune": {"lpp_by_replica": tune["lpp"].tolist(),
+                                                      "table_ms_class_by_lpp_1_2_4_8": tune["table_ms"].tolist(),
+                                                      "before_ms": tune["before_ms"], "after_ms": tune["after_ms"]},
+            "exchanges_attempted": len(pt.log), "exchanges_accepted": accepted, "resorts": resorts,
+            "n_min": min(ns), "n_max": max(ns)}
+
+
+def c1_single_replica(steps=2000, seed=2024):
+    """The 32,000-particle LJ fluid (jittered lattice, rho 0.8442, Langevin T = 1,
+    gamma = 1, dt 0.005) as a batch of one replica: every step on the device, graph
+    replayed, no host round trip per step."""
+    from espressopp_amd.batch import ReplicaBatch
+    n = 32000
+    pos, box = lattice(n, 0.8442, seed=seed)
+    b = ReplicaBatch([n], [box[0]], rc=2.5, skin=0.3, gamma=1.0, temperature=[1.0], seed=[seed])
+    b.load(0, np.arange(n, dtype=np.int32), pos, velocities(n, 1.0, seed + 1))
+    b.autotune(reps=10)
+    b.run(500, 0.005)
+    lib = __import__("espressopp_amd")._lib.load()
+    lib.espp_device_synchronize()
+    t0 = time.perf_counter()
+    b.run(steps, 0.005)
+    lib.espp_device_synchronize()
+    dt = time.perf_counter() - t0
+    return {"workload": "LJ fluid N=32000 rho 0.8442 Langevin T=1 gamma=1 dt 0.005, one replica of the batched "
+                        "engine", "us_per_step": dt / steps * 1e6, "value": n * steps / dt,
+            "unit": "particle-steps/s", "lpp": int(b.lpp[0]), "resorts": b.stats(0)["resorts"]}
+
+
+def c5_learned_cg(n_sites=250000, reps=5, seed=7):
+    """Throughput of the learned pair force's BF16 MFMA kernels on a coarse-grained
+    system of n_sites sites at the 4:1-mapped melt density 0.85 / 4 (random positions,
+    the same pair count per site), cutoff 3.0 (+ skin 0.3): inference (forces) and one
+    force-matching step (forward + backward gradient) per frame."""
+    import espressopp_amd as espressopp
+    from espressopp_amd.tools.decomp import cellGrid
+    rho = 0.85 / 4
+    L = (n_sites / rho) ** (1.0 / 3.0)
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0, L, (n_sites, 3))
+    box = (L, L, L)
+    system = espressopp.System(precision="fp64")
+    system.rng = espressopp.esutil.RNG(seed)
+    system.bc = espressopp.bc.OrthorhombicBC(system.rng, box)
+    system.skin = 0.3
+    system.storage = espressopp.storage.DomainDecomposition(system, (1, 1, 1), cellGrid(box, (1, 1, 1), 3.0, 0.3))
+    system.storage.addParticlesArray(np.arange(n_sites, dtype=np.int32), pos)
+    system.storage.decompose()
+    vl = espressopp.VerletList(system, cutoff=3.0)
+    mlp = espressopp.interaction.VerletListPairMLP(vl, cutoff=3.0, densityRadii=(1.5, 3.0),
+                                                   densityNorm=((1.0, 1.0), (10.0, 3.0)))
+    mlp.setParameters(espressopp.interaction.VerletListPairMLP.initialParameters(seed))
+    system.addInteraction(mlp)
+    h = system.storage._h
+    lib = espressopp._lib.load()
+    h.call("espp_compute_forces")
+    fref = rng.normal(0, 1.0, (n_sites, 3))
+    mlp.trainFrame(fref)
+    lib.espp_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h.call("espp_compute_forces")
+    lib.espp_device_synchronize()
+    t_inf = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mlp.trainFrame(fref)
+    t_train = (time.perf_counter() - t0) / reps
+    entries, cap, _ = vl_entries(h, vl)
+    # pairs inside the model cutoff (the kernels evaluate every listed entry, skin shell included)
+    within = entries * (3.0 / 3.3) ** 3
+    # algorithmic FLOP per evaluated pair: forward 2 (34*64 + 64*64 + 64*64 + 64); the
+    # training pass recomputes the forward, back-propagates through W3 and W2 and forms
+    # the three weight gradients: 2 (64*64 + 64*64) + 2 (34*64 + 64*64 + 64*64) more
+    f_fwd = 2.0 * (34 * 64 + 64 * 64 + 64 * 64 + 64)
+    f_train = f_fwd + 2.0 * (64 * 64 * 2) + 2.0 * (34 * 64 + 64 * 64 * 2)
+    return {"workload": "learned CG pair force, %d sites at density %.4f, cutoff 3.0, list %.1f entries per site"
+                        % (n_sites, rho, entries / n_sites),
+            "inference_ms": t_inf * 1e3, "train_frame_ms": t_train * 1e3,
+            "pair_evaluations_per_s_inference": entries / t_inf, "pairs_within_cutoff": within,
+            "mfma_tflops_inference": entries * f_fwd / t_inf / 1e12,
+            "mfma_tflops_train": entries * f_train / t_train / 1e12,
+            "mfma_peak_tflops_bf16": 2500.0,
+            "epoch_s_500_frames": 500 * t_train}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +304,8 @@ def main():
                     help="NVE steps after the timed run with per-kernel HIP events")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the batched parallel-tempering measurement")
+    ap.add_argument("--no-c5", action="store_true", help="skip the learned CG force measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     args = ap.parse_args()
@@ -302,6 +442,22 @@ def main():
                          % (args.cpu_steps, threads, ctime)}
         log("cpu:", cpu)
 
+    c4 = None
+    if not args.no_c4:
+        t = time.perf_counter()
+        c4 = c4_batched_pt()
+        log("c4 batched PT (%.1fs): %s" % (time.perf_counter() - t, json.dumps(c4)))
+    c1 = None
+    if not args.no_c4:
+        t = time.perf_counter()
+        c1 = c1_single_replica()
+        log("c1 via the batched engine (%.1fs): %s" % (time.perf_counter() - t, json.dumps(c1)))
+    c5 = None
+    if not args.no_c5:
+        t = time.perf_counter()
+        c5 = c5_learned_cg()
+        log("c5 learned CG (%.1fs): %s" % (time.perf_counter() - t, json.dumps(c5)))
+
     out = {
         "metric": "particle-steps/s, LJ 4M: mixed vs FP64 GPU path at force rel err <=1e-5",
         "value": value,
@@ -331,6 +487,9 @@ def main():
         "nve_drift": drift,
         "resorts": resorts,
         "cpu_baseline": cpu,
+        "c4_batched_parallel_tempering": c4,
+        "c1_single_replica_batched": c1,
+        "c5_learned_cg_force": c5,
     }
     traffic_file = os.path.join(ROOT, "profiles", "lj_mixed_4M_traffic.json")
     if os.path.exists(traffic_file):

@@ -1229,7 +1229,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 
 // ------------------------------------------------------------------ step helpers
 
-// Philox4x32-10 (Salmon et al., SC'11); same stream as the oracle.
 // Thermostat on particle i's force (LangevinThermostat::frictionThermo,
 // src/integrator/LangevinThermostat.cpp:141-177); false when it does not apply.
 __device__ __forceinline__ bool step_thermostat(int i, const double4& v, double& f0, double& f1, double& f2,

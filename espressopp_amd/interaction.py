@@ -450,6 +450,17 @@ class VerletListPairMLP(Interaction):
         self._id = self._h.out_int("espp_mlp_create", vl._id, _ptr(spec))
         self.cutoff = float(cutoff)
 
+    @classmethod
+    def initialParameters(cls, seed=0, scale=1.0):
+        """He-normal weights (std scale sqrt(2 / fan_in); sqrt(1 / 64) for the output
+        row), zero biases."""
+        rng = np.random.default_rng(seed)
+        parts = [rng.normal(0, scale * math.sqrt(2.0 / 34), 34 * 64), np.zeros(64),
+                 rng.normal(0, scale * math.sqrt(2.0 / 64), 64 * 64), np.zeros(64),
+                 rng.normal(0, scale * math.sqrt(2.0 / 64), 64 * 64), np.zeros(64),
+                 rng.normal(0, scale * math.sqrt(1.0 / 64), 64), np.zeros(1)]
+        return np.concatenate(parts).astype(np.float32)
+
     def setParameters(self, theta):
         t = np.ascontiguousarray(theta, dtype=np.float32)
         if t.size != self.N_PARAMS:

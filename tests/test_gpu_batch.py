@@ -13,7 +13,7 @@ import random
 import numpy as np
 import pytest
 
-from tests._systems import cells_for, jittered_lattice, mb_velocities
+from tests._systems import cells_for, force_rel_err, jittered_lattice, mb_velocities
 
 pytestmark = pytest.mark.gpu
 
@@ -72,17 +72,23 @@ def test_batch_pairs_forces_energy_match_oracle(gpu, lpp):
         ref, vl, ia = oracle_for(st, seed=1000 + r)
         ref.calc_forces()
         rp = ref.vl_pairs(vl)
+        rp = np.sort(np.asarray(rp, dtype=np.int32).reshape(-1, 2), axis=1)
+        rp = rp[np.lexsort((rp[:, 1], rp[:, 0]))]
         gp = b.pairs(r)
         print("replica %d: %d pairs (oracle %d)" % (r, len(gp), len(rp)), flush=True)
         assert np.array_equal(gp, rp)
+        # positions enter the pair kernel as 32-bit fixed point (L / 2^32 ~ 3e-9 at these
+        # boxes): the mixed path's bound is 1e-5 per particle (floor 0.1 RMS |F|), 1e-6 energy
         f_ref = by_id(ref, "f")
         f = b.get(r, "f")
-        err = np.linalg.norm(f - f_ref, axis=1) / np.maximum(np.linalg.norm(f_ref, axis=1), 1.0)
-        assert err.max() < 1e-9
+        err = force_rel_err(f, f_ref, 0.1)
         e_ref = ref.energy(ia)
         w_ref, _ = ref.virial(ia)
-        assert abs(e[r] - e_ref) <= 1e-9 * abs(e_ref)
-        assert abs(w[r] - w_ref) <= 1e-9 * abs(w_ref)
+        print("  force rel err max %.2e, energy rel %.2e, virial rel %.2e" %
+              (err.max(), abs(e[r] - e_ref) / abs(e_ref), abs(w[r] - w_ref) / abs(w_ref)), flush=True)
+        assert err.max() < 1e-5
+        assert abs(e[r] - e_ref) <= 1e-6 * abs(e_ref)
+        assert abs(w[r] - w_ref) <= 1e-6 * abs(w_ref)
 
 
 def test_batch_nve_and_langevin_follow_oracle(gpu):
