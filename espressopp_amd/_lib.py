@@ -56,6 +56,7 @@ SIGNATURES = {
     "espp_ia_set_fpl": [_vp, _c_int, _c_int],
     "espp_ia_energy": [_vp, _c_int, _P(_c_dbl)],
     "espp_ia_virial": [_vp, _c_int, _P(_c_dbl), _vp],
+    "espp_ia_energy_noalloc": [_vp, _c_int, _P(_c_dbl), _P(_c_dbl), _vp],
     "espp_compute_forces": [_vp],
     "espp_langevin": [_vp, _c_int, _c_dbl, _c_dbl, _c_i64, _vp],
     "espp_run": [_vp, _c_int, _c_dbl],

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <string>
@@ -32,6 +33,28 @@ thread_local std::string espp_host::g_err;
 namespace {
 
 constexpr size_t kPosqPad = 64;   // spare fixed-point mirror entries after the last particle
+
+// Device scratch of espp_ia_energy_noalloc: per-workgroup partial sums of an
+// interaction's energy and virial, and their reduction in the last 8 entries.
+// One copy per GPU, held until the process ends; every call on that GPU shares it.
+constexpr size_t kScratchDoubles = (size_t)1 << 18;          // 2 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch, looked up once per device with that
+// device current (the array's host-side name is not a device address).
+double* scratch_for_device(int device) {
+  if (device < 0 || device >= kMaxDevices) fail(ESPP_ERR_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
 
 struct VerletListState {
   double cut = 0.0;
@@ -290,6 +313,7 @@ struct espp_sim {
   unsigned long long* dmax = nullptr;     // device view of hmax
   DBuf<double> partial, red;
   double* hred = nullptr;
+  double* scratch_dev = nullptr;   // this device's scratch (espp_ia_energy_noalloc)
 
   std::vector<VerletListState*> vls;
   std::vector<FixedPairListState*> fpls;
@@ -1163,6 +1187,7 @@ int espp_create(const double box[3], const int32_t cell_grid[3], int32_t half_ce
       hip_check(hipHostGetDevicePointer((void**)&s->dmax, s->hmax, 0), "hipHostGetDevicePointer");
       hip_check(hipHostMalloc((void**)&s->hred, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
       s->red.resize(16);
+      s->scratch_dev = scratch_for_device(s->device);
       int64_t nc = s->ncell();
       need(nc < (int64_t)1 << 30, "cell grid too large");
       s->cell_count.resize((size_t)nc);
@@ -1750,6 +1775,53 @@ int espp_ia_virial(espp_sim* s, int32_t ia, double* w, double* t6) {
     s->interaction_sums(a, o);
     *w = o[1];
     if (t6) for (int q = 0; q < 6; ++q) t6[q] = o[2 + q];
+  });
+}
+
+// espp_ia_energy / espp_ia_virial that allocates nothing: the per-workgroup
+// partials and their reduction live in the device's static scratch.  Needs lists,
+// tile plan and parameter tables current (any run or force computation leaves
+// them so); otherwise ESPP_ERR_STATE, since bringing them up to date allocates.
+int espp_ia_energy_noalloc(espp_sim* s, int32_t ia, double* energy, double* virial, double* tensor6) {
+  return guarded([&] {
+    InteractionState& a = get_ia(s, ia);
+    need(energy != nullptr, "NULL argument");
+    if (a.kind == 4) fail(ESPP_ERR_STATE, "the learned pair force is not conservative: it has no energy or virial");
+    if (!s->scratch_dev) fail(ESPP_ERR_STATE, "no device scratch for this handle");
+    const char* stale = "state needs a rebuild or upload first (call espp_compute_forces or espp_run once): "
+                        "that allocates, this entry point does not";
+    if (!s->sorted || (s->plan_pending) || (!s->plan_valid && s->n > 0)) fail(ESPP_ERR_STATE, stale);
+    bool tiled = false;
+    if (a.kind == 0 || a.kind == 2) {
+      VerletListState& v = *s->vls[(size_t)a.list];
+      if (v.stale) fail(ESPP_ERR_STATE, stale);
+      tiled = v.tiled;
+      if (a.dirty || a.dtab_nt != s->lj_ntypes(a)) fail(ESPP_ERR_STATE, stale);
+    } else {
+      FixedPairListState& f = *s->fpls[(size_t)a.list];
+      if (f.dirty || f.built_nid != s->nid) fail(ESPP_ERR_STATE, stale);
+      if (a.kind == 3 && (a.dirty || a.dtab_nt != 1)) fail(ESPP_ERR_STATE, stale);
+    }
+    const int n = (int)s->n;
+    double out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (n > 0) {
+      const int nblk = std::max(tiled ? lj_tiled_blocks(n, s->T) : lj_blocks(n), 1);
+      if ((size_t)nblk * 8 + 8 > kScratchDoubles)
+        fail(ESPP_ERR_ARG, "system too large for the fixed scratch: " + std::to_string(nblk) +
+                               " workgroups need " + std::to_string((size_t)nblk * 8 + 8) + " doubles, the scratch holds " +
+                               std::to_string(kScratchDoubles));
+      double* part = s->scratch_dev;
+      double* tail = s->scratch_dev + kScratchDoubles - 8;
+      s->run_interaction(a, true, false, part);
+      launch_reduce_partials(s->stream, nblk, 8, part, tail);
+      hip_check(hipMemcpyAsync(s->hred, tail, 8 * sizeof(double), hipMemcpyDeviceToHost, s->stream),
+                "scratch readback");
+      s->sync();
+      std::memcpy(out, s->hred, sizeof(out));
+    }
+    *energy = out[0];
+    if (virial) *virial = out[1];
+    if (tensor6) for (int q = 0; q < 6; ++q) tensor6[q] = out[2 + q];
   });
 }
 

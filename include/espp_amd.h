@@ -168,6 +168,18 @@ int espp_tab_set(espp_sim *sim, int32_t ia, int32_t type1, int32_t type2, int32_
 int espp_tabbond_create(espp_sim *sim, int32_t fpl, int32_t itype, int64_t n, const double *r,
                         const double *e, const double *f, double cutoff, int32_t *ia);
 
+/* espp_ia_energy + espp_ia_virial in one call that allocates nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight monitoring loops, harnesses that forbid hipMalloc).  The per-workgroup
+ * partial sums and their reduction use a static 2 MiB device scratch per GPU (kept until the
+ * process ends), so the system may have at most 262,136 / 8 = 32,767 force workgroups
+ * (about 8.4 M particles with 256-particle tiles, 2.1 M with 64-particle tiles); larger
+ * systems get ESPP_ERR_ARG.  The Verlet lists, tile plan and parameter tables must be
+ * current (any espp_run or espp_compute_forces leaves them so); otherwise ESPP_ERR_STATE.
+ * All calls on one GPU share the scratch: calls that overlap on one GPU race.  virial and
+ * tensor6 (xx, yy, zz, xy, xz, yz) may be NULL. */
+int espp_ia_energy_noalloc(espp_sim *sim, int32_t ia, double *energy, double *virial, double *tensor6);
+
 /* System::addInteraction (src/System.cpp:75-85, bound at :221): appends the
  * interaction to the short-range list whose order is the force summation order. */
 int espp_add_interaction(espp_sim *sim, int32_t ia);
