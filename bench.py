@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched parallel-tempering measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the learned CG force measurement")
+    ap.add_argument("--only", choices=["c1", "c4", "c5"], default=None,
+                    help="run only one companion workload and print its JSON (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     args = ap.parse_args()
@@ -324,6 +326,10 @@ def main():
     if espressopp._lib.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.only:
+        fn = {"c1": c1_single_replica, "c4": c4_batched_pt, "c5": c5_learned_cg}[args.only]
+        print(json.dumps(fn()), flush=True)
+        return
 
     n, rho, rc, skin = args.n, 0.8442, 2.5, 0.3
     pos, box = lattice(n, rho, seed=2024)

@@ -4,16 +4,19 @@
 // together.  Every launch covers all replicas; a replica's per-step decisions
 // (the skin/2 rebuild trigger of VelocityVerlet::run,
 // src/integrator/VelocityVerlet.cpp:121-148, and its thermostat call counter)
-// live in device memory, so a step is a fixed sequence of four launches with
+// live in device memory, so a step is a fixed sequence of launches with
 // no host round trip, captured once into a HIP graph and replayed:
 //
 //   kb_step       thermostat + integrate2 of the previous step and integrate1
 //                 (src/integrator/LangevinThermostat.cpp:141-177,
 //                 src/integrator/VelocityVerlet.cpp:236-305); per-replica
 //                 max |dx|^2 by one atomic per wavefront
-//   kb_check_bin  one workgroup per replica: accumulate maxDist, decide the
-//                 rebuild, and when it is due fold + bin + order the replica's
-//                 particles in LDS (Storage::decompose for one node,
+//   kb_check      one workgroup per replica: accumulate maxDist, decide the
+//                 rebuild (and clear the rebuilding replicas' cell counts)
+//   binning       kb_fold_count, kb_scan, kb_scatter, kb_sort_cells,
+//                 kb_permute, kb_copy_back over all particles (or cells) of
+//                 every replica; workgroups of replicas that do not rebuild
+//                 return at once (Storage::decompose for one node,
 //                 src/storage/DomainDecomposition.cpp:684-1179)
 //   kb_build      full Verlet list of the replicas that rebuilt
 //                 (src/VerletList.cpp:123-298, pair test in the reference's
@@ -198,26 +201,13 @@ __device__ void block_scan_lds(int* cnt, int nc, int* wsum) {
 
 // One workgroup per replica.  Thread 0 accumulates the step's displacement into
 // maxDist and decides the rebuild (maxDist > skin/2, a NaN, or a requested
-// one).  A rebuilding replica then, in its own workgroup: folds every position
-// (OrthorhombicBC::foldCoordinate, src/bc/OrthorhombicBC.cpp:144-161) and maps
-// it to its cell (CellGrid::mapPositionToCellChecked, src/storage/CellGrid.cpp:98-136),
-// counts cells in LDS, scans, scatters, orders each cell by particle id (so the
-// order does not depend on atomics), and permutes the particle arrays into the
-// new slot order (through the second buffers and back).
-__global__ __launch_bounds__(kBinThreads) void kb_check_bin(
-    const RepDesc* __restrict__ rd, RepState* __restrict__ st, int use_step, double4* __restrict__ pos,
-    double4* __restrict__ vel, int* __restrict__ id, int4* __restrict__ img, uint4* __restrict__ posq,
-    double4* __restrict__ pos2, double4* __restrict__ vel2, int* __restrict__ id2, int4* __restrict__ img2,
-    int* __restrict__ cell_of_slot, int* __restrict__ cstart, int* __restrict__ tcell, int* __restrict__ trank,
-    int* __restrict__ order) {
-#pragma clang fp contract(off)
-  __shared__ int cnt[kMaxBinCells + 1];
-  __shared__ int wsum[kBinThreads / 64 + 1];
+// one); a rebuilding replica's cell counts are then cleared for kb_fold_count.
+__global__ __launch_bounds__(kBinThreads) void kb_check(const RepDesc* __restrict__ rd, RepState* __restrict__ st,
+                                                        int use_step, int* __restrict__ ccount) {
   __shared__ int flag;
   const int r = blockIdx.x;
-  const int tid = threadIdx.x;
   const RepDesc& d = rd[r];
-  if (tid == 0) {
+  if (threadIdx.x == 0) {
     RepState& s = st[r];
     int rb = s.force;
     if (use_step) {
@@ -238,53 +228,110 @@ __global__ __launch_bounds__(kBinThreads) void kb_check_bin(
   }
   __syncthreads();
   if (!flag) return;
-  const int n = d.n, s0 = d.slot0, nc = d.nc;
-  const int ncell = nc * nc * nc;
-  for (int c = tid; c <= ncell; c += kBinThreads) cnt[c] = 0;
-  __syncthreads();
-  // fold, cell, rank by LDS atomics
-  bool bad = false;
-  for (int i = tid; i < n; i += kBinThreads) {
-    const int s = s0 + i;
-    double4 p = pos[s];
-    int4 im = img[s];
-    double x[3] = {p.x, p.y, p.z};
-    int iv[3] = {im.x, im.y, im.z};
-    int c[3];
-    bool moved = false;
+  const int ncell = d.nc * d.nc * d.nc;
+  for (int c = threadIdx.x; c <= ncell; c += kBinThreads) ccount[d.cell0 + c] = 0;
+}
+
+// Fold every position (OrthorhombicBC::foldCoordinate, src/bc/OrthorhombicBC.cpp:144-161),
+// map it to its cell (CellGrid::mapPositionToCellChecked, src/storage/CellGrid.cpp:98-136)
+// and count it there; lanes of a wavefront in one cell share one atomic.
+__global__ __launch_bounds__(kPartThreads) void kb_fold_count(const BlockMap* __restrict__ bm,
+                                                              const RepDesc* __restrict__ rd,
+                                                              RepState* __restrict__ st, double4* __restrict__ pos,
+                                                              int4* __restrict__ img, int* __restrict__ ccount,
+                                                              int* __restrict__ tcell, int* __restrict__ trank) {
+#pragma clang fp contract(off)
+  const BlockMap m = bm[blockIdx.x];
+  const int r = m.rep;
+  if (!st[r].rebuild) return;
+  const RepDesc& d = rd[r];
+  const int t = threadIdx.x;
+  const bool live = t < m.count;
+  const int s = m.s0 + (live ? t : 0);
+  double4 p = pos[s];
+  int4 im = img[s];
+  double x[3] = {p.x, p.y, p.z};
+  int iv[3] = {im.x, im.y, im.z};
+  int c[3];
+  bool bad = false, moved = false;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if (!(fabs(x[q] * d.invL) < 1.0e9)) { bad = true; c[q] = 0; continue; }
-      const int tmp = (int)floor(x[q] * d.invL);
-      iv[q] += tmp;
-      x[q] -= (double)tmp * d.L;
-      moved |= tmp != 0;
-      int cc = (int)((x[q] - 0.0) * d.ics) + 1;
-      if (cc < 1) cc = 1;
-      else if (cc >= nc + 1) cc = nc;
-      c[q] = cc - 1;
-    }
-    if (moved) {
-      p.x = x[0]; p.y = x[1]; p.z = x[2];
-      pos[s] = p;
-      img[s] = make_int4(iv[0], iv[1], iv[2], 0);
-    }
-    const int cell = c[0] + nc * (c[1] + nc * c[2]);
-    tcell[s] = cell;
-    trank[s] = atomicAdd(&cnt[cell], 1);
+  for (int q = 0; q < 3; ++q) {
+    if (!(fabs(x[q] * d.invL) < 1.0e9)) { bad = true; c[q] = 0; continue; }
+    const int tmp = (int)floor(x[q] * d.invL);
+    iv[q] += tmp;
+    x[q] -= (double)tmp * d.L;
+    moved |= tmp != 0;
+    int cc = (int)((x[q] - 0.0) * d.ics) + 1;
+    if (cc < 1) cc = 1;
+    else if (cc >= d.nc + 1) cc = d.nc;
+    c[q] = cc - 1;
   }
+  const int cell = c[0] + d.nc * (c[1] + d.nc * c[2]);
+  const int rank = wave_aggregated_rank(cell, live, ccount + d.cell0);
+  if (!live) return;
   if (bad) st[r].nanpos = 1;
+  if (moved) {
+    p.x = x[0]; p.y = x[1]; p.z = x[2];
+    pos[s] = p;
+    img[s] = make_int4(iv[0], iv[1], iv[2], 0);
+  }
+  tcell[s] = cell;
+  trank[s] = rank;
+}
+
+// Exclusive scan of a rebuilding replica's cell counts into its cell starts
+// (one workgroup per replica, the counts in LDS).
+__global__ __launch_bounds__(kBinThreads) void kb_scan(const RepDesc* __restrict__ rd,
+                                                       const RepState* __restrict__ st,
+                                                       const int* __restrict__ ccount, int* __restrict__ cstart) {
+  __shared__ int cnt[kMaxBinCells + 1];
+  __shared__ int wsum[kBinThreads / 64 + 1];
+  const int r = blockIdx.x;
+  if (!st[r].rebuild) return;
+  const RepDesc& d = rd[r];
+  const int ncell = d.nc * d.nc * d.nc;
+  for (int c = threadIdx.x; c < ncell; c += kBinThreads) cnt[c] = ccount[d.cell0 + c];
   __syncthreads();
   block_scan_lds(cnt, ncell, wsum);
-  for (int c = tid; c <= ncell; c += kBinThreads) cstart[d.cell0 + c] = cnt[c];
-  for (int i = tid; i < n; i += kBinThreads) order[s0 + cnt[tcell[s0 + i]] + trank[s0 + i]] = i;
-  __syncthreads();
-  // order each cell by particle id: one wavefront per cell, up to four entries per lane
-  const int lane = tid & 63, wv = tid >> 6;
-  for (int c = wv; c < ncell; c += kBinThreads / 64) {
-    const int b = cnt[c], m = cnt[c + 1] - b;
-    if (m <= 1) continue;
-    if (m > kMaxCellFill) {
+  for (int c = threadIdx.x; c <= ncell; c += kBinThreads) cstart[d.cell0 + c] = cnt[c];
+}
+
+// Slot order of the new binning: local index of each particle at its cell's
+// start plus its rank.
+__global__ __launch_bounds__(kPartThreads) void kb_scatter(const BlockMap* __restrict__ bm,
+                                                           const RepDesc* __restrict__ rd,
+                                                           const RepState* __restrict__ st,
+                                                           const int* __restrict__ cstart,
+                                                           const int* __restrict__ tcell,
+                                                           const int* __restrict__ trank, int* __restrict__ order) {
+  const BlockMap m = bm[blockIdx.x];
+  const int r = m.rep;
+  if (!st[r].rebuild || threadIdx.x >= (unsigned)m.count) return;
+  const RepDesc& d = rd[r];
+  const int s = m.s0 + threadIdx.x;
+  order[d.slot0 + cstart[d.cell0 + tcell[s]] + trank[s]] = s - d.slot0;
+}
+
+// Each cell ordered by particle id, so that the slot order does not depend on
+// the atomics' order: one wavefront per cell, up to four entries per lane,
+// ranks by comparison.  A cell map block covers kSortCells cells of one replica.
+constexpr int kSortCells = 64;
+__global__ __launch_bounds__(kPartThreads) void kb_sort_cells(const BlockMap* __restrict__ cm,
+                                                              const RepDesc* __restrict__ rd,
+                                                              RepState* __restrict__ st,
+                                                              const int* __restrict__ cstart,
+                                                              const int* __restrict__ id, int* __restrict__ order) {
+  const BlockMap m = cm[blockIdx.x];
+  const int r = m.rep;
+  if (!st[r].rebuild) return;
+  const RepDesc& d = rd[r];
+  const int s0 = d.slot0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int cc = wv; cc < m.count; cc += kPartThreads / 64) {
+    const int c = m.s0 + cc;                      // cell index within the replica
+    const int b = cstart[d.cell0 + c], mlen = cstart[d.cell0 + c + 1] - b;
+    if (mlen <= 1) continue;
+    if (mlen > kMaxCellFill) {
       if (lane == 0) st[r].cellfull = 1;
       continue;
     }
@@ -292,13 +339,13 @@ __global__ __launch_bounds__(kBinThreads) void kb_check_bin(
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int k = lane + 64 * u;
-      ent[u] = k < m ? order[s0 + b + k] : 0;
-      key[u] = k < m ? id[s0 + ent[u]] : INT_MAX;
+      ent[u] = k < mlen ? order[s0 + b + k] : 0;
+      key[u] = k < mlen ? id[s0 + ent[u]] : INT_MAX;
     }
     int rk[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int u2 = 0; u2 < 4; ++u2) {
-      if (64 * u2 >= m) break;     // wavefront-uniform
+      if (64 * u2 >= mlen) break;     // wavefront-uniform
       for (int q = 0; q < 64; ++q) {
         const int other = __shfl(key[u2], q, 64);
         const int ok = q + 64 * u2;
@@ -306,33 +353,60 @@ __global__ __launch_bounds__(kBinThreads) void kb_check_bin(
         for (int u = 0; u < 4; ++u) {
           const int k = lane + 64 * u;
           // ids are unique within a replica; ties (padding) keep the entry index order
-          rk[u] += (ok < m) && (other < key[u] || (other == key[u] && ok < k));
+          rk[u] += (ok < mlen) && (other < key[u] || (other == key[u] && ok < k));
         }
       }
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (lane + 64 * u < m) order[s0 + b + rk[u]] = ent[u];
+      if (lane + 64 * u < mlen) order[s0 + b + rk[u]] = ent[u];
   }
-  __syncthreads();
-  for (int k = tid; k < n; k += kBinThreads) {
-    const int o = s0 + order[s0 + k];
-    pos2[s0 + k] = pos[o];
-    vel2[s0 + k] = vel[o];
-    id2[s0 + k] = id[o];
-    img2[s0 + k] = img[o];
-    cell_of_slot[s0 + k] = tcell[o];
-  }
-  __syncthreads();
-  for (int k = tid; k < n; k += kBinThreads) {
-    const int s = s0 + k;
-    const double4 p = pos2[s];
-    pos[s] = p;
-    vel[s] = vel2[s];
-    id[s] = id2[s];
-    img[s] = img2[s];
-    posq[s] = make_uint4(fixq(p.x, d.qinv), fixq(p.y, d.qinv), fixq(p.z, d.qinv), (uint32_t)(int)p.w);
-  }
+}
+
+// Gather the particle arrays into the new slot order (second buffers).
+__global__ __launch_bounds__(kPartThreads) void kb_permute(const BlockMap* __restrict__ bm,
+                                                           const RepDesc* __restrict__ rd,
+                                                           const RepState* __restrict__ st,
+                                                           const int* __restrict__ order,
+                                                           const double4* __restrict__ pos,
+                                                           const double4* __restrict__ vel,
+                                                           const int* __restrict__ id, const int4* __restrict__ img,
+                                                           const int* __restrict__ tcell, double4* __restrict__ pos2,
+                                                           double4* __restrict__ vel2, int* __restrict__ id2,
+                                                           int4* __restrict__ img2, int* __restrict__ cell_of_slot) {
+  const BlockMap m = bm[blockIdx.x];
+  const int r = m.rep;
+  if (!st[r].rebuild || threadIdx.x >= (unsigned)m.count) return;
+  const int s = m.s0 + threadIdx.x;
+  const int o = rd[r].slot0 + order[s];
+  pos2[s] = pos[o];
+  vel2[s] = vel[o];
+  id2[s] = id[o];
+  img2[s] = img[o];
+  cell_of_slot[s] = tcell[o];
+}
+
+// ... and back into the primary buffers, with the fixed-point mirror.
+__global__ __launch_bounds__(kPartThreads) void kb_copy_back(const BlockMap* __restrict__ bm,
+                                                             const RepDesc* __restrict__ rd,
+                                                             const RepState* __restrict__ st,
+                                                             const double4* __restrict__ pos2,
+                                                             const double4* __restrict__ vel2,
+                                                             const int* __restrict__ id2,
+                                                             const int4* __restrict__ img2, double4* __restrict__ pos,
+                                                             double4* __restrict__ vel, int* __restrict__ id,
+                                                             int4* __restrict__ img, uint4* __restrict__ posq) {
+  const BlockMap m = bm[blockIdx.x];
+  const int r = m.rep;
+  if (!st[r].rebuild || threadIdx.x >= (unsigned)m.count) return;
+  const int s = m.s0 + threadIdx.x;
+  const double4 p = pos2[s];
+  const double qinv = rd[r].qinv;
+  pos[s] = p;
+  vel[s] = vel2[s];
+  id[s] = id2[s];
+  img[s] = img2[s];
+  posq[s] = make_uint4(fixq(p.x, qinv), fixq(p.y, qinv), fixq(p.z, qinv), (uint32_t)(int)p.w);
 }
 
 // Full Verlet list of every replica that rebuilt this step.  Membership is the
@@ -586,7 +660,9 @@ struct espp_batch {
   DBuf<int4> img, img2, img_snap;
   DBuf<uint4> posq;
   DBuf<double> fx, fy, fz;
-  DBuf<int> cell_of_slot, cstart, tcell, trank, order;
+  DBuf<int> cell_of_slot, cstart, ccount, tcell, trank, order;
+  DBuf<BlockMap> bm_cell;                 // cell blocks of kb_sort_cells
+  int nb_cell = 0;
   DBuf<int> nbr, cnt;
   DBuf<double> partial, red;
   double* hred = nullptr;
@@ -632,6 +708,13 @@ struct espp_batch {
       for (int k = 0; k < n; k += per) mf.push_back({r, s0 + k, std::min(per, n - k), lpp[r]});
       fnb[r] = (int)mf.size() - fb0[r];
     }
+    std::vector<BlockMap> mc;
+    for (int r = 0; r < R; ++r) {
+      const int nc3 = hd[r].nc * hd[r].nc * hd[r].nc;
+      for (int c = 0; c < nc3; c += kSortCells) mc.push_back({r, c, std::min(kSortCells, nc3 - c), 1});
+    }
+    nb_cell = (int)mc.size();
+    bm_cell.resize(mc.size());
     nb_part = (int)mp.size();
     nb_force = (int)mf.size();
     bm_part.resize(mp.size());
@@ -643,6 +726,7 @@ struct espp_batch {
     };
     up(bm_part.p, mp.data(), mp.size() * sizeof(BlockMap));
     up(bm_force.p, mf.data(), mf.size() * sizeof(BlockMap));
+    up(bm_cell.p, mc.data(), mc.size() * sizeof(BlockMap));
     up(pblk0.p, pb0.data(), R * sizeof(int));
     up(pnblk.p, pnb.data(), R * sizeof(int));
     up(fblk0.p, fb0.data(), R * sizeof(int));
@@ -651,10 +735,20 @@ struct espp_batch {
     drop_graphs();
   }
 
+  // the skin check and, for the replicas that rebuild, the binning
   void launch_check_bin(int use_step) {
-    hipLaunchKernelGGL(kb_check_bin, dim3(R), dim3(kBinThreads), 0, stream, drd.p, dst.p, use_step, pos.p, vel.p,
-                       id.p, img.p, posq.p, pos2.p, vel2.p, id2.p, img2.p, cell_of_slot.p, cstart.p, tcell.p,
-                       trank.p, order.p);
+    hipLaunchKernelGGL(kb_check, dim3(R), dim3(kBinThreads), 0, stream, drd.p, dst.p, use_step, ccount.p);
+    hipLaunchKernelGGL(kb_fold_count, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos.p,
+                       img.p, ccount.p, tcell.p, trank.p);
+    hipLaunchKernelGGL(kb_scan, dim3(R), dim3(kBinThreads), 0, stream, drd.p, dst.p, ccount.p, cstart.p);
+    hipLaunchKernelGGL(kb_scatter, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, cstart.p,
+                       tcell.p, trank.p, order.p);
+    hipLaunchKernelGGL(kb_sort_cells, dim3(nb_cell), dim3(kPartThreads), 0, stream, bm_cell.p, drd.p, dst.p,
+                       cstart.p, id.p, order.p);
+    hipLaunchKernelGGL(kb_permute, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, order.p,
+                       pos.p, vel.p, id.p, img.p, tcell.p, pos2.p, vel2.p, id2.p, img2.p, cell_of_slot.p);
+    hipLaunchKernelGGL(kb_copy_back, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos2.p,
+                       vel2.p, id2.p, img2.p, pos.p, vel.p, id.p, img.p, posq.p);
   }
   void launch_build() {
     hipLaunchKernelGGL(kb_build, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos.p,
@@ -924,6 +1018,7 @@ int espp_batch_create(int32_t nrep, const int32_t* n, const double* box, double 
     b->fx.resize(S); b->fy.resize(S); b->fz.resize(S);
     b->cell_of_slot.resize(S); b->tcell.resize(S); b->trank.resize(S); b->order.resize(S);
     b->cstart.resize((size_t)cell);
+    b->ccount.resize((size_t)cell);
     b->cnt.resize(S);
     b->nbr.resize((S / 64) * (size_t)b->cap * 64);
     b->red.resize(2 * (size_t)nrep);

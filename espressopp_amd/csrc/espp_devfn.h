@@ -1,5 +1,5 @@
 // Device helper functions shared by the kernel files: exact FP64 squares, the
-// fixed-point position mirror, reciprocals, wavefront reductions, the
+// fixed-point position mirror, reciprocals, wavefront reductions and ranks, the
 // Philox4x32-10 counter-based generator and the Lennard-Jones force factor.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -45,6 +45,26 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+
+// Rank of this lane among all particles counted into cell `key`: lanes of one
+// wavefront holding the same key share one atomicAdd (a wavefront spans only a
+// few cells because particles arrive in cell order) and are ranked in lane order.
+__device__ __forceinline__ int wave_aggregated_rank(int key, bool live, int* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(live);
+  int rank = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lk = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(live && key == lk);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&count[lk], __popcll(same));
+    base = __shfl(base, leader, 64);
+    if (live && key == lk) rank = base + __popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return rank;
+}
 
 // Philox4x32-10 (Salmon et al., SC'11; Random123 constants): ten rounds of the
 // multiply-xor bijection on the counter (c0..c3) with a Weyl-sequence key.

@@ -50,26 +50,6 @@ __device__ __forceinline__ double lj_ff_masked(double r2, bool in, double ff1, d
 
 // ------------------------------------------------------------------ binning
 
-// Rank of this lane among all particles counted into cell `key`: lanes of one
-// wavefront holding the same key share one atomicAdd (a wavefront spans only a
-// few cells because particles arrive in cell order) and are ranked in lane order.
-__device__ __forceinline__ int wave_aggregated_rank(int key, bool live, int* __restrict__ count) {
-  const int lane = threadIdx.x & 63;
-  unsigned long long todo = __ballot(live);
-  int rank = 0;
-  while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const int lk = __shfl(key, leader, 64);
-    const unsigned long long same = __ballot(live && key == lk);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&count[lk], __popcll(same));
-    base = __shfl(base, leader, 64);
-    if (live && key == lk) rank = base + __popcll(same & ((1ull << lane) - 1ull));
-    todo &= ~same;
-  }
-  return rank;
-}
-
 // Fold every coordinate (OrthorhombicBC::foldCoordinate, src/bc/OrthorhombicBC.cpp:144-161)
 // and map the folded position to its inner cell (CellGrid::mapPositionToCellChecked
 // for one node, src/storage/CellGrid.cpp:98-136).  Cell ranks come from atomics;
