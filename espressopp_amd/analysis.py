@@ -1,4 +1,5 @@
-"""espressopp.analysis: Temperature, Pressure, PressureTensor, NPart, RadialDistrF.
+"""espressopp.analysis: Temperature, Pressure, PressureTensor, NPart, RadialDistrF,
+Configurations.
 
 Reference: src/analysis/Temperature.hpp:60-125 (T = sum m v^2 / (3 N), no
 centre-of-mass correction), src/analysis/Pressure.cpp:41-121
@@ -10,6 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from ._vec3 import Real3D
 
 
 class Observable:
@@ -95,3 +97,78 @@ class RadialDistrF(Observable):
         out = np.zeros(int(rdfN))
         self._h().call("espp_rdf", int(rdfN), out.ctypes.data_as(ctypes.c_void_p))
         return out.tolist()
+
+
+class Configuration:
+    """One gathered snapshot (src/analysis/Configuration.cpp:103-140): coordinates,
+    velocities and forces by particle id; an id that is not stored gives (0, 0, 0),
+    as the reference's map lookup does."""
+
+    def __init__(self, ids, pos=None, vel=None, force=None):
+        self._index = {int(p): k for k, p in enumerate(ids)}
+        self._ids = [int(p) for p in ids]
+        self._pos, self._vel, self._force = pos, vel, force
+
+    def _get(self, arr, what, pid):
+        if arr is None:
+            print("Error: This configuration has no information about %s" % what)
+            return Real3D(0.0, 0.0, 0.0)
+        k = self._index.get(int(pid))
+        return Real3D(0.0, 0.0, 0.0) if k is None else Real3D(arr[k])
+
+    def getCoordinates(self, pid):
+        return self._get(self._pos, "coordinates", pid)
+
+    __getitem__ = getCoordinates
+
+    def getVelocities(self, pid):
+        return self._get(self._vel, "velocities", pid)
+
+    def getForces(self, pid):
+        return self._get(self._force, "forces", pid)
+
+    def getIds(self):
+        return list(self._ids)
+
+    @property
+    def size(self):
+        return len(self._ids)
+
+
+class Configurations(Observable):
+    """analysis.Configurations(system, pos, vel, force, radius, folded)
+    (src/analysis/Configurations.py:76-98, Configurations.cpp:120-190): gather()
+    appends a snapshot of the particles; [k] is the k-th snapshot.  folded=False
+    stores unfolded positions (position + image * box)."""
+
+    def __init__(self, system, pos=True, vel=False, force=False, radius=False, folded=True):
+        Observable.__init__(self, system)
+        self._flags = (pos, vel, force)
+        self._folded = folded
+        self._snaps = []
+        self.capacity = 0
+
+    def gather(self):
+        a = self.system.storage.getParticleArrays()
+        p = a["pos"] if self._folded else a["pos"] + a["image"] * np.asarray(list(self.system.bc.boxL))
+        snap = Configuration(a["id"], p if self._flags[0] else None, a["v"] if self._flags[1] else None,
+                             a["f"] if self._flags[2] else None)
+        if self.capacity and len(self._snaps) >= self.capacity:
+            self._snaps.pop(0)
+        self._snaps.append(snap)
+
+    def __getitem__(self, k):
+        return self._snaps[k]
+
+    def __iter__(self):
+        return iter(self._snaps)
+
+    def back(self):
+        return self._snaps[-1]
+
+    def clear(self):
+        self._snaps = []
+
+    @property
+    def size(self):
+        return len(self._snaps)

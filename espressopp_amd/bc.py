@@ -12,6 +12,11 @@ class OrthorhombicBC:
     def getBoxL(self):
         return Real3D(self.boxL)
 
+    def getRandomPos(self):
+        """A uniform position in the box: boxL * (rng(), rng(), rng())
+        (OrthorhombicBC::getRandomPos, src/bc/OrthorhombicBC.cpp:171-178)."""
+        return Real3D([self.boxL[k] * self.rng() for k in range(3)])
+
     def getFoldedPosition(self, pos, imageBox=None):
         """foldCoordinate for x, y, z (src/bc/OrthorhombicBC.cpp:144-161)."""
         p = [float(x) for x in pos]

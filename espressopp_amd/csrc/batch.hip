@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kPartThreads) void kb_scatter(const BlockMap* __res
 // Each cell ordered by particle id, so that the slot order does not depend on
 // the atomics' order: one wavefront per cell, up to four entries per lane,
 // ranks by comparison.  A cell map block covers kSortCells cells of one replica.
-constexpr int kSortCells = 64;
+constexpr int kSortCells = 16;
 __global__ __launch_bounds__(kPartThreads) void kb_sort_cells(const BlockMap* __restrict__ cm,
                                                               const RepDesc* __restrict__ rd,
                                                               RepState* __restrict__ st,
@@ -343,17 +343,25 @@ __global__ __launch_bounds__(kPartThreads) void kb_sort_cells(const BlockMap* __
       key[u] = k < mlen ? id[s0 + ent[u]] : INT_MAX;
     }
     int rk[4] = {0, 0, 0, 0};
+    if (mlen <= 64) {
+      // one entry per lane (the common case)
+      for (int q = 0; q < mlen; ++q) {
+        const int other = __shfl(key[0], q, 64);
+        rk[0] += other < key[0] || (other == key[0] && q < lane);
+      }
+    } else {
 #pragma unroll
-    for (int u2 = 0; u2 < 4; ++u2) {
-      if (64 * u2 >= mlen) break;     // wavefront-uniform
-      for (int q = 0; q < 64; ++q) {
-        const int other = __shfl(key[u2], q, 64);
-        const int ok = q + 64 * u2;
+      for (int u2 = 0; u2 < 4; ++u2) {
+        if (64 * u2 >= mlen) break;     // wavefront-uniform
+        for (int q = 0; q < 64; ++q) {
+          const int other = __shfl(key[u2], q, 64);
+          const int ok = q + 64 * u2;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int k = lane + 64 * u;
-          // ids are unique within a replica; ties (padding) keep the entry index order
-          rk[u] += (ok < mlen) && (other < key[u] || (other == key[u] && ok < k));
+          for (int u = 0; u < 4; ++u) {
+            const int k = lane + 64 * u;
+            // ids are unique within a replica; ties (padding) keep the entry index order
+            rk[u] += (ok < mlen) && (other < key[u] || (other == key[u] && ok < k));
+          }
         }
       }
     }
@@ -449,16 +457,25 @@ __global__ __launch_bounds__(kPartThreads) void kb_build(const BlockMap* __restr
           const double shx = (double)sx * L, shy = (double)sy * L, shz = (double)sz * L;
           const int c2 = d.cell0 + qx + nc * (qy + nc * qz);
           const int jb = s0 + cstart[c2], je = s0 + cstart[c2 + 1];
-          for (int j = jb; j < je; ++j) {
-            if (j == i && unshifted) continue;
-            const double4 pj = pos[j];
-            double dx, dy, dz;
-            if (fwd) { dx = pi.x - (pj.x + shx); dy = pi.y - (pj.y + shy); dz = pi.z - (pj.z + shz); }
-            else { dx = pj.x - (pi.x - shx); dy = pj.y - (pi.y - shy); dz = pj.z - (pi.z - shz); }
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 <= cutsq) {
-              if (k < cap) out[(size_t)k * 64] = j;
-              ++k;
+          // candidates eight at a time: their loads are issued together
+          for (int j0 = jb; j0 < je; j0 += 8) {
+            double4 pc[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) pc[u] = pos[min(j0 + u, je - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int j = j0 + u;
+              if (j >= je) break;
+              if (j == i && unshifted) continue;
+              const double4 pj = pc[u];
+              double dx, dy, dz;
+              if (fwd) { dx = pi.x - (pj.x + shx); dy = pi.y - (pj.y + shy); dz = pi.z - (pj.z + shz); }
+              else { dx = pj.x - (pi.x - shx); dy = pj.y - (pi.y - shy); dz = pj.z - (pi.z - shz); }
+              const double d2 = dx * dx + dy * dy + dz * dz;
+              if (d2 <= cutsq) {
+                if (k < cap) out[(size_t)k * 64] = j;
+                ++k;
+              }
             }
           }
         }
@@ -525,9 +542,7 @@ __global__ __launch_bounds__(kPartThreads) void kb_force(const BlockMap* __restr
     const int* np = nbr + (size_t)(i >> 6) * (size_t)cap * 64 + (i & 63);
     const int c = min(cnt[i], cap);
     const double qs = d.qsd, ff1 = d.ff1, ff2 = d.ff2, rc2 = d.rc2, lo = d.dlo, hi = d.dhi;
-    for (int k = q; k < c; k += lpp) {
-      const int j = np[(size_t)k * 64];
-      const uint4 qj = posq[j];
+    auto pair = [&](int j, const uint4& qj) {
       const double ex = (double)(int)(qi.x - qj.x) * qs;
       const double ey = (double)(int)(qi.y - qj.y) * qs;
       const double ez = (double)(int)(qi.z - qj.z) * qs;
@@ -546,6 +561,23 @@ __global__ __launch_bounds__(kPartThreads) void kb_force(const BlockMap* __restr
         E += d.eps4 * (f6 * f6 - f6) - d.shift;
         W += r2 * ff;
       }
+    };
+    // entries four at a time (indices, then positions, issued together), in the
+    // same order as one at a time
+    int k = q;
+    for (; k + 3 * lpp < c; k += 4 * lpp) {
+      int jj[4];
+      uint4 qq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jj[u] = np[(size_t)(k + u * lpp) * 64];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) qq[u] = posq[jj[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pair(jj[u], qq[u]);
+    }
+    for (; k < c; k += lpp) {
+      const int j = np[(size_t)k * 64];
+      pair(j, posq[j]);
     }
   }
   // combine the lpp lanes of a particle (consecutive lanes, power of two)

@@ -112,24 +112,30 @@ struct PairCtx {
   int valid;
 };
 
-// Geometry of pair (owner slot i, its k-th neighbour) for the thread with q == 0;
-// invalid pairs (past the list or r >= rc) get r = rc so that their features stay finite.
-__device__ __forceinline__ PairCtx pair_geom(int i, bool owner_live, int k, int cnt_i, const int* __restrict__ nbr,
-                                             int cap, const double4* __restrict__ pos, const Geom& g, float rc,
-                                             int& j_out) {
+// The neighbour data of one tile (read by the thread with q == 0): its slot, its
+// position and its densities.  Entries past the owner's list read the owner itself.
+struct Nbr {
+  int j;
+  double4 p;
+  float2 rho;
+};
+
+__device__ __forceinline__ int nbr_slot(int i, bool live, int k, int cnt_i, const int* __restrict__ nbr, int cap) {
+  return (live && k < cnt_i) ? nbr[((size_t)(i >> 6) * cap + k) * 64 + (i & 63)] : i;
+}
+
+// Geometry of the pair (owner i at pi, neighbour nb); a pair past the list, at
+// r >= rc or at r = 0 is invalid and gets r = rc so that its features stay finite.
+__device__ __forceinline__ PairCtx pair_geom(const double4& pi, int i, const Nbr& nb, const Geom& g, float rc) {
   PairCtx c{0.f, 0.f, 0.f, rc, 0};
-  j_out = i;
-  if (owner_live && k < cnt_i) {
-    const int j = nbr[((size_t)(i >> 6) * cap + k) * 64 + (i & 63)];
-    const double4 pi = pos[i], pj = pos[j];
-    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+  if (nb.j != i) {
+    double dx = pi.x - nb.p.x, dy = pi.y - nb.p.y, dz = pi.z - nb.p.z;
     dx -= g.L[0] * __builtin_rint(dx * g.invL[0]);
     dy -= g.L[1] * __builtin_rint(dy * g.invL[1]);
     dz -= g.L[2] * __builtin_rint(dz * g.invL[2]);
     const float r = (float)sqrt(dx * dx + dy * dy + dz * dz);
     if (r < rc && r > 0.f) {
       c.dx = (float)dx; c.dy = (float)dy; c.dz = (float)dz; c.r = r; c.valid = 1;
-      j_out = j;
     }
   }
   return c;
@@ -262,14 +268,33 @@ __global__ __launch_bounds__(kThreads) void k_mlp(int n, const double4* __restri
   const int col = 16 * w + (l & 15);             // output column of this lane in the GEMMs
   const int rq = 4 * (l >> 4);                   // first of its four rows in each 16-row block
 
+  // Neighbour data is software-pipelined: the slot two tiles ahead and the
+  // position and densities one tile ahead are loaded while the current tile's
+  // GEMMs run, so the dependent global loads do not stall each tile.
+  const bool loader = live && q == 0;
+  const int ic = live ? i : 0;
+  const double4 pi = pos[ic];
+  Nbr cur, nxt;
+  cur.j = nbr_slot(ic, loader, 0, cnt_i, nbr, cap);
+  int j2 = nbr_slot(ic, loader, 1, cnt_i, nbr, cap);
+  cur.p = pos[cur.j];
+  cur.rho = rho[cur.j];
   for (int k = 0; k < kmax; ++k) {
-    int j;
-    PairCtx pc = pair_geom(i, live && q == 0, k, cnt_i, nbr, cap, pos, g, sp.rc, j);
+    nxt.j = j2;
+    nxt.p = pos[j2];
+    nxt.rho = rho[j2];
+    j2 = nbr_slot(ic, loader, k + 2, cnt_i, nbr, cap);
+    PairCtx pc = pair_geom(pi, ic, cur, g, sp.rc);
+    if (!loader) pc.valid = 0;
     // the thread with q == 0 computed the geometry; share it with the pair's other three
     const int src = l & ~3;
     pc.dx = __shfl(pc.dx, src, 64); pc.dy = __shfl(pc.dy, src, 64); pc.dz = __shfl(pc.dz, src, 64);
-    pc.r = __shfl(pc.r, src, 64); pc.valid = __shfl(pc.valid, src, 64); j = __shfl(j, src, 64);
-    const float2 rho_j = pc.valid ? rho[j] : rho_i;
+    pc.r = __shfl(pc.r, src, 64); pc.valid = __shfl(pc.valid, src, 64);
+    float2 rho_j;
+    rho_j.x = __shfl(cur.rho.x, src, 64);
+    rho_j.y = __shfl(cur.rho.y, src, 64);
+    if (!pc.valid) rho_j = rho_i;
+    cur = nxt;
     const float u0 = (0.5f * (rho_i.x + rho_j.x) - sp.mean[0]) * sp.inv_scale[0];
     const float u1 = (0.5f * (rho_i.y + rho_j.y) - sp.mean[1]) * sp.inv_scale[1];
     write_features<TRAIN>(p, q, pc.r, u0, u1, sp, B0, Xt);
