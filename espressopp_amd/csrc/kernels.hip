@@ -1532,6 +1532,103 @@ __global__ __launch_bounds__(256) void k_lj_tiled_fp64(
   }
 }
 
+// All-FP64 LJ for one particle type, two lanes per particle: a tile of T
+// particles runs 2T threads (the two lanes of a particle take alternate
+// 8-entry groups of its list and add their sums with one shuffle), and the
+// staged region holds x, y, z only (24 bytes per particle instead of 32).  A
+// group's eight LDS reads are issued before its arithmetic.  Per pair the
+// arithmetic is the single-lane kernel's (minimum image by rint, FP64 r^2
+// without contraction, the reference's test for pairs at the cutoff).
+struct P3 {
+  double x, y, z;
+};
+
+template <bool ENERGY, bool ACCUM>
+__global__ __launch_bounds__(256) void k_lj_tiled_fp64_pair(
+    int n, int ntiles, int T, const double4* __restrict__ pos, const uint4* __restrict__ nbr,
+    const int* __restrict__ cnt, int capg, Geom g, const TileRun* __restrict__ runs, const int* __restrict__ nruns,
+    const int* __restrict__ lane_slot, const LJParam* __restrict__ prm, double* __restrict__ fx,
+    double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ partial) {
+  extern __shared__ uint4 lds_raw[];
+  P3* lds = reinterpret_cast<P3*>(lds_raw);
+  __shared__ RunRef sr[kMaxRuns + 1];
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int nr = stage_region(load_run_table(runs, nruns, tile), pos, lds, sr,
+                              [](const double4& v) { return P3{v.x, v.y, v.z}; }, [] {});
+  const int h = (int)threadIdx.x & 1;               // which alternate groups this lane takes
+  const int L = tile * T + ((int)threadIdx.x >> 1); // list lane
+  const bool live = L < n;
+  double acc[kNPart];
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
+  double Ax = 0.0, Ay = 0.0, Az = 0.0;
+  int i = 0;
+  if (live) {
+    i = lane_slot[L];
+    const double4 pi = pos[i];
+    const uint4* np = nbr + tiled_group(i, capg, 0);
+    const int c = nr > 0 ? min(cnt[i], 8 * capg) : 0;
+    const LJParam& p = prm[0];
+    const double ff1 = p.dff1, ff2 = p.dff2, rc2 = p.drc2, lo = p.dlo, hi = p.dhi;
+    const int set = p.set;
+    const int ng = (c + 7) >> 3;
+    uint4 wn = h < ng ? np[(size_t)h * 64] : make_uint4(0u, 0u, 0u, 0u);
+    for (int kg = h; kg < ng; kg += 2) {
+      const uint32_t w[4] = {wn.x, wn.y, wn.z, wn.w};
+      wn = (kg + 2 < ng) ? np[(size_t)(kg + 2) * 64] : make_uint4(0u, 0u, 0u, 0u);
+      P3 qa[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
+        qa[u] = lds[e];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (8 * kg + u >= c) break;
+        double dx = pi.x - qa[u].x, dy = pi.y - qa[u].y, dz = pi.z - qa[u].z;
+        dx = __builtin_fma(-g.L[0], __builtin_rint(dx * g.invL[0]), dx);
+        dy = __builtin_fma(-g.L[1], __builtin_rint(dy * g.invL[1]), dy);
+        dz = __builtin_fma(-g.L[2], __builtin_rint(dz * g.invL[2]), dz);
+        const double r2 = sqr_nc(dx, dy, dz);
+        bool in = set && r2 <= rc2;
+        if (set && r2 >= lo && r2 <= hi) {   // at the cutoff (rare)
+          const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
+          const int j = slot_of_lds(sr, nr, e);
+          in = ref_pair_r2(i, j, pi, pos[j], g) <= rc2;
+        }
+        const double inv = rcp_nr(r2);
+        const double r6 = inv * inv * inv;
+        double ff = r6 * (ff1 * r6 - ff2) * inv;
+        ff = in ? ff : 0.0;
+        Ax = __builtin_fma(dx, ff, Ax);
+        Ay = __builtin_fma(dy, ff, Ay);
+        Az = __builtin_fma(dz, ff, Az);
+        if (ENERGY) {
+          const double f2 = p.sig2 * inv;
+          const double f6 = f2 * f2 * f2;
+          acc[0] += in ? p.eps4 * (f6 * f6 - f6) - p.dshift : 0.0;
+          acc[1] += r2 * ff;
+          acc[2] += dx * dx * ff; acc[3] += dy * dy * ff; acc[4] += dz * dz * ff;
+          acc[5] += dx * dy * ff; acc[6] += dx * dz * ff; acc[7] += dy * dz * ff;
+        }
+      }
+    }
+  }
+  if (!ENERGY) {
+    Ax += __shfl_xor(Ax, 1, 64);
+    Ay += __shfl_xor(Ay, 1, 64);
+    Az += __shfl_xor(Az, 1, 64);
+    if (live && h == 0) {
+      if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
+      else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
+    }
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < kNPart; ++q) acc[q] *= 0.5;
+  block_reduce_store<kNPart>(acc, partial);
+}
+
 // ------------------------------------------------------------------ Tabulated
 //
 // interaction.Tabulated over a Verlet list or a fixed pair list
@@ -2257,6 +2354,15 @@ void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool 
     if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }
     else if (onetype) { if (accum) LJM(true, false, true); else LJM(true, false, false); }
     else { if (accum) LJM(false, false, true); else LJM(false, false, false); }
+  } else if (onetype && 2 * T <= 256) {
+    // two lanes per particle, 24-byte slots: the staged region needs 3/4 of lds_bytes
+    const int lb = lds_bytes / 32 * 24;
+    dim3 block2(2 * T);
+#define LJP(EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64_pair<EN, AC>), grid, block2, lb, s, n, ntiles, T, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, fx, fy, fz, partial)
+    if (energy) LJP(true, false);
+    else if (accum) LJP(false, true);
+    else LJP(false, false);
+#undef LJP
   } else {
     if (energy) { if (onetype) LJD(true, true, false); else LJD(false, true, false); }
     else if (onetype) { if (accum) LJD(true, false, true); else LJD(true, false, false); }

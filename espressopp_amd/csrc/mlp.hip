@@ -50,9 +50,12 @@ __device__ __forceinline__ unsigned short bf16_bits(float f) {
 __device__ __forceinline__ float bf16_val(unsigned short b) { return __uint_as_float((unsigned int)b << 16); }
 __device__ __forceinline__ float bf16_round(float f) { return bf16_val(bf16_bits(f)); }
 
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+// sigmoid with the hardware reciprocal (about 1 ulp): the results are rounded to
+// bfloat16 right after, so an IEEE division would buy nothing
+__device__ __forceinline__ float sigmoid_f(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_f(float z) { return z * sigmoid_f(z); }
 __device__ __forceinline__ float dsilu_f(float z) {
-  const float sg = 1.0f / (1.0f + __expf(-z));
+  const float sg = sigmoid_f(z);
   return sg * (1.0f + z * (1.0f - sg));
 }
 __device__ __forceinline__ float cosine_cut(float r, float R) {

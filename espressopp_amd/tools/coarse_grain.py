@@ -160,9 +160,9 @@ def ibi_update(r, V, g_model, g_target, kT, rc, alpha=0.5):
 
 # ------------------------------------------------------------------ systems
 
-def _system(espressopp, box, cutoffs, skin, pos, seed, mass=None):
+def _system(espressopp, box, cutoffs, skin, pos, seed, mass=None, precision="fp64"):
     from .decomp import cellGrid
-    system = espressopp.System(precision="fp64")
+    system = espressopp.System(precision=precision)
     system.rng = espressopp.esutil.RNG(seed)
     system.bc = espressopp.bc.OrthorhombicBC(system.rng, box)
     system.skin = skin
@@ -197,18 +197,20 @@ def random_walk_melt(nchains, nbeads, rho, seed, bond=0.97):
 
 
 def melt_frames(espressopp, nchains, nbeads, nframes, stride, seed=1, rho=0.85, gamma=0.5, T=1.0, dt=0.01,
-                pushoff_steps=2000, equil_steps=4000, tmpdir=None):
+                pushoff_steps=2000, equil_steps=4000, tmpdir=None, on_frame=None, precision="fp64"):
     """A Kremer-Grest melt run by the plain method: random-walk chains, a soft
     push-off (tabulated cosine repulsion growing in strength, with FENE bonds),
     then WCA (rc 2^(1/6)) + FENE (K 30, R0 1.5) under Langevin (T, gamma) with time
     step dt.  Returns the box, the bonds and nframes frames taken every stride
-    steps, each (unwrapped positions, conservative forces) in id order."""
+    steps, each (unwrapped positions, conservative forces) in id order; with
+    on_frame, each frame is handed to on_frame(pos, force) and its result kept
+    instead."""
     import tempfile
     pos, box, bonds = random_walk_melt(nchains, nbeads, rho, seed)
     rc = 2.0 ** (1.0 / 6.0)
     skin = 0.4
     tmp = tmpdir or tempfile.mkdtemp()
-    system = _system(espressopp, box, (rc,), skin, pos, seed)
+    system = _system(espressopp, box, (rc,), skin, pos, seed, precision=precision)
     fpl = espressopp.FixedPairList(system.storage)
     fpl.addBonds(bonds.tolist())
     fene = espressopp.interaction.FixedPairListFENE(system, fpl, espressopp.interaction.FENE(30.0, 0.0, 1.5))
@@ -239,8 +241,9 @@ def melt_frames(espressopp, nchains, nbeads, nframes, stride, seed=1, rho=0.85, 
     for _ in range(nframes):
         integ.run(stride)
         system.storage._h.call("espp_compute_forces")
-        a = system.storage.getParticleArrays()
-        frames.append((a["pos"] + a["image"] * L, a["f"].copy()))
+        h = system.storage._h
+        p, f = h.get("pos") + h.get("image") * L, h.get("f")
+        frames.append(on_frame(p, f) if on_frame else (p, f))
     return box, bonds, frames
 
 
@@ -324,9 +327,11 @@ def ibi(espressopp, box, bonds, start, bond_rows, r, g_target, tmpdir, kT=1.0, r
 
 
 def train_pair_mlp(espressopp, box, bonds, frames, residual, spec, theta0, epochs=40, lr=2e-3, seed=3,
-                   tmpdir=None, bond_rows=None):
+                   tmpdir=None, bond_rows=None, batch_frames=None):
     """Force matching of the learned pair force on (positions, residual force)
-    frames: one Adam step per epoch on the mean loss over the frames."""
+    frames: each epoch visits the frames in a shuffled order and takes one Adam
+    step per batch_frames frames (all frames when None) on their mean loss.
+    Returns the parameters and the mean loss of every epoch."""
     theta = np.asarray(theta0, dtype=np.float32)
     from .decomp import cellGrid
     system = espressopp.System(precision="fp64")
@@ -344,16 +349,21 @@ def train_pair_mlp(espressopp, box, bonds, frames, residual, spec, theta0, epoch
     mlp.setParameters(theta)
     system.addInteraction(mlp)
     ids = np.arange(n, dtype=np.int32)
+    nf = len(frames)
+    bsz = nf if not batch_frames else max(1, min(int(batch_frames), nf))
+    rng = np.random.default_rng(seed)
     history = []
-    w = 1.0 / len(frames)
     for ep in range(epochs):
-        mlp.zeroGradient()
+        order = rng.permutation(nf) if bsz < nf else np.arange(nf)
         tot = 0.0
-        for p, fr in zip(frames, residual):
-            system.storage.setParticleArrays("pos", ids, p)
-            system.storage.decompose()
-            tot += mlp.trainFrame(fr, weight=w)
-        mlp.adamStep(lr=lr)
+        for b0 in range(0, nf, bsz):
+            chunk = order[b0:b0 + bsz]
+            mlp.zeroGradient()
+            for k in chunk:
+                system.storage.setParticleArrays("pos", ids, frames[k])
+                system.storage.decompose()
+                tot += mlp.trainFrame(residual[k], weight=1.0 / len(chunk)) * len(chunk) / nf
+            mlp.adamStep(lr=lr)
         history.append(tot)
     return mlp.getParameters(), history
 
@@ -392,3 +402,89 @@ def density_norm(frames, box, bonds, radii):
             vals[a].append(rho)
     return tuple((float(np.mean(v)), float(max(np.std(v), 1e-3))) for v in (np.concatenate(vals[0]),
                                                                                np.concatenate(vals[1])))
+
+
+def main(argv=None):
+    """The coarse-grained workload end to end: melt frames by the plain method,
+    4 -> 1 mapping, Boltzmann-inverted bonds, IBI, force matching of the learned
+    pair force, and the judge; prints progress and one JSON line of results."""
+    import argparse
+    import json
+    import tempfile
+    import time
+
+    import espressopp_amd as espressopp
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--chains", type=int, default=5000)
+    ap.add_argument("--beads", type=int, default=200)
+    ap.add_argument("--per-blob", type=int, default=4)
+    ap.add_argument("--frames", type=int, default=500)
+    ap.add_argument("--stride", type=int, default=100)
+    ap.add_argument("--test-frames", type=int, default=50)
+    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--batch-frames", type=int, default=10)
+    ap.add_argument("--ibi-iterations", type=int, default=4)
+    ap.add_argument("--ibi-steps", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=5)
+    args = ap.parse_args(argv)
+    log = lambda *a: print(*a, flush=True)  # noqa: E731
+    tmp = tempfile.mkdtemp()
+    t0 = time.perf_counter()
+    nch, nb, m = args.chains, args.beads, args.per_blob
+    total = args.frames + args.test_frames
+
+    box, bonds, raw = melt_frames(espressopp, nch, nb, total, args.stride, seed=args.seed, tmpdir=tmp,
+                                  precision="mixed",
+                                  on_frame=lambda p, f: blob_map(p, f, nch, nb, m, _box_of(nch * nb)))
+    pos = [x[0] for x in raw]
+    F = [x[1] for x in raw]
+    del raw
+    cgb = cg_bonds(nch, nb, m)
+    train, test = list(range(args.frames)), list(range(args.frames, total))
+    log("melt: %d beads, %d frames mapped to %d sites (%.1fs)" % (nch * nb, total, len(pos[0]),
+                                                                  time.perf_counter() - t0))
+    sample = [pos[k] for k in train[:: max(1, len(train) // 25)]]
+    r, g_t = pair_rdf(sample, box, 3.0, 60, exclude=cgb)
+    bond_rows = bond_table(sample, cgb, box, kT=1.0)
+    t1 = time.perf_counter()
+    ibi_rows, ibi_hist = ibi(espressopp, box, cgb, pos[train[-1]], bond_rows, r, g_t, tmp,
+                             iterations=args.ibi_iterations, steps=args.ibi_steps, every=100)
+    log("IBI: RDF error by iteration %s (%.1fs)" % (["%.4f" % e for e in ibi_hist], time.perf_counter() - t1))
+    bonds_only = CGModel(espressopp, box, cgb, pos[0], bond_rows, tmp, tag="bonds")
+    resid = [F[k] - bonds_only.forces(pos[k]) for k in range(total)]
+    radii = (1.5, 3.0)
+    spec = dict(cutoff=3.0, densityRadii=radii, densityNorm=density_norm(pos, box, cgb, radii))
+    theta0 = espressopp.interaction.VerletListPairMLP.initialParameters(seed=1, scale=0.5)
+    t2 = time.perf_counter()
+    theta, hist = train_pair_mlp(espressopp, box, cgb, [pos[k] for k in train], [resid[k] for k in train], spec,
+                                 theta0, epochs=args.epochs, lr=args.lr, batch_frames=args.batch_frames)
+    t_train = time.perf_counter() - t2
+    log("training: %d epochs x %d frames (Adam step per %d frames), loss %.4f -> %.4f (%.1fs)"
+        % (args.epochs, len(train), args.batch_frames, hist[0], hist[-1], t_train))
+    fb = [F[k] - resid[k] for k in test]
+    ibi_model = CGModel(espressopp, box, cgb, pos[0], bond_rows, tmp, pair_rows=ibi_rows, tag="ibi_eval")
+    mlp_model = CGModel(espressopp, box, cgb, pos[0], bond_rows, tmp, mlp=(spec, theta), tag="mlp_eval")
+    e_bond = force_error(fb, [F[k] for k in test])
+    e_ibi = force_error([ibi_model.forces(pos[k]) for k in test], [F[k] for k in test])
+    e_mlp = force_error([mlp_model.forces(pos[k]) for k in test], [F[k] for k in test])
+    _, g_ibi = pair_rdf(ibi_model.sample(pos[test[0]], args.ibi_steps, 100), box, 3.0, 60, exclude=cgb)
+    _, g_mlp = pair_rdf(mlp_model.sample(pos[test[0]], args.ibi_steps, 100), box, 3.0, 60, exclude=cgb)
+    out = {"beads": nch * nb, "sites": len(pos[0]), "train_frames": len(train), "test_frames": len(test),
+           "epochs": args.epochs, "batch_frames": args.batch_frames, "lr": args.lr, "train_s": t_train, "train_s_per_epoch": t_train / args.epochs,
+           "loss_first": hist[0], "loss_last": hist[-1],
+           "force_error": {"bonds_only": e_bond, "ibi": e_ibi, "learned": e_mlp},
+           "explained_force_variance": {"bonds_only": 1 - e_bond ** 2, "ibi": 1 - e_ibi ** 2, "learned": 1 - e_mlp ** 2},
+           "rdf_error": {"ibi": rdf_error(g_ibi, g_t), "learned": rdf_error(g_mlp, g_t)},
+           "ibi_rdf_error_by_iteration": ibi_hist, "wall_s": time.perf_counter() - t0}
+    log(json.dumps(out))
+    return out
+
+
+def _box_of(n, rho=0.85):
+    L = (n / rho) ** (1.0 / 3.0)
+    return (L, L, L)
+
+
+if __name__ == "__main__":
+    main()

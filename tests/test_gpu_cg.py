@@ -45,7 +45,7 @@ def test_learned_cg_force_matching_against_ibi(gpu, tmp_path):
     spec = dict(cutoff=3.0, densityRadii=radii, densityNorm=cg.density_norm(pos, box, cgb, radii))
     theta0 = espressopp.interaction.VerletListPairMLP.initialParameters(seed=1, scale=0.5)
     theta, hist = cg.train_pair_mlp(espressopp, box, cgb, [pos[k] for k in train], [resid[k] for k in train], spec,
-                                    theta0, epochs=200, lr=1e-2)
+                                    theta0, epochs=60, lr=3e-3, batch_frames=6)
     print("training loss: %.4f -> %.4f" % (hist[0], hist[-1]), flush=True)
     assert hist[-1] < hist[0]
     ibi_model = cg.CGModel(espressopp, box, cgb, pos[0], bond_rows, tmp, pair_rows=ibi_rows, tag="ibi_eval")
