@@ -63,14 +63,13 @@ def pair_rdf(frames, box, rmax, nbins, exclude=None):
     ex = None
     if exclude is not None and len(exclude):
         e = np.sort(np.asarray(exclude, dtype=np.int64), axis=1)
-        ex = set((e[:, 0] * n + e[:, 1]).tolist())
+        ex = np.unique(e[:, 0] * n + e[:, 1])
     for p in frames:
         t = cKDTree(np.mod(p, L), boxsize=L)
         pr = t.query_pairs(rmax, output_type="ndarray")
         if ex is not None:
             key = np.minimum(pr[:, 0], pr[:, 1]).astype(np.int64) * n + np.maximum(pr[:, 0], pr[:, 1])
-            keep = np.fromiter((k not in ex for k in key.tolist()), dtype=bool, count=len(key))
-            pr = pr[keep]
+            pr = pr[~np.isin(key, ex, assume_unique=False)]
         d = _min_image(p[pr[:, 0]] - p[pr[:, 1]], L)
         hist += np.histogram(np.sqrt(np.sum(d * d, axis=1)), bins=edges)[0]
     npairs = n * (n - 1) / 2 - (len(exclude) if exclude is not None else 0)
@@ -387,14 +386,13 @@ def density_norm(frames, box, bonds, radii):
     L = float(box[0])
     n = len(frames[0])
     b = np.sort(np.asarray(bonds, dtype=np.int64), axis=1)
-    ex = set((b[:, 0] * n + b[:, 1]).tolist())
+    ex = np.unique(b[:, 0] * n + b[:, 1])
     vals = [[], []]
     for p in frames[: max(1, min(len(frames), 5))]:
         t = cKDTree(np.mod(p, L), boxsize=L)
         pr = t.query_pairs(max(radii), output_type="ndarray")
         key = np.minimum(pr[:, 0], pr[:, 1]).astype(np.int64) * n + np.maximum(pr[:, 0], pr[:, 1])
-        keep = np.fromiter((k not in ex for k in key.tolist()), dtype=bool, count=len(key))
-        pr = pr[keep]
+        pr = pr[~np.isin(key, ex)]
         d = np.linalg.norm(_min_image(p[pr[:, 0]] - p[pr[:, 1]], L), axis=1)
         for a, R in enumerate(radii):
             w = np.where(d < R, 0.5 * (1 + np.cos(np.pi * d / R)), 0.0)
