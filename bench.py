@@ -93,6 +93,14 @@ def kernel_stats(h, name):
     return cnt.value, ms.value
 
 
+def prune_stats(h, ia):
+    import ctypes
+    from espressopp_amd import _lib
+    p, w, e = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    _lib.check(h.lib.espp_prune_stats(h.h, ia._id, ctypes.byref(p), ctypes.byref(w), ctypes.byref(e)))
+    return p.value, w.value, e.value
+
+
 def vl_entries(h, vl):
     import ctypes
     from espressopp_amd import _lib
@@ -302,6 +310,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--prof-steps", type=int, default=100,
                     help="NVE steps after the timed run with per-kernel HIP events")
+    ap.add_argument("--prune-shell", type=float, default=None,
+                    help="shell width of the pruned LJ lists (0: none; default: the library's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched parallel-tempering measurement")
@@ -340,6 +350,8 @@ def main():
     # mixed path: Langevin equilibration (untimed), then timed NVE
     t_setup = time.perf_counter()
     system, vl, lj, integ = make_system(espressopp, box, cells, rc, skin, "mixed", pos, vel)
+    if args.prune_shell is not None:
+        system.storage._h.call("espp_set_prune_shell", float(args.prune_shell))
     log("setup mixed: %.2fs" % (time.perf_counter() - t_setup))
     lang = espressopp.integrator.LangevinThermostat(system)
     lang.gamma = 1.0
@@ -384,15 +396,24 @@ def main():
     integ.run(args.prof_steps)
     h.call("espp_prof_enable", 0)
     # The dominant kernel: the LJ pair kernel that also finishes the step (every step of
-    # a run but the last).  Algorithmic bytes per particle: own mirror 16 + 4 per list
-    # entry (int32 index) + x, v, mass/type read 56 + x, v, mirror written 64; the
-    # force itself stays in registers.
-    launches, pair_ms = kernel_stats(h, "pair_step")
+    # a run but the last), over all its launches: those that walk the full list and
+    # write its pruned copy, and those that walk the pruned list.  Algorithmic bytes
+    # per particle, the reference's full-list walk: own mirror 16 + 4 per list entry
+    # (int32 index) + x, v, mass/type read 56 + x, v, mirror written 64; the force
+    # itself stays in registers.
+    l_full, ms_full = kernel_stats(h, "pair_step")
+    l_pruned, ms_pruned = kernel_stats(h, "pair_step_pruned")
+    launches, pair_ms = l_full + l_pruned, ms_full + ms_pruned
     entries, cap, maxcount = vl_entries(h, vl)
     nbar = entries / n
     b_force = 16 + 4 * nbar + 56 + 64
     avg_s = pair_ms / max(launches, 1) / 1e3
     achieved = b_force * n / avg_s / 1e9
+    prunes, pruned_walks, pruned_entries = prune_stats(h, lj)
+    # the same with the entries the launches walked: the full list on pruning launches,
+    # the current pruned list (a sample of its length) on the others
+    nbar_walked = (l_full * nbar + l_pruned * pruned_entries / n) / max(launches, 1)
+    b_walked = 16 + 4 * nbar_walked + 56 + 64
     stats = {k: kernel_stats(h, k) for k in ("pair", "nbr_build", "integrate1", "integrate2", "binning")}
     resorts = integ.getNumResorts()
     import ctypes
@@ -405,6 +426,9 @@ def main():
         (lb.value / n, wb.value * 64.0 / n))
     log("mixed: %.3f ms/step, %.4g particle-steps/s, pair+step kernel %.3f ms avg (%d launches), nbar %.2f, "
         "cap %d, resorts %d" % (dt_run / args.steps * 1e3, value, avg_s * 1e3, launches, nbar, cap, resorts))
+    log("  full-list walks (pruning) %d, %.3f ms avg; pruned-list walks %d, %.3f ms avg; pruned list %.2f "
+        "entries per particle" % (l_full, ms_full / max(l_full, 1), l_pruned, ms_pruned / max(l_pruned, 1),
+                                   pruned_entries / n))
     for k, (c, ms) in stats.items():
         log("  %-11s %5d launches, %.3f ms avg" % (k, c, ms / max(c, 1)))
     drift = None
@@ -485,7 +509,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_lj_tiled_mixed (step variant)", "bytes_per_particle": b_force, "nbar": nbar,
-                     "avg_launch_ms": avg_s * 1e3, "launches": launches},
+                     "avg_launch_ms": avg_s * 1e3, "launches": launches,
+                     "full_list_launches": l_full, "full_list_avg_ms": ms_full / max(l_full, 1),
+                     "pruned_list_launches": l_pruned, "pruned_list_avg_ms": ms_pruned / max(l_pruned, 1),
+                     "pruned_nbar": pruned_entries / n, "walked_nbar": nbar_walked,
+                     "walked_bytes_per_particle": b_walked,
+                     "walked_frac": b_walked * n / avg_s / 1e9 / HBM_PEAK_GBS},
         "fp64_path": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "accuracy": err,

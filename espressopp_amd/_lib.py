@@ -104,11 +104,14 @@ PROF_SIGNATURES = {
     "espp_prof_enable": [_vp, _c_int],
     "espp_prof_reset": [_vp],
     "espp_prof_stats": [_vp, _c_int, _P(_c_i64), _P(_c_dbl)],
+    "espp_set_prune_shell": [_vp, _c_dbl],
+    "espp_prune_stats": [_vp, _c_int, _P(_c_i64), _P(_c_i64), _P(_c_i64)],
     "espp_vl_stats": [_vp, _c_int, _P(_c_i64), _P(_c_int), _P(_c_int)],
     "espp_vl_batch_stats": [_vp, _c_int, _c_int, _P(_c_i64), _P(_c_i64)],
     "espp_tile_stats": [_vp, _P(_c_int), _P(_c_int)],
 }
-KERNEL = {"pair": 0, "nbr_build": 1, "integrate1": 2, "integrate2": 3, "binning": 4, "bond": 5, "pair_step": 6}
+KERNEL = {"pair": 0, "nbr_build": 1, "integrate1": 2, "integrate2": 3, "binning": 4, "bond": 5, "pair_step": 6,
+          "pair_step_pruned": 7}
 _RESTYPE = {"espp_last_error": ctypes.c_char_p, "espp_version": ctypes.c_char_p}
 
 _lib = None

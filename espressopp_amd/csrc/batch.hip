@@ -59,6 +59,7 @@ struct RepDesc {
   double L, invL, ics, qinv, qsd;
   double skin_half, cutsq;              // skin / 2 and (rc + skin)^2
   double ff1, ff2, rc2, shift, sig2, eps4, dlo, dhi;
+  double cqlo, cqhi;                    // (rc + skin)^2 window in fixed-point units^2 (list build)
   double pref1, pref2;                  // -gamma, sqrt(24 T gamma / dt)
   unsigned long long seed;
   int slot0, n, cell0, nc;
@@ -425,6 +426,7 @@ __global__ __launch_bounds__(kPartThreads) void kb_copy_back(const BlockMap* __r
 __global__ __launch_bounds__(kPartThreads) void kb_build(const BlockMap* __restrict__ bm,
                                                          const RepDesc* __restrict__ rd,
                                                          RepState* __restrict__ st, const double4* __restrict__ pos,
+                                                         const uint4* __restrict__ posq,
                                                          const int* __restrict__ cell_of_slot,
                                                          const int* __restrict__ cstart, int* __restrict__ nbr,
                                                          int* __restrict__ cnt, int cap) {
@@ -438,41 +440,65 @@ __global__ __launch_bounds__(kPartThreads) void kb_build(const BlockMap* __restr
   if (t < m.count) {
     const int i = m.s0 + t;
     const double4 pi = pos[i];
+    const uint4 qi = posq[i];
     const int nc = d.nc, s0 = d.slot0;
     const int c = cell_of_slot[i];
     const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
     int* out = nbr + (size_t)(i >> 6) * (size_t)cap * 64 + (i & 63);
     const double L = d.L, cutsq = d.cutsq;
+    // A neighbour cell whose box lies farther than the cutoff from particle i
+    // holds no partner: the binning just folded every position into its cell
+    // (to FP64 rounding, which gmargin covers), so that cell is skipped.
+    const double cs = L / (double)nc, gmargin = 1e-9 * L;
+    auto gap = [&](double xi, int q, double sh) {
+      return fmax(0.0, fmax((double)q * cs + sh - xi, xi - ((double)(q + 1) * cs + sh)) - gmargin);
+    };
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nc; sz = -1; } else if (qz >= nc) { qz -= nc; sz = 1; }
+      const double gz = oz == 0 ? 0.0 : gap(pi.z, qz, (double)sz * L);
       for (int oy = -1; oy <= 1; ++oy) {
         int qy = cy + oy, sy = 0;
         if (qy < 0) { qy += nc; sy = -1; } else if (qy >= nc) { qy -= nc; sy = 1; }
+        const double gy = oy == 0 ? 0.0 : gap(pi.y, qy, (double)sy * L);
+        if (gy * gy + gz * gz > cutsq) continue;
         for (int ox = -1; ox <= 1; ++ox) {
           int qx = cx + ox, sx = 0;
           if (qx < 0) { qx += nc; sx = -1; } else if (qx >= nc) { qx -= nc; sx = 1; }
+          const double gx = ox == 0 ? 0.0 : gap(pi.x, qx, (double)sx * L);
+          if (gx * gx + gy * gy + gz * gz > cutsq) continue;
           const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
           const bool unshifted = (sx | sy | sz) == 0;
           const double shx = (double)sx * L, shy = (double)sy * L, shz = (double)sz * L;
           const int c2 = d.cell0 + qx + nc * (qy + nc * qz);
           const int jb = s0 + cstart[c2], je = s0 + cstart[c2 + 1];
-          // candidates eight at a time: their loads are issued together
+          // Candidates eight at a time, their mirrors loaded together.  The
+          // mirrors' differences are the minimum image (period L, at least three
+          // cells of cutoff + skin), which for a pair within cutoff + skin is this
+          // cell's shift; r^2 in fixed-point units settles all but a thin window
+          // around the cut, where the reference's FP64 test decides.
           for (int j0 = jb; j0 < je; j0 += 8) {
-            double4 pc[8];
+            uint4 qc[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) pc[u] = pos[min(j0 + u, je - 1)];
+            for (int u = 0; u < 8; ++u) qc[u] = posq[min(j0 + u, je - 1)];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
               const int j = j0 + u;
               if (j >= je) break;
               if (j == i && unshifted) continue;
-              const double4 pj = pc[u];
-              double dx, dy, dz;
-              if (fwd) { dx = pi.x - (pj.x + shx); dy = pi.y - (pj.y + shy); dz = pi.z - (pj.z + shz); }
-              else { dx = pj.x - (pi.x - shx); dy = pj.y - (pi.y - shy); dz = pj.z - (pi.z - shz); }
-              const double d2 = dx * dx + dy * dy + dz * dz;
-              if (d2 <= cutsq) {
+              const double ux = (double)(int)(qi.x - qc[u].x);
+              const double uy = (double)(int)(qi.y - qc[u].y);
+              const double uz = (double)(int)(qi.z - qc[u].z);
+              const double u2 = ux * ux + uy * uy + uz * uz;
+              bool in = u2 < d.cqlo;
+              if (!in && u2 <= d.cqhi) {
+                const double4 pj = pos[j];
+                double dx, dy, dz;
+                if (fwd) { dx = pi.x - (pj.x + shx); dy = pi.y - (pj.y + shy); dz = pi.z - (pj.z + shz); }
+                else { dx = pj.x - (pi.x - shx); dy = pj.y - (pi.y - shy); dz = pj.z - (pi.z - shz); }
+                in = dx * dx + dy * dy + dz * dz <= cutsq;
+              }
+              if (in) {
                 if (k < cap) out[(size_t)k * 64] = j;
                 ++k;
               }
@@ -784,7 +810,7 @@ struct espp_batch {
   }
   void launch_build() {
     hipLaunchKernelGGL(kb_build, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos.p,
-                       cell_of_slot.p, cstart.p, nbr.p, cnt.p, cap);
+                       posq.p, cell_of_slot.p, cstart.p, nbr.p, cnt.p, cap);
   }
   void launch_force() {
     hipLaunchKernelGGL(kb_force<false>, dim3(nb_force), dim3(kPartThreads), 0, stream, bm_force.p, drd.p, dst.p,
@@ -1022,6 +1048,11 @@ int espp_batch_create(int32_t nrep, const int32_t* n, const double* box, double 
       const double win = 4.0 * std::sqrt(3.0) * d.qsd * rc + 1e-300;
       d.dlo = d.rc2 - win;
       d.dhi = d.rc2 + win;
+      // the list build's prefilter: r^2 of the mirrors' differences, in fixed-point
+      // units, below cqlo is surely within cutoff + skin, above cqhi surely not
+      const double cwin = 4.0 * std::sqrt(3.0) * d.qsd * cutV + 1e-12 * d.cutsq;
+      d.cqlo = (d.cutsq - cwin) * d.qinv * d.qinv;
+      d.cqhi = (d.cutsq + cwin) * d.qinv * d.qinv;
       d.pref1 = -gamma;
       d.seed = seed[r];
       d.lang = gamma > 0.0 ? 1 : 0;

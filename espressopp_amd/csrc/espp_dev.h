@@ -89,6 +89,19 @@ struct StepArgs {
   double* blockmax;                // per-block max |dx|^2
 };
 
+// The fixed-point mirror has one scale in all three directions.
+inline bool cubic_box(const Geom& g) { return g.qsd[0] == g.qsd[1] && g.qsd[1] == g.qsd[2]; }
+
+// Pruned copy of a tiled list, written by a step kernel while it walks the full
+// list: the entries with r^2 below r2 (the kernel's units) at the step's
+// positions, in their order, in the full list's layout and capacity.
+struct PruneOut {
+  uint4* nbr;
+  int* cnt;                        // lengths, by slot
+  int* lane;                       // lane -> slot of the pruned list (ordered by its lengths)
+  double r2;
+};
+
 // Field groups of Ctl that launch_reset_ctl clears: the binning's (max |dx|^2,
 // nanpos), the list build's (overflow, maxcount) and the tile plan's
 // (plan_fail, max_region).

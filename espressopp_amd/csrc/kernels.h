@@ -20,6 +20,11 @@ void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, 
                     double* fy2, double* fz2, int* id2, int4* img2, int* cell_of_slot,
                     int* slot_of_id, uint4* posq, float4* posf, const Geom& g);
 void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf);
+// x sub-bins of every cell for the tiled list build (ncell * (kXBins + 1) ints),
+// from the positions the binning has just sorted
+constexpr int kXBins = 16;
+void launch_xbins(hipStream_t s, int n, int ncell, const int* cstart, const int* cell_of_slot, const float4* posf,
+                  const Geom& g, int* xbin);
 void launch_fill_int(hipStream_t s, int n, int* a, int v);
 void launch_reset_ctl(hipStream_t s, Ctl* ctl, int fields = kCtlAll);
 void launch_nbr_build(hipStream_t s, int n, const double4* pos, const int* id,
@@ -37,11 +42,12 @@ void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, con
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
                             const int* ex_ids, uint4* nbr, int* cnt, int capg, int* lane_slot,
-                            float clip_margin, Ctl* ctl);
+                            float clip_margin, const int* xbin, Ctl* ctl);
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
-                     int ntypes, double* fx, double* fy, double* fz, double* partial);
+                     int ntypes, double* fx, double* fy, double* fz, double* partial,
+                     const PruneOut* prune = nullptr);
 int lj_tiled_blocks(int n, int T);
 // FENE bonds as the last force of a step, finishing it per particle as
 // launch_lj_tiled_step does (sa.blockmax needs nblocks(n, 256) entries).
@@ -52,12 +58,13 @@ void launch_fene_step(hipStream_t s, bool mixed, bool accum, int n, const double
 // Mixed tiled LJ as the last force of a step, finishing the step per particle
 // (thermostat, integrate2, integrate1 into sa.pos_out / sa.posq_out), then the
 // step's max |dx|^2 (sa.blockmax needs lj_tiled_blocks(n, T) entries) into
-// ctl and host_out.
+// ctl and host_out.  With prune, it also writes the pruned list *prune (same
+// capacity capg as nbr, lane order of n entries).
 void launch_lj_tiled_step(hipStream_t s, bool onetype, bool accum, int n, int T, int lds_bytes, const uint4* posq,
                           const double4* pos, const uint4* nbr, const int* cnt, int capg, const Geom& g,
                           const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
                           int ntypes, double* fx, double* fy, double* fz, const StepArgs& sa, Ctl* ctl,
-                          unsigned long long* host_out);
+                          unsigned long long* host_out, const PruneOut* prune = nullptr);
 void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, const double4* pos,
                  const uint4* posq, const int* id, const int* slot_of_id, int nid,
                  const int* bstart, const int* bpartner, const Geom& g, const FENEParam& fp,

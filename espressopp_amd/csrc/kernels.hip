@@ -380,6 +380,41 @@ __global__ void k_permute(int n, const int* __restrict__ order, const int* __res
 }
 
 // FP32 copy of the positions, the candidates of the tiled list build.
+// x sub-bin of a candidate position for the list build's row clipping: cell
+// units u = x / cs from the FP32 copy, k = floor((u - cx) kXBins) clamped to the
+// cell (the same arithmetic as the build's lookups, so both order alike).
+__device__ __forceinline__ int xsub_bin(float x, float inv_cs, int cx) {
+  const float u = x * inv_cs;
+  return min(max((int)floorf((u - (float)cx) * (float)kXBins), 0), kXBins - 1);
+}
+
+// Per cell c, xbin[c (kXBins+1) + k] is the first slot of the cell whose x
+// sub-bin is at least k (k < kXBins), and the cell's end for k = kXBins.  The
+// binning has just sorted every cell by x, so the sub-bins rise along a cell:
+// slot s writes the entries from its predecessor's sub-bin (exclusive) to its
+// own, the last slot of a cell the entries above its sub-bin, and an empty
+// cell all of its row (threads past n take the cells).
+__global__ void k_xbins(int n, int ncell, const int* __restrict__ cstart, const int* __restrict__ cell_of_slot,
+                        const float4* __restrict__ posf, Geom g, int* __restrict__ xbin) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = g.n[0];
+  const float inv_cs = 1.0f / (float)(g.L[0] / nx);
+  if (t < n) {
+    const int c = cell_of_slot[t];
+    const int cx = c % nx;
+    int* out = xbin + (size_t)c * (kXBins + 1);
+    const int k = xsub_bin(posf[t].x, inv_cs, cx);
+    const int kp = (t > 0 && cell_of_slot[t - 1] == c) ? xsub_bin(posf[t - 1].x, inv_cs, cx) : -1;
+    for (int q = kp + 1; q <= k; ++q) out[q] = t;
+    if (t + 1 == n || cell_of_slot[t + 1] != c)
+      for (int q = k + 1; q <= kXBins; ++q) out[q] = t + 1;
+  }
+  if (t < ncell && cstart[t] == cstart[t + 1]) {
+    int* out = xbin + (size_t)t * (kXBins + 1);
+    for (int q = 0; q <= kXBins; ++q) out[q] = cstart[t];
+  }
+}
+
 __global__ void k_posf(int n, const double4* __restrict__ pos, float4* __restrict__ posf) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1047,7 +1082,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,
     const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint4* __restrict__ nbr,
     int* __restrict__ cnt, int capg, int* __restrict__ lane_slot, float clip_margin,
-    Ctl* ctl) {
+    const int* __restrict__ xbin, Ctl* ctl) {
   // a lost particle: the binning put it in cell 0 and the run ends in an error;
   // do not scan what may be one huge cell.  A failed tile plan: the host plans
   // smaller tiles and builds again.
@@ -1077,9 +1112,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     // reach in x: the prefilter's top with a relative margin for FP32 rounding, plus
     // clip_margin when the x order inside the cells may be stale: a particle moved
     // by at most d since its cell was sorted leaves the order wrong by at most 2d,
-    // and a binary search with that much more reach keeps every candidate within
-    // the prefilter's top
+    // and a stretch with that much more reach keeps every candidate within the
+    // prefilter's top
     const float reach = sqrtf(cut_hi) * 1.0001f + clip_margin;
+    const float cs_x = (float)(g.L[0] / nx), inv_cs = 1.0f / cs_x;
+    const float bmargin = 1e-4f * cs_x;
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nz; sz = -1; } else if (qz >= nz) { qz -= nz; sz = 1; }
@@ -1106,23 +1143,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
           const int jb = cstart[ca], je0 = cstart[row + xb + 1];
           const TileRun R = sr[find_run_by_cell(sr, nr, ca)];
           const int base = R.lds_off + (jb - R.slot_start);
-          // the piece is sorted by x, so x_i - x_j falls along it: keep the stretch
-          // with |x_i - x_j| <= reach
-          // ja: first j with x_i - x_j <= reach; je: first j with x_i - x_j < -reach.
-          // Both searches run in lockstep, so their loads are in flight together
-          // (a finished search reloads its last probe, which the padding of posf
-          // keeps inside the buffer).
+          // the piece is sorted by x, so x_i - x_j falls along it: keep a stretch
+          // holding every j with |x_i - x_j| <= reach.  ja and je come from the
+          // cells' x sub-bins (k_xbins): a bin boundary at or below xs - reach and
+          // one above xs + reach (bmargin covers the FP32 rounding of the bin
+          // arithmetic), so the stretch is that of an exact search widened to the
+          // enclosing sub-bins.
           int ja, je;
           {
-            int lo1 = jb, hi1 = je0, lo2 = jb, hi2 = je0;
-            while (lo1 < hi1 || lo2 < hi2) {
-              const int m1 = (lo1 + hi1) >> 1, m2 = (lo2 + hi2) >> 1;
-              const float x1 = posf[m1].x, x2 = posf[m2].x;
-              if (lo1 < hi1) { if (xs - x1 > reach) lo1 = m1 + 1; else hi1 = m1; }
-              if (lo2 < hi2) { if (xs - x2 >= -reach) lo2 = m2 + 1; else hi2 = m2; }
+            const float ulo = (xs - reach - bmargin) * inv_cs;
+            const float uhi = (xs + reach + bmargin) * inv_cs;
+            if (!(ulo > (float)xa)) {
+              ja = jb;
+            } else if (ulo >= (float)(xb + 1)) {
+              ja = je0;
+            } else {
+              const int cc = min((int)floorf(ulo), xb);
+              const int k = min(max((int)floorf((ulo - (float)cc) * (float)kXBins), 0), kXBins - 1);
+              ja = xbin[(size_t)(row + cc) * (kXBins + 1) + k];
             }
-            ja = lo1;
-            je = lo2;
+            if (!(uhi < (float)(xb + 1))) {
+              je = je0;
+            } else if (uhi < (float)xa) {
+              je = jb;
+            } else {
+              const int cc = max((int)floorf(uhi), xa);
+              const int k = min(max((int)floorf((uhi - (float)cc) * (float)kXBins), 0) + 1, kXBins);
+              je = xbin[(size_t)(row + cc) * (kXBins + 1) + k];
+            }
+            je = max(je, ja);
           }
           for (int j0 = ja; j0 < je; j0 += kCandChunk) {
             const int jn = je - j0;
@@ -1281,14 +1330,20 @@ __device__ __forceinline__ void step_block_max(double sq, const StepArgs& a) {
 // force itself is not stored: the next step recomputes it, and the last step
 // of a run takes the unfused path.  The particle's velocity and position are
 // loaded before the pair loop so that the loads overlap it.
-template <bool ONETYPE, bool ENERGY, bool ACCUM, bool CUBIC, bool STEP>
+// PRUNE (not with ENERGY): the walk also writes the pruned list po, the entries with
+// r^2 < po.r2 at this step's positions in list order (the entries it drops have
+// r^2 above the cutoff window, so they add exactly zero here), each lane
+// appending to a 128-bit shift register as the list build does; then the
+// pruned lengths rank the tile's lanes for the walks of the pruned list.
+template <bool ONETYPE, bool ENERGY, bool ACCUM, bool CUBIC, bool STEP, bool PRUNE = false>
 __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     int n, int ntiles, const uint4* __restrict__ posq, const double4* __restrict__ pos,
     const uint4* __restrict__ nbr, const int* __restrict__ cnt, int capg, Geom g,
     const TileRun* __restrict__ runs, const int* __restrict__ nruns, const int* __restrict__ lane_slot,
     const LJParam* __restrict__ prm, int ntypes, double* __restrict__ fx, double* __restrict__ fy,
-    double* __restrict__ fz, double* __restrict__ partial, StepArgs sa) {
+    double* __restrict__ fz, double* __restrict__ partial, StepArgs sa, PruneOut po) {
   static_assert(!(STEP && ENERGY), "the step variant computes forces only");
+  static_assert(!(PRUNE && ENERGY), "the list is pruned by force walks");
   // one type: LDS holds x, y, z (12 bytes); several types: x, y, z, type
   using Slot = typename std::conditional<ONETYPE, uint3, uint4>::type;
   extern __shared__ uint4 lds_raw[];
@@ -1322,6 +1377,10 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     if (STEP) { step_v = sa.vel[i]; step_p = sa.pos[i]; }
   });
   double stepsq = 0.0;                             // |dx|^2 of the drift (STEP)
+  // PRUNE: entries not yet stored (the newest at the top of pw3) and their count
+  uint32_t pw0 = 0, pw1 = 0, pw2 = 0, pw3 = 0;
+  int kp = 0;
+  uint4* pout = PRUNE ? po.nbr + tiled_group(i, capg, 0) : nullptr;
   double acc[kNPart];
 #pragma unroll
   for (int q = 0; q < kNPart; ++q) acc[q] = 0.0;
@@ -1361,6 +1420,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
         qa[u] = lds[e];
       }
       bool anynear = false;
+      uint32_t keep = 0;                           // PRUNE: entries of the batch that stay
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         const bool valid = !CHECKED || kBatch * it + u < c;
@@ -1378,6 +1438,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
         const bool in = valid & (r2 < lo);
         // in the window: at most hi and not in (in implies at most hi, hence the xor)
         anynear |= (valid & (r2 <= hi)) ^ in;
+        if (PRUNE) keep |= (valid && r2 < po.r2) ? (1u << u) : 0u;
         double inv;
         const double ff = lj_ff_masked(r2, in, ff1, ff2, inv);
         Ax = __builtin_fma(ex, ff, Ax);
@@ -1393,6 +1454,20 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
           acc[1] += r2 * ff;
           acc[2] += ex * ex * ff; acc[3] += ey * ey * ff; acc[4] += ez * ez * ff;
           acc[5] += ex * ey * ff; acc[6] += ex * ez * ff; acc[7] += ey * ez * ff;
+        }
+      }
+      if (PRUNE) {
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+          if (keep & (1u << u)) {
+            const uint32_t e = (u & 1) ? (w[u >> 1] >> 16) : (w[u >> 1] & 0xFFFFu);
+            pw0 = __builtin_amdgcn_alignbit(pw1, pw0, 16);
+            pw1 = __builtin_amdgcn_alignbit(pw2, pw1, 16);
+            pw2 = __builtin_amdgcn_alignbit(pw3, pw2, 16);
+            pw3 = __builtin_amdgcn_alignbit(e, pw3, 16);
+            ++kp;
+            if ((kp & 7) == 0) pout[(size_t)((kp >> 3) - 1) * 64] = make_uint4(pw0, pw1, pw2, pw3);
+          }
         }
       }
       if (anynear) {
@@ -1444,6 +1519,24 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
       if (ACCUM) { fx[i] += Ax; fy[i] += Ay; fz[i] += Az; }
       else { fx[i] = Ax; fy[i] = Ay; fz[i] = Az; }
     }
+    if (PRUNE) {
+      // the last partial group shifted down to its place, zero entries above it
+      if ((kp & 7) != 0) {
+        for (int r = kp & 7; r < 8; ++r) {
+          pw0 = __builtin_amdgcn_alignbit(pw1, pw0, 16);
+          pw1 = __builtin_amdgcn_alignbit(pw2, pw1, 16);
+          pw2 = __builtin_amdgcn_alignbit(pw3, pw2, 16);
+          pw3 = pw3 >> 16;
+        }
+        pout[(size_t)(kp >> 3) * 64] = make_uint4(pw0, pw1, pw2, pw3);
+      }
+      po.cnt[i] = kp;
+    }
+  }
+  if (PRUNE) {
+    // the tile's lanes in order of the pruned lengths, as the list build orders them
+    const int rank = block_class_rank(live ? min((kp + 15) >> 4, 62) : 63);
+    if (live) po.lane[tile * blockDim.x + rank] = i;
   }
   if (STEP) step_block_max(stepsq, sa);
   if (ENERGY) {
@@ -2266,6 +2359,13 @@ void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, 
                        pos2, vel2, fx2, fy2, fz2, id2, img2, cell_of_slot, slot_of_id, posq, posf, g);
 }
 
+void launch_xbins(hipStream_t s, int n, int ncell, const int* cstart, const int* cell_of_slot, const float4* posf,
+                  const Geom& g, int* xbin) {
+  const int m = std::max(n, ncell);
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_xbins, dim3(nblocks(m, 256)), dim3(256), 0, s, n, ncell, cstart, cell_of_slot, posf, g, xbin);
+}
+
 void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf) {
   if (n > 0) hipLaunchKernelGGL(k_posf, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, posf);
 }
@@ -2310,6 +2410,7 @@ void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum,
     else { if (accum) LJD(false, false, true); else LJD(false, false, false); }
   }
 #undef LJM
+#undef LJM1
 #undef LJD
 }
 
@@ -2322,17 +2423,16 @@ void launch_tile_plan(hipStream_t s, int n, int T, int ntiles, const int* cell_o
 }
 
 
-static bool cubic_box(const Geom& g) { return g.qsd[0] == g.qsd[1] && g.qsd[1] == g.qsd[2]; }
 
 void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, const double4* pos,
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
                             const int* ex_ids, uint4* nbr, int* cnt, int capg, int* lane_slot,
-                            float clip_margin, Ctl* ctl) {
+                            float clip_margin, const int* xbin, Ctl* ctl) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
-#define NB(EX) hipLaunchKernelGGL((k_nbr_build_tiled<EX>), grid, block, 0, s, n, ntiles, posf, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, lane_slot, clip_margin, ctl)
+#define NB(EX) hipLaunchKernelGGL((k_nbr_build_tiled<EX>), grid, block, 0, s, n, ntiles, posf, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, lane_slot, clip_margin, xbin, ctl)
   if (ex_start) NB(true);
   else NB(false);
 #undef NB
@@ -2341,14 +2441,18 @@ void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, con
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
-                     int ntypes, double* fx, double* fy, double* fz, double* partial) {
+                     int ntypes, double* fx, double* fy, double* fz, double* partial, const PruneOut* prune) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
   const bool cubic = cubic_box(g);
   const StepArgs sa{};
-#define LJM(OT, EN, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, true, false>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial, sa); \
-                             else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, false, false>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial, sa); } while (0)
+  // only mixed force walks prune
+  if (!mixed || energy) prune = nullptr;
+  const PruneOut po = prune ? *prune : PruneOut{};
+#define LJM1(OT, EN, AC, CU, PR) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, EN, AC, CU, false, PR>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial, sa, po)
+#define LJM(OT, EN, AC) do { if (cubic) { if (!EN && prune) LJM1(OT, false, AC, true, true); else LJM1(OT, EN, AC, true, false); } \
+                             else { if (!EN && prune) LJM1(OT, false, AC, false, true); else LJM1(OT, EN, AC, false, false); } } while (0)
 #define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_tiled_fp64<OT, EN, AC>), grid, block, lds_bytes, s, n, ntiles, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, partial)
   if (mixed) {
     if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }
@@ -2369,6 +2473,7 @@ void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool 
     else { if (accum) LJD(false, false, true); else LJD(false, false, false); }
   }
 #undef LJM
+#undef LJM1
 #undef LJD
 }
 
@@ -2378,16 +2483,19 @@ void launch_lj_tiled_step(hipStream_t s, bool onetype, bool accum, int n, int T,
                           const double4* pos, const uint4* nbr, const int* cnt, int capg, const Geom& g,
                           const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
                           int ntypes, double* fx, double* fy, double* fz, const StepArgs& sa, Ctl* ctl,
-                          unsigned long long* host_out) {
+                          unsigned long long* host_out, const PruneOut* prune) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
   const bool cubic = cubic_box(g);
-#define LJS(OT, AC) do { if (cubic) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, false, AC, true, true>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, nullptr, sa); \
-                         else hipLaunchKernelGGL((k_lj_tiled_mixed<OT, false, AC, false, true>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, nullptr, sa); } while (0)
+  const PruneOut po = prune ? *prune : PruneOut{};
+#define LJS1(OT, AC, CU, PR) hipLaunchKernelGGL((k_lj_tiled_mixed<OT, false, AC, CU, true, PR>), grid, block, lds_bytes, s, n, ntiles, posq, pos, nbr, cnt, capg, g, runs, nruns, lane_slot, prm, ntypes, fx, fy, fz, nullptr, sa, po)
+#define LJS(OT, AC) do { if (cubic) { if (prune) LJS1(OT, AC, true, true); else LJS1(OT, AC, true, false); } \
+                         else { if (prune) LJS1(OT, AC, false, true); else LJS1(OT, AC, false, false); } } while (0)
   if (onetype) { if (accum) LJS(true, true); else LJS(true, false); }
   else { if (accum) LJS(false, true); else LJS(false, false); }
 #undef LJS
+#undef LJS1
   hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, ntiles, sa.blockmax, ctl, host_out);
 }
 

@@ -109,6 +109,13 @@ struct InteractionState {
   DBuf<double> loss, fref;
   bool params_set = false;
   int64_t adam_steps = 0;
+  // tiled mixed LJ: its list pruned to the pairs within the largest cutoff plus
+  // the prune shell at one step's positions, walked in place of the full list
+  // while no pair can have come within the cutoff since (in_build: the list
+  // build it was taken from, in_dist: maxDist at the time)
+  DBuf<int> in_nbr, in_cnt, in_lane;
+  int64_t in_build = -1;
+  double in_dist = 0.0;
 };
 
 // ------------------------------------------------------------------ tables
@@ -283,6 +290,7 @@ struct espp_sim {
   DBuf<uint4> posq2;              // the next step's mirror while a fused force kernel reads posq
   DBuf<float4> posf;              // FP32 positions, candidates of the tiled list build
   DBuf<int> cell_of, rank, order, cell_of_slot, cell_count, cstart;
+  DBuf<int> xbin;                 // per cell, the x sub-bin starts of the tiled list build
   DBuf<int> scan_sum, scan_off;   // per-chunk totals and offsets of the cell scan
   DBuf<double> blockmax;          // per-block max |dx|^2 of the fused step
   DBuf<int> slot_of_id;
@@ -322,6 +330,10 @@ struct espp_sim {
 
   int64_t step = 0;
   double maxDist = 0.0;
+  // width of the shell kept by the pruned LJ lists (0: no pruning, the default;
+  // below 0: skin / 3)
+  double prune_shell = 0.0;
+  int64_t prunes = 0, pruned_walks = 0;   // step kernels that pruned / walked a pruned list
   double disp_since_bin = 0.0;    // bound on any particle's displacement since the last binning
   double coord_bound = 0.0;       // bound on |coordinate| of the current positions
   bool resortFlag = true;
@@ -478,6 +490,7 @@ struct espp_sim {
       pos.swap(pos2); vel.swap(vel2);
       if (carry_forces) { fx.swap(fx2); fy.swap(fy2); fz.swap(fz2); }
       id.swap(id2); img.swap(img2);
+      launch_xbins(stream, ni, (int)ncell(), cstart.p, cell_of_slot.p, posf.p, g, xbin.p);
     }
     prof_end(4, pe);
     disp_since_bin = 0.0;
@@ -635,7 +648,7 @@ struct espp_sim {
         launch_nbr_build_tiled(stream, (int)n, T, posf.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
                                cutsq, cut_lo_q, cut_hi_q, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
                                v.has_excl ? v.ex_ids.p : nullptr, (uint4*)v.nbr.p, v.cnt.p, v.cap / 8,
-                               v.lane_slot.p, clip_margin, ctl.p);
+                               v.lane_slot.p, clip_margin, xbin.p, ctl.p);
       } else {
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
         launch_nbr_build(stream, (int)n, pos.p, id.p, cell_of_slot.p, cstart.p, g, cutsq,
@@ -747,6 +760,57 @@ struct espp_sim {
     upload_bytes(a.dtab.p, t.data(), t.size() * sizeof(LJParam), stream, "lj table upload");
     a.dtab_nt = nt;
     a.dirty = false;
+    a.in_build = -1;     // cutoffs may have changed
+  }
+
+  // ------------------------------------------------------------ pruned LJ lists
+  double prune_width() const { return prune_shell < 0.0 ? skin / 3.0 : prune_shell; }
+
+  // A pair left out of a pruned list was at least rc + w apart at the prune, in
+  // fixed point (rc: the largest cutoff of the interaction, w: the shell).  Every
+  // particle has moved by at most maxDist - in_dist since, and a fixed-point
+  // distance is within sqrt(3) LSB of the true one, so the pair is still beyond
+  // the cutoff window (rc + 2 sqrt(3) LSB) while 2 (maxDist - in_dist) + 8 sqrt(3)
+  // LSB < w.
+  bool pruned_valid(const InteractionState& a, const VerletListState& v) const {
+    const double w = prune_width();
+    const double qmax = std::max(g.qsd[0], std::max(g.qsd[1], g.qsd[2]));
+    return w > 0.0 && w < skin && a.in_build == v.builds &&
+           2.0 * (maxDist - a.in_dist) + 8.0 * std::sqrt(3.0) * qmax < w;
+  }
+
+  // How a mixed force walk of LJ interaction a over tiled list v runs: true to
+  // walk the pruned list; false to walk the full list, having filled po when
+  // that walk is to write a new pruned list.
+  bool prune_plan(InteractionState& a, const VerletListState& v, PruneOut& po) {
+    if (pruned_valid(a, v)) {
+      ++pruned_walks;
+      return true;
+    }
+    const double w = prune_width();
+    if (w > 0.0 && w < skin) {
+      a.in_nbr.resize(v.nbr.n);
+      a.in_cnt.resize((size_t)std::max<int64_t>(n, 1));
+      a.in_lane.resize((size_t)std::max<int64_t>(n, 1));
+      po = PruneOut{(uint4*)a.in_nbr.p, a.in_cnt.p, a.in_lane.p, prune_r2(a)};
+      a.in_build = v.builds;
+      a.in_dist = maxDist;
+      ++prunes;
+    }
+    return false;
+  }
+
+  // The pruned list's threshold in the kernel's units of r^2: (rc + w)^2, rc the
+  // largest cutoff of the set type pairs.
+  double prune_r2(const InteractionState& a) const {
+    double rc = 0.0;
+    for (int i = 0; i <= a.maxset; ++i)
+      for (int j = 0; j <= a.maxset; ++j) {
+        const LJParam& p = a.tab[(size_t)i * kMaxTypes + j];
+        if (p.set && std::isfinite(p.drc2)) rc = std::max(rc, std::sqrt(p.drc2));
+      }
+    const double r = rc + prune_width();
+    return cubic_box(g) ? r * r / g.qs2 : r * r;
   }
 
   // Device tables of a Tabulated interaction: TabParam per type pair (kind 2) or
@@ -841,12 +905,17 @@ struct espp_sim {
       upload_lj(a);
       bool onetype = a.dtab_nt == 1;
       hipEvent_t pe = energy ? nullptr : prof_begin();
+      // mixed force walks take the pruned list while it covers the positions, and
+      // otherwise prune the full list at these positions
+      PruneOut po{};
+      const bool inner = !energy && v.tiled && mixed() && prune_plan(a, v, po);
       if (v.tiled)
         launch_lj_tiled(stream, mixed(), onetype, energy, accum, (int)n, T,
                         max_region * (mixed() ? (onetype ? 12 : 16) : 32), posq.p,
-                        pos.p, (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, v.lane_slot.p,
+                        pos.p, (const uint4*)(inner ? a.in_nbr.p : v.nbr.p), inner ? a.in_cnt.p : v.cnt.p,
+                        v.cap / 8, g, runs.p, nruns.p, inner ? a.in_lane.p : v.lane_slot.p,
                         a.dtab.p, a.dtab_nt,
-                        fx.p, fy.p, fz.p, partial_out);
+                        fx.p, fy.p, fz.p, partial_out, po.nbr ? &po : nullptr);
       else
         launch_lj(stream, mixed(), onetype, energy, accum, (int)n, posq.p, pos.p, v.nbr.p, v.cnt.p, v.cap, g,
                   a.dtab.p, a.dtab_nt, fx.p, fy.p, fz.p, partial_out);
@@ -894,12 +963,17 @@ struct espp_sim {
     sa.pos_out = pos2.p;
     sa.posq_out = posq2.p;
     sa.blockmax = blockmax.p;
+    // walk the pruned list while it covers the positions; otherwise walk the
+    // full list and prune it at these positions
+    PruneOut po{};
+    const bool inner = prune_plan(last, v, po);
     __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
     hipEvent_t pe = prof_begin();
     launch_lj_tiled_step(stream, onetype, active.size() > 1, (int)n, T, max_region * (onetype ? 12 : 16), posq.p,
-                         pos.p, (const uint4*)v.nbr.p, v.cnt.p, v.cap / 8, g, runs.p, nruns.p, v.lane_slot.p,
-                         last.dtab.p, last.dtab_nt, fx.p, fy.p, fz.p, sa, ctl.p, dmax);
-    prof_end(6, pe);
+                         pos.p, (const uint4*)(inner ? last.in_nbr.p : v.nbr.p), inner ? last.in_cnt.p : v.cnt.p,
+                         v.cap / 8, g, runs.p, nruns.p, inner ? last.in_lane.p : v.lane_slot.p,
+                         last.dtab.p, last.dtab_nt, fx.p, fy.p, fz.p, sa, ctl.p, dmax, po.nbr ? &po : nullptr);
+    prof_end(inner ? 7 : 6, pe);
     pos.swap(pos2);
     posq.swap(posq2);
     return true;
@@ -1192,6 +1266,7 @@ int espp_create(const double box[3], const int32_t cell_grid[3], int32_t half_ce
       need(nc < (int64_t)1 << 30, "cell grid too large");
       s->cell_count.resize((size_t)nc);
       s->cstart.resize((size_t)nc + 1);
+      s->xbin.resize((size_t)nc * (kXBins + 1));
       s->scan_sum.resize((size_t)scan_blocks((int)nc) + 1);
       s->scan_off.resize((size_t)scan_blocks((int)nc) + 1);
       s->reserve_particles(1024);
@@ -1730,6 +1805,7 @@ int espp_set_run_state(espp_sim* s, double maxdist, int32_t resort) {
     need(maxdist >= 0.0, "negative displacement");
     s->maxDist = maxdist;
     s->resortFlag = resort != 0;
+    for (auto* a : s->ias) a->in_build = -1;   // the pruned lists' displacement bound is lost
   });
 }
 
@@ -1981,6 +2057,30 @@ int espp_prof_stats(espp_sim* s, int32_t kernel, int64_t* launches, double* tota
     s->prof_collect();
     *launches = s->prof_count[kernel];
     *total_ms = s->prof_ms[kernel];
+  });
+}
+
+int espp_set_prune_shell(espp_sim* s, double width) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    need(std::isfinite(width), "prune shell must be finite");
+    s->prune_shell = width;
+    for (auto* a : s->ias) a->in_build = -1;
+  });
+}
+
+int espp_prune_stats(espp_sim* s, int32_t ia, int64_t* prunes, int64_t* pruned_walks, int64_t* entries) {
+  return guarded([&] {
+    need(s && prunes && pruned_walks && entries, "NULL argument");
+    need(ia >= 0 && ia < (int)s->ias.size(), "bad interaction handle");
+    *prunes = s->prunes;
+    *pruned_walks = s->pruned_walks;
+    *entries = 0;
+    InteractionState& a = *s->ias[(size_t)ia];
+    if (a.kind == 0 && a.in_build >= 0 && a.in_cnt.n >= (size_t)s->n && s->n > 0) {
+      std::vector<int> c = download(a.in_cnt, (size_t)s->n, s->stream);
+      for (int x : c) *entries += x;
+    }
   });
 }
 

@@ -432,9 +432,16 @@ def main(argv=None):
     nch, nb, m = args.chains, args.beads, args.per_blob
     total = args.frames + args.test_frames
 
+    mapped = []
+
+    def on_frame(p, f):
+        mapped.append(None)
+        if len(mapped) % 100 == 0:
+            log("melt: frame %d of %d (%.1fs)" % (len(mapped), total, time.perf_counter() - t0))
+        return blob_map(p, f, nch, nb, m, _box_of(nch * nb))
+
     box, bonds, raw = melt_frames(espressopp, nch, nb, total, args.stride, seed=args.seed, tmpdir=tmp,
-                                  precision="mixed",
-                                  on_frame=lambda p, f: blob_map(p, f, nch, nb, m, _box_of(nch * nb)))
+                                  precision="mixed", on_frame=on_frame)
     pos = [x[0] for x in raw]
     F = [x[1] for x in raw]
     del raw

@@ -27,7 +27,9 @@ extern "C" {
 #define ESPP_K_BOND 5        /* FENE bond-force kernel */
 #define ESPP_K_PAIR_STEP 6   /* Lennard-Jones pair forces that also finish the step: thermostat,
                                 half kick, and the next step's half kick, drift and max
-                                displacement (every step of a run but the last) */
+                                displacement (every step of a run but the last), walking
+                                the full list and writing its pruned copy */
+#define ESPP_K_PAIR_STEP_PRUNED 7 /* the same walking the pruned list */
 
 /* hipDeviceSynchronize on the current device. */
 int espp_device_synchronize(void);
@@ -51,6 +53,20 @@ int espp_vl_stats(espp_sim *sim, int32_t vl, int64_t *entries, int32_t *cap, int
  * wavefronts of 64 consecutive lanes with each wavefront counted at its longest
  * lane, which is what a wavefront runs. */
 int espp_vl_batch_stats(espp_sim *sim, int32_t vl, int32_t batch, int64_t *lane_batches, int64_t *wave_batches);
+
+/* Width w of the shell that the pruned Lennard-Jones lists keep beyond the
+ * largest cutoff (0 or at least the skin: no pruning, the default; below 0:
+ * skin / 3).  A force walk of the mixed path over the full list also writes
+ * the pairs within rc + w at its positions as the pruned list; the following
+ * walks take the pruned list until twice the displacement bound accumulated
+ * since (the skin check's maxDist) reaches w.  It pays only where lists live
+ * many steps: the pruning walk runs at lower occupancy than the plain one. */
+int espp_set_prune_shell(espp_sim *sim, double width);
+
+/* Step kernels that pruned and that walked a pruned list (all interactions,
+ * since the handle was made), and the entries of interaction ia's current
+ * pruned list (0 when it has none). */
+int espp_prune_stats(espp_sim *sim, int32_t ia, int64_t *prunes, int64_t *pruned_walks, int64_t *entries);
 
 /* Particles per tile of the current tile plan (0: untiled kernels) and the
  * largest staged region in particles. */
