@@ -1088,6 +1088,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   // smaller tiles and builds again.
   if (ctl->nanpos || ctl->plan_fail) return;
   __shared__ TileRun sr[kMaxRuns];
+  __shared__ __align__(16) unsigned short build_ring[256 * 16];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = nruns[tile];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)tile * kMaxRuns + threadIdx.x];
@@ -1105,10 +1106,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int myid = HAS_EXCL ? id[i] : 0;
     uint4* out = nbr + tiled_group(i, capg, 0);
     const int cap = 8 * capg;
-    // entries not yet stored: (w3:w2:w1:w0) is a 128-bit shift register of 16-bit
-    // entries, each new entry entering at the top; after 8 entries it holds the
-    // group in the list layout (first entry in the low half of w0)
-    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+    // entries not yet stored: a ring of two 8-entry groups in LDS, entry q at
+    // ring[q & 15]; a completed group is stored from it as one 16-byte write
+    unsigned short* ring = build_ring + (size_t)threadIdx.x * 16;
+    const uint4* ring4 = reinterpret_cast<const uint4*>(ring);
     // reach in x: the prefilter's top with a relative margin for FP32 rounding, plus
     // clip_margin when the x order inside the cells may be stale: a particle moved
     // by at most d since its cell was sorted leaves the order wrong by at most 2d,
@@ -1187,8 +1188,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
               for (int v = 3; v >= 0; --v) {
                 const float dx = xs - q[v].x, dy = ys - q[v].y, dz = zs - q[v].z;
                 const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                in_bits = (in_bits << 1) + (r2 < cut_lo ? 1u : 0u);
-                maybe_bits = (maybe_bits << 1) + (r2 <= cut_hi ? 1u : 0u);
+                // the sign bit of r2 - cut shifted in at the bottom: r2 < cut_lo, and
+                // r2 < cut_hi (the window's top lies above every r2 that may be in)
+                in_bits = __builtin_amdgcn_alignbit(in_bits, __float_as_uint(r2 - cut_lo), 31);
+                maybe_bits = __builtin_amdgcn_alignbit(maybe_bits, __float_as_uint(r2 - cut_hi), 31);
               }
             }
             uint32_t valid = jn >= kCandChunk ? kCandAll : ((1u << jn) - 1u);
@@ -1214,31 +1217,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
               if (exact_sqr_le(ex, ey, ez, cutsq)) in_bits |= 1u << t;
             }
             const uint32_t ebase = (uint32_t)(base + (j0 - jb));   // entry of candidate 0
+            // up to four entries per pass, so the wavefront's passes follow the
+            // largest count of its lanes divided by four
             while (in_bits) {
-              const int t = __ffs(in_bits) - 1;
-              in_bits &= in_bits - 1u;
-              if (HAS_EXCL && is_excluded(myid, id[j0 + t], ex_start, ex_ids)) continue;
-              w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
-              w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
-              w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-              w3 = __builtin_amdgcn_alignbit(ebase + (uint32_t)t, w3, 16);
-              ++k;
-              if ((k & 7) == 0 && k <= cap) out[(size_t)((k >> 3) - 1) * 64] = make_uint4(w0, w1, w2, w3);
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                if (in_bits) {
+                  const int t = __ffs(in_bits) - 1;
+                  in_bits &= in_bits - 1u;
+                  if (!(HAS_EXCL && is_excluded(myid, id[j0 + t], ex_start, ex_ids))) {
+                    ring[k & 15] = (unsigned short)(ebase + (uint32_t)t);
+                    ++k;
+                    if ((k & 7) == 0 && k <= cap)
+                      out[(size_t)((k >> 3) - 1) * 64] = ring4[((k >> 3) - 1) & 1];
+                  }
+                }
+              }
             }
           }
         }
       }
     }
-    // last partial group: shift its entries down to their places (entries past the
-    // count are never read)
+    // last partial group: the entries past the count are zero (offset 0 stages a
+    // valid slot; the pair kernels read but never use them)
     if ((k & 7) != 0 && k < cap) {
-      for (int r = k & 7; r < 8; ++r) {
-        w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
-        w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
-        w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
-        w3 = w3 >> 16;
-      }
-      out[(size_t)(k >> 3) * 64] = make_uint4(w0, w1, w2, w3);
+      for (int r = k & 7; r < 8; ++r) ring[(k & 8) + r] = 0;
+      out[(size_t)(k >> 3) * 64] = ring4[(k >> 3) & 1];
     }
     cnt[i] = k;
     if (k > cap) ctl->overflow = 1;
