@@ -60,6 +60,7 @@ struct RepDesc {
   double skin_half, cutsq;              // skin / 2 and (rc + skin)^2
   double ff1, ff2, rc2, shift, sig2, eps4, dlo, dhi;
   double cqlo, cqhi;                    // (rc + skin)^2 window in fixed-point units^2 (list build)
+  double ulo, uhi, uff1, uff2, usig2;   // dlo, dhi, ff1, ff2, sig2 in fixed-point units (force launch)
   double pref1, pref2;                  // -gamma, sqrt(24 T gamma / dt)
   unsigned long long seed;
   int slot0, n, cell0, nc;
@@ -567,29 +568,54 @@ __global__ __launch_bounds__(kPartThreads) void kb_force(const BlockMap* __restr
     const uint4 qi = posq[i];
     const int* np = nbr + (size_t)(i >> 6) * (size_t)cap * 64 + (i & 63);
     const int c = min(cnt[i], cap);
-    const double qs = d.qsd, ff1 = d.ff1, ff2 = d.ff2, rc2 = d.rc2, lo = d.dlo, hi = d.dhi;
-    auto pair = [&](int j, const uint4& qj) {
-      const double ex = (double)(int)(qi.x - qj.x) * qs;
-      const double ey = (double)(int)(qi.y - qj.y) * qs;
-      const double ez = (double)(int)(qi.z - qj.z) * qs;
-      const double r2 = ex * ex + ey * ey + ez * ez;
-      bool in = r2 <= rc2;
-      if (r2 >= lo && r2 <= hi) in = ref_r2(pos[i], pos[j], cell_of_slot[i], cell_of_slot[j], d) <= rc2;
-      double inv;
-      double ff = lj_ff_seeded(r2, ff1, ff2, inv);
-      ff = in ? ff : 0.0;
-      Ax = __builtin_fma(ex, ff, Ax);
-      Ay = __builtin_fma(ey, ff, Ay);
-      Az = __builtin_fma(ez, ff, Az);
-      if (ENERGY && in) {
-        const double f2 = d.sig2 * inv;
+    // In fixed-point units: u = mirror difference (exact), u2 = |u|^2, F = u q ff_u.
+    // u2 below ulo is inside the cutoff and u2 above uhi outside; a pair in
+    // between is left out of the pass and decided by the reference's FP64 test
+    // after it (rare).
+    const double uff1 = d.uff1, uff2 = d.uff2, ulo = d.ulo, uhi = d.uhi, rc2 = d.rc2;
+    auto contrib = [&](double ux, double uy, double uz, double u2, double ff, double inv) {
+      Ax = __builtin_fma(ux, ff, Ax);
+      Ay = __builtin_fma(uy, ff, Ay);
+      Az = __builtin_fma(uz, ff, Az);
+      if (ENERGY) {
+        const double f2 = d.usig2 * inv;
         const double f6 = f2 * f2 * f2;
         E += d.eps4 * (f6 * f6 - f6) - d.shift;
-        W += r2 * ff;
+        W += u2 * ff;
       }
     };
-    // entries four at a time (indices, then positions, issued together), in the
-    // same order as one at a time
+    auto pair = [&](const uint4& qj, bool& near) {
+      const double ux = (double)(int)(qi.x - qj.x);
+      const double uy = (double)(int)(qi.y - qj.y);
+      const double uz = (double)(int)(qi.z - qj.z);
+      const double u2 = ux * ux + uy * uy + uz * uz;
+      const bool in = u2 < ulo;
+      near |= (u2 <= uhi) & !in;
+      double inv;
+      const double ff = lj_ff_masked(u2, in, uff1, uff2, inv);
+      Ax = __builtin_fma(ux, ff, Ax);
+      Ay = __builtin_fma(uy, ff, Ay);
+      Az = __builtin_fma(uz, ff, Az);
+      if (ENERGY && in) {
+        const double f2 = d.usig2 * inv;
+        const double f6 = f2 * f2 * f2;
+        E += d.eps4 * (f6 * f6 - f6) - d.shift;
+        W += u2 * ff;
+      }
+    };
+    auto settle = [&](int j, const uint4& qj) {
+      const double ux = (double)(int)(qi.x - qj.x);
+      const double uy = (double)(int)(qi.y - qj.y);
+      const double uz = (double)(int)(qi.z - qj.z);
+      const double u2 = ux * ux + uy * uy + uz * uz;
+      if (u2 < ulo || !(u2 <= uhi)) return;
+      if (!(ref_r2(pos[i], pos[j], cell_of_slot[i], cell_of_slot[j], d) <= rc2)) return;
+      double inv;
+      const double ff = lj_ff_seeded(u2, uff1, uff2, inv);
+      contrib(ux, uy, uz, u2, ff, inv);
+    };
+    // entries four at a time (indices, then positions, issued together); pairs at
+    // the cutoff are added after their group of four
     int k = q;
     for (; k + 3 * lpp < c; k += 4 * lpp) {
       int jj[4];
@@ -598,13 +624,23 @@ __global__ __launch_bounds__(kPartThreads) void kb_force(const BlockMap* __restr
       for (int u = 0; u < 4; ++u) jj[u] = np[(size_t)(k + u * lpp) * 64];
 #pragma unroll
       for (int u = 0; u < 4; ++u) qq[u] = posq[jj[u]];
+      bool near = false;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) pair(jj[u], qq[u]);
+      for (int u = 0; u < 4; ++u) pair(qq[u], near);
+      if (near) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) settle(jj[u], qq[u]);
+      }
     }
     for (; k < c; k += lpp) {
       const int j = np[(size_t)k * 64];
-      pair(j, posq[j]);
+      const uint4 qj = posq[j];
+      bool near = false;
+      pair(qj, near);
+      if (near) settle(j, qj);
     }
+    Ax *= d.qsd; Ay *= d.qsd; Az *= d.qsd;
+    if (ENERGY) W *= d.qsd * d.qsd;
   }
   // combine the lpp lanes of a particle (consecutive lanes, power of two)
   for (int o = 1; o < lpp; o <<= 1) {
@@ -1053,6 +1089,14 @@ int espp_batch_create(int32_t nrep, const int32_t* n, const double* box, double 
       const double cwin = 4.0 * std::sqrt(3.0) * d.qsd * cutV + 1e-12 * d.cutsq;
       d.cqlo = (d.cutsq - cwin) * d.qinv * d.qinv;
       d.cqhi = (d.cutsq + cwin) * d.qinv * d.qinv;
+      // the force launch works in fixed-point units: r^2 = u^2 q^2 (q = L / 2^32),
+      // so ff(r^2) = u^-8 (ff1 q^-14 u^-6 - ff2 q^-8) and F = u q ff
+      const double q2 = d.qsd * d.qsd;
+      d.ulo = d.dlo / q2;
+      d.uhi = d.dhi / q2;
+      d.uff1 = d.ff1 / std::pow(q2, 7);
+      d.uff2 = d.ff2 / std::pow(q2, 4);
+      d.usig2 = d.sig2 / q2;
       d.pref1 = -gamma;
       d.seed = seed[r];
       d.lang = gamma > 0.0 ? 1 : 0;

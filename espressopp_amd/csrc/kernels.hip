@@ -36,17 +36,6 @@ __device__ __forceinline__ double pair_r2(const uint4& qi, int ix, int iy, int i
   return ex * ex + ey * ey + ez * ez;
 }
 
-// lj_ff_seeded for a pair that may be out of range: such a pair seeds the
-// reciprocal with rcp(+inf) = 0, after which u, r6 and the factor are exactly 0
-// (r2 is finite), so no select on the FP64 result is needed.
-__device__ __forceinline__ double lj_ff_masked(double r2, bool in, double ff1, double ff2, double& inv) {
-  const float rf = in ? (float)r2 : __builtin_inff();
-  double u = (double)__builtin_amdgcn_rcpf(rf);
-  u = u * __builtin_fma(-r2, u, 2.0);
-  inv = u;
-  const double r6 = u * u * u;
-  return r6 * __builtin_fma(ff1, r6, -ff2) * u;
-}
 
 // ------------------------------------------------------------------ binning
 
