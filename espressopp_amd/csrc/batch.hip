@@ -616,14 +616,21 @@ __global__ __launch_bounds__(kPartThreads) void kb_force(const BlockMap* __restr
     };
     // entries four at a time (indices, then positions, issued together); pairs at
     // the cutoff are added after their group of four
+    // (the next group's indices are loaded while this group's positions arrive)
     int k = q;
+    int jn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jn[u] = k + 3 * lpp < c ? np[(size_t)(k + u * lpp) * 64] : 0;
     for (; k + 3 * lpp < c; k += 4 * lpp) {
       int jj[4];
       uint4 qq[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) jj[u] = np[(size_t)(k + u * lpp) * 64];
+      for (int u = 0; u < 4; ++u) jj[u] = jn[u];
 #pragma unroll
       for (int u = 0; u < 4; ++u) qq[u] = posq[jj[u]];
+      const int kn = k + 4 * lpp;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) jn[u] = kn + 3 * lpp < c ? np[(size_t)(kn + u * lpp) * 64] : 0;
       bool near = false;
 #pragma unroll
       for (int u = 0; u < 4; ++u) pair(qq[u], near);
