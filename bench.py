@@ -238,6 +238,32 @@ def c1_single_replica(steps=2000, seed=2024):
             "unit": "particle-steps/s", "lpp": int(b.lpp[0]), "resorts": b.stats(0)["resorts"]}
 
 
+def c1_per_system(steps=2000, seed=2024):
+    """The 32,000-particle LJ fluid (jittered lattice, rho 0.8442, Langevin T = 1,
+    gamma = 1, dt 0.005) on the per-system path, mixed precision: steps launched in
+    windows with the skin check on the device (the default), and for comparison
+    with a host wait after every step."""
+    import espressopp_amd as espressopp
+    n = 32000
+    pos, box = lattice(n, 0.8442, seed=seed)
+    vel = velocities(n, 1.0, seed + 1)
+    cells = [int(b / 2.8) for b in box]
+    out = {"workload": "LJ fluid N=32000 rho 0.8442 Langevin T=1 gamma=1 dt 0.005, mixed path, per-system handle"}
+    for window, key in ((32, "windowed"), (0, "host_wait_per_step")):
+        system, vl, lj, integ = make_system(espressopp, box, cells, 2.5, 0.3, "mixed", pos, vel)
+        system.storage._h.call("espp_set_step_window", window)
+        lang = espressopp.integrator.LangevinThermostat(system)
+        lang.gamma = 1.0
+        lang.temperature = 1.0
+        integ.addExtension(lang)
+        integ.run(500)
+        dt = timed_run(espressopp, system, integ, steps)
+        out[key] = {"us_per_step": dt / steps * 1e6, "value": n * steps / dt, "resorts": integ.getNumResorts()}
+    out["value"] = out["windowed"]["value"]
+    out["unit"] = "particle-steps/s"
+    return out
+
+
 def c5_learned_cg(n_sites=250000, reps=5, seed=7):
     """Throughput of the learned pair force's BF16 MFMA kernels on a coarse-grained
     system of n_sites sites at the 4:1-mapped melt density 0.85 / 4 (random positions,
@@ -316,7 +342,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched parallel-tempering measurement")
     ap.add_argument("--no-c5", action="store_true", help="skip the learned CG force measurement")
-    ap.add_argument("--only", choices=["c1", "c4", "c5"], default=None,
+    ap.add_argument("--only", choices=["c1", "c1s", "c4", "c5"], default=None,
                     help="run only one companion workload and print its JSON (profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
@@ -337,7 +363,7 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     if args.only:
-        fn = {"c1": c1_single_replica, "c4": c4_batched_pt, "c5": c5_learned_cg}[args.only]
+        fn = {"c1": c1_single_replica, "c1s": c1_per_system, "c4": c4_batched_pt, "c5": c5_learned_cg}[args.only]
         print(json.dumps(fn()), flush=True)
         return
 
@@ -478,10 +504,14 @@ def main():
         c4 = c4_batched_pt()
         log("c4 batched PT (%.1fs): %s" % (time.perf_counter() - t, json.dumps(c4)))
     c1 = None
+    c1s = None
     if not args.no_c4:
         t = time.perf_counter()
         c1 = c1_single_replica()
         log("c1 via the batched engine (%.1fs): %s" % (time.perf_counter() - t, json.dumps(c1)))
+        t = time.perf_counter()
+        c1s = c1_per_system()
+        log("c1 per system (%.1fs): %s" % (time.perf_counter() - t, json.dumps(c1s)))
     c5 = None
     if not args.no_c5:
         t = time.perf_counter()
@@ -524,6 +554,7 @@ def main():
         "cpu_baseline": cpu,
         "c4_batched_parallel_tempering": c4,
         "c1_single_replica_batched": c1,
+        "c1_per_system": c1s,
         "c5_learned_cg_force": c5,
     }
     traffic_file = os.path.join(ROOT, "profiles", "lj_mixed_4M_traffic.json")

@@ -105,6 +105,8 @@ PROF_SIGNATURES = {
     "espp_prof_reset": [_vp],
     "espp_prof_stats": [_vp, _c_int, _P(_c_i64), _P(_c_dbl)],
     "espp_set_prune_shell": [_vp, _c_dbl],
+    "espp_set_step_window": [_vp, _c_int],
+    "espp_step_window_stats": [_vp, _P(_c_i64), _P(_c_i64), _P(_c_i64)],
     "espp_prune_stats": [_vp, _c_int, _P(_c_i64), _P(_c_i64), _P(_c_i64)],
     "espp_vl_stats": [_vp, _c_int, _P(_c_i64), _P(_c_int), _P(_c_int)],
     "espp_vl_batch_stats": [_vp, _c_int, _c_int, _P(_c_i64), _P(_c_i64)],

@@ -87,6 +87,16 @@ struct StepArgs {
   unsigned long long seed, call;
   int lang, nid, do_int2, pad;
   double* blockmax;                // per-block max |dx|^2
+  const int* stop;                 // when set and nonzero at launch, the kernel does nothing (null: none)
+};
+
+// Device-side skin check of a window of steps launched without waiting on the
+// host: each step's max |dx| is added to dmax (VelocityVerlet's maxDist), done
+// counts the steps that ran, and stop is raised once dmax exceeds skin/2, after
+// which the window's remaining kernels return at once.
+struct SpecState {
+  double dmax, skin_half;
+  int stop, done;
 };
 
 // The fixed-point mirror has one scale in all three directions.

@@ -35,7 +35,7 @@ int lj_blocks(int n);
 void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n,
                const uint4* posq, const double4* pos, const int* nbr, const int* cnt, int cap,
                const Geom& g, const LJParam* prm, int ntypes, double* fx, double* fy, double* fz,
-               double* partial);
+               double* partial, const int* stop = nullptr);
 void launch_tile_plan(hipStream_t s, int n, int T, int ntiles, const int* cell_of_slot, const int* cstart,
                       const Geom& g, int budget, TileRun* runs, int* nruns, Ctl* ctl);
 void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, const double4* pos,
@@ -47,14 +47,14 @@ void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool 
                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
                      int ntypes, double* fx, double* fy, double* fz, double* partial,
-                     const PruneOut* prune = nullptr);
+                     const PruneOut* prune = nullptr, const int* stop = nullptr);
 int lj_tiled_blocks(int n, int T);
 // FENE bonds as the last force of a step, finishing it per particle as
 // launch_lj_tiled_step does (sa.blockmax needs nblocks(n, 256) entries).
 void launch_fene_step(hipStream_t s, bool mixed, bool accum, int n, const double4* pos, const uint4* posq,
                       const int* id, const int* slot_of_id, int nid, const int* bstart, const int* bpartner,
                       const Geom& g, const FENEParam& fp, double* fx, double* fy, double* fz, const StepArgs& sa,
-                      Ctl* ctl, unsigned long long* host_out);
+                      Ctl* ctl, unsigned long long* host_out, SpecState* spec = nullptr);
 // Mixed tiled LJ as the last force of a step, finishing the step per particle
 // (thermostat, integrate2, integrate1 into sa.pos_out / sa.posq_out), then the
 // step's max |dx|^2 (sa.blockmax needs lj_tiled_blocks(n, T) entries) into
@@ -64,11 +64,14 @@ void launch_lj_tiled_step(hipStream_t s, bool onetype, bool accum, int n, int T,
                           const double4* pos, const uint4* nbr, const int* cnt, int capg, const Geom& g,
                           const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
                           int ntypes, double* fx, double* fy, double* fz, const StepArgs& sa, Ctl* ctl,
-                          unsigned long long* host_out, const PruneOut* prune = nullptr);
+                          unsigned long long* host_out, const PruneOut* prune = nullptr,
+                          SpecState* spec = nullptr);
+// Start of a window of steps with the device-side skin check (SpecState).
+void launch_spec_begin(hipStream_t s, SpecState* sp, double dmax, double skin_half);
 void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, const double4* pos,
                  const uint4* posq, const int* id, const int* slot_of_id, int nid,
                  const int* bstart, const int* bpartner, const Geom& g, const FENEParam& fp,
-                 double* fx, double* fy, double* fz, double* partial);
+                 double* fx, double* fy, double* fz, double* partial, const int* stop = nullptr);
 void launch_tab_tiled(hipStream_t s, bool mixed, bool energy, bool accum, int n, int T, int lds_bytes,
                       const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                       const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const TabParam* prm,

@@ -574,7 +574,8 @@ __global__ __launch_bounds__(256) void k_lj_mixed(
     int n, const uint4* __restrict__ posq, const double4* __restrict__ pos,
     const int* __restrict__ nbr, const int* __restrict__ cnt, int cap, Geom g,
     const LJParam* __restrict__ prm, int ntypes, double* __restrict__ fx,
-    double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ partial) {
+    double* __restrict__ fy, double* __restrict__ fz, double* __restrict__ partial, const int* stop) {
+  if (stop && *reinterpret_cast<const volatile int*>(stop)) return;   // a window's step after the stop
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[kNPart];
 #pragma unroll
@@ -1337,6 +1338,7 @@ __global__ __launch_bounds__(256) void k_lj_tiled_mixed(
     double* __restrict__ fz, double* __restrict__ partial, StepArgs sa, PruneOut po) {
   static_assert(!(STEP && ENERGY), "the step variant computes forces only");
   static_assert(!(PRUNE && ENERGY), "the list is pruned by force walks");
+  if (sa.stop && *reinterpret_cast<const volatile int*>(sa.stop)) return;   // a window's step after the stop
   // one type: LDS holds x, y, z (12 bytes); several types: x, y, z, type
   using Slot = typename std::conditional<ONETYPE, uint3, uint4>::type;
   extern __shared__ uint4 lds_raw[];
@@ -2023,6 +2025,7 @@ __global__ __launch_bounds__(256) void k_fene(
     double* __restrict__ partial, StepArgs sa) {
 #pragma clang fp contract(off)
   static_assert(!(STEP && ENERGY), "the step variant computes forces only");
+  if (sa.stop && *reinterpret_cast<const volatile int*>(sa.stop)) return;   // a window's step after the stop
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double stepsq = 0.0;
   double acc[kNPart];
@@ -2236,8 +2239,10 @@ __global__ __launch_bounds__(256) void k_step_fused(int n, double* __restrict__ 
 // host_out (pinned host memory, may be null) receives the same bits, so the host
 // reads the step's displacement without a copy or a stream synchronisation.
 __global__ __launch_bounds__(1024) void k_finish_max(int nblk, const double* __restrict__ blockmax, Ctl* ctl,
-                                                     unsigned long long* host_out) {
+                                                     unsigned long long* host_out, SpecState* sp) {
   __shared__ double red[16];
+  // a window's step that did not run (stop raised before it) has no displacement
+  if (sp && *reinterpret_cast<volatile int*>(&sp->stop)) return;
   double m = 0.0;
   // sixteen independent loads in flight per thread, not one round trip per load
   for (int b0 = 0; b0 < nblk; b0 += 16 * (int)blockDim.x) {
@@ -2259,7 +2264,20 @@ __global__ __launch_bounds__(1024) void k_finish_max(int nblk, const double* __r
     const unsigned long long bits = (unsigned long long)__double_as_longlong(t);
     ctl->maxsq_bits = bits;
     if (host_out) *reinterpret_cast<volatile unsigned long long*>(host_out) = bits;
+    if (sp) {
+      // the skin check of VelocityVerlet::run (src/integrator/VelocityVerlet.cpp:121-136)
+      sp->dmax += sqrt(t);
+      sp->done += 1;
+      if (!(sp->dmax <= sp->skin_half)) sp->stop = 1;
+    }
   }
+}
+
+__global__ void k_spec_begin(SpecState* sp, double dmax, double skin_half) {
+  sp->dmax = dmax;
+  sp->skin_half = skin_half;
+  sp->stop = 0;
+  sp->done = 0;
 }
 
 // Kinetic sums per block: [sum m v^2, count, m vx vx, m vy vy, m vz vz, m vx vy,
@@ -2328,7 +2346,7 @@ void launch_step_fused(hipStream_t s, int n, double* fx, double* fy, double* fz,
   if (n <= 0) return;
   const int nb = nblocks(n, 256);
   hipLaunchKernelGGL(k_step_fused, dim3(nb), dim3(256), 0, s, n, fx, fy, fz, a, g);
-  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, a.blockmax, ctl, host_out);
+  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, a.blockmax, ctl, host_out, nullptr);
 }
 
 
@@ -2350,6 +2368,10 @@ void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, 
   if (n > 0)
     hipLaunchKernelGGL(k_permute, dim3(nblocks(n, 256)), dim3(256), 0, s, n, order, cell_of, pos, vel, fx, fy, fz, id, img,
                        pos2, vel2, fx2, fy2, fz2, id2, img2, cell_of_slot, slot_of_id, posq, posf, g);
+}
+
+void launch_spec_begin(hipStream_t s, SpecState* sp, double dmax, double skin_half) {
+  hipLaunchKernelGGL(k_spec_begin, dim3(1), dim3(1), 0, s, sp, dmax, skin_half);
 }
 
 void launch_xbins(hipStream_t s, int n, int ncell, const int* cstart, const int* cell_of_slot, const float4* posf,
@@ -2388,10 +2410,10 @@ int lj_blocks(int n) { return nblocks(n, 256); }
 void launch_lj(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n,
                const uint4* posq, const double4* pos, const int* nbr, const int* cnt, int cap,
                const Geom& g, const LJParam* prm, int ntypes, double* fx, double* fy, double* fz,
-               double* partial) {
+               double* partial, const int* stop) {
   if (n <= 0) return;
   dim3 grid(nblocks(n, 256)), block(256);
-#define LJM(OT, EN, AC) hipLaunchKernelGGL((k_lj_mixed<OT, EN, AC>), grid, block, 0, s, n, posq, pos, nbr, cnt, cap, g, prm, ntypes, fx, fy, fz, partial)
+#define LJM(OT, EN, AC) hipLaunchKernelGGL((k_lj_mixed<OT, EN, AC>), grid, block, 0, s, n, posq, pos, nbr, cnt, cap, g, prm, ntypes, fx, fy, fz, partial, stop)
 #define LJD(OT, EN, AC) hipLaunchKernelGGL((k_lj_fp64<OT, EN, AC>), grid, block, 0, s, n, pos, nbr, cnt, cap, g, prm, ntypes, fx, fy, fz, partial)
   if (mixed) {
     if (energy) { if (onetype) LJM(true, true, false); else LJM(false, true, false); }
@@ -2434,12 +2456,14 @@ void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, con
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
-                     int ntypes, double* fx, double* fy, double* fz, double* partial, const PruneOut* prune) {
+                     int ntypes, double* fx, double* fy, double* fz, double* partial, const PruneOut* prune,
+                     const int* stop) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
   const bool cubic = cubic_box(g);
-  const StepArgs sa{};
+  StepArgs sa{};
+  sa.stop = stop;
   // only mixed force walks prune
   if (!mixed || energy) prune = nullptr;
   const PruneOut po = prune ? *prune : PruneOut{};
@@ -2476,7 +2500,7 @@ void launch_lj_tiled_step(hipStream_t s, bool onetype, bool accum, int n, int T,
                           const double4* pos, const uint4* nbr, const int* cnt, int capg, const Geom& g,
                           const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,
                           int ntypes, double* fx, double* fy, double* fz, const StepArgs& sa, Ctl* ctl,
-                          unsigned long long* host_out, const PruneOut* prune) {
+                          unsigned long long* host_out, const PruneOut* prune, SpecState* spec) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
@@ -2489,16 +2513,17 @@ void launch_lj_tiled_step(hipStream_t s, bool onetype, bool accum, int n, int T,
   else { if (accum) LJS(false, true); else LJS(false, false); }
 #undef LJS
 #undef LJS1
-  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, ntiles, sa.blockmax, ctl, host_out);
+  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, ntiles, sa.blockmax, ctl, host_out, spec);
 }
 
 void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, const double4* pos,
                  const uint4* posq, const int* id, const int* slot_of_id, int nid, const int* bstart,
                  const int* bpartner, const Geom& g, const FENEParam& fp, double* fx, double* fy,
-                 double* fz, double* partial) {
+                 double* fz, double* partial, const int* stop) {
   if (n <= 0) return;
   dim3 grid(nblocks(n, 256)), block(256);
-  const StepArgs sa{};
+  StepArgs sa{};
+  sa.stop = stop;
 #define FN(MX, EN, AC) hipLaunchKernelGGL((k_fene<MX, EN, AC, false>), grid, block, 0, s, n, pos, posq, id, slot_of_id, nid, bstart, bpartner, g, fp, fx, fy, fz, partial, sa)
   if (mixed) {
     if (energy) FN(true, true, false); else if (accum) FN(true, false, true); else FN(true, false, false);
@@ -2511,14 +2536,14 @@ void launch_fene(hipStream_t s, bool mixed, bool energy, bool accum, int n, cons
 void launch_fene_step(hipStream_t s, bool mixed, bool accum, int n, const double4* pos, const uint4* posq,
                       const int* id, const int* slot_of_id, int nid, const int* bstart, const int* bpartner,
                       const Geom& g, const FENEParam& fp, double* fx, double* fy, double* fz, const StepArgs& sa,
-                      Ctl* ctl, unsigned long long* host_out) {
+                      Ctl* ctl, unsigned long long* host_out, SpecState* spec) {
   if (n <= 0) return;
   const int nb = nblocks(n, 256);
 #define FN(MX, AC) hipLaunchKernelGGL((k_fene<MX, false, AC, true>), dim3(nb), dim3(256), 0, s, n, pos, posq, id, slot_of_id, nid, bstart, bpartner, g, fp, fx, fy, fz, nullptr, sa)
   if (mixed) { if (accum) FN(true, true); else FN(true, false); }
   else { if (accum) FN(false, true); else FN(false, false); }
 #undef FN
-  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, sa.blockmax, ctl, host_out);
+  hipLaunchKernelGGL(k_finish_max, dim3(1), dim3(1024), 0, s, nb, sa.blockmax, ctl, host_out, spec);
 }
 
 void launch_tab_tiled(hipStream_t s, bool mixed, bool energy, bool accum, int n, int T, int lds_bytes,

@@ -321,6 +321,12 @@ struct espp_sim {
   unsigned long long* dmax = nullptr;     // device view of hmax
   DBuf<double> partial, red;
   double* hred = nullptr;
+  // windows of steps launched without a host wait (device-side skin check)
+  DBuf<SpecState> spec;
+  SpecState* hspec = nullptr;
+  int spec_max = 32;               // longest window (0: wait on the host after every step)
+  double spec_rate = 0.0;          // recent max displacement per step
+  int64_t spec_windows = 0, spec_launched = 0, spec_ran = 0;
   double* scratch_dev = nullptr;   // this device's scratch (espp_ia_energy_noalloc)
 
   std::vector<VerletListState*> vls;
@@ -401,6 +407,7 @@ struct espp_sim {
     if (hctl) (void)hipHostFree(hctl);
     if (hmax) (void)hipHostFree(hmax);
     if (hred) (void)hipHostFree(hred);
+    if (hspec) (void)hipHostFree(hspec);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -763,6 +770,24 @@ struct espp_sim {
     a.in_build = -1;     // cutoffs may have changed
   }
 
+  // Whether the steps of a run can go in windows without a host wait: every
+  // force kernel of a step must honour the window's stop flag (mixed LJ, FENE),
+  // the last must finish the step (tiled mixed LJ, FENE), the pruned lists
+  // (judged on the host) must be off, and profiling (which times each launch)
+  // must be off.
+  bool window_ok() const {
+    if (spec_max < 2 || prof || !mixed() || active.empty() || n == 0) return false;
+    const double w = prune_width();
+    if (w > 0.0 && w < skin) return false;
+    for (int k : active) {
+      const int kind = ias[(size_t)k]->kind;
+      if (kind != 0 && kind != 1) return false;
+    }
+    // the last interaction must be one whose kernel finishes the step
+    const InteractionState& last = *ias[(size_t)active.back()];
+    return last.kind == 1 || vls[(size_t)last.list]->tiled;
+  }
+
   // ------------------------------------------------------------ pruned LJ lists
   double prune_width() const { return prune_shell < 0.0 ? skin / 3.0 : prune_shell; }
 
@@ -862,7 +887,8 @@ struct espp_sim {
     a.dirty = false;
   }
 
-  void run_interaction(InteractionState& a, bool energy, bool accum, double* partial_out) {
+  void run_interaction(InteractionState& a, bool energy, bool accum, double* partial_out,
+                       const int* stop = nullptr) {
     if (a.kind == 4) {
       if (energy) fail(ESPP_ERR_STATE, "the learned pair force is not conservative: it has no energy or virial");
       if (!a.params_set) fail(ESPP_ERR_STATE, "the learned pair force has no parameters (setParameters)");
@@ -915,17 +941,17 @@ struct espp_sim {
                         pos.p, (const uint4*)(inner ? a.in_nbr.p : v.nbr.p), inner ? a.in_cnt.p : v.cnt.p,
                         v.cap / 8, g, runs.p, nruns.p, inner ? a.in_lane.p : v.lane_slot.p,
                         a.dtab.p, a.dtab_nt,
-                        fx.p, fy.p, fz.p, partial_out, po.nbr ? &po : nullptr);
+                        fx.p, fy.p, fz.p, partial_out, po.nbr ? &po : nullptr, stop);
       else
         launch_lj(stream, mixed(), onetype, energy, accum, (int)n, posq.p, pos.p, v.nbr.p, v.cnt.p, v.cap, g,
-                  a.dtab.p, a.dtab_nt, fx.p, fy.p, fz.p, partial_out);
+                  a.dtab.p, a.dtab_nt, fx.p, fy.p, fz.p, partial_out, stop);
       prof_end(0, pe);
     } else {
       FixedPairListState& f = *fpls[(size_t)a.list];
       if (f.dirty || f.built_nid != nid) upload_bonds(f);
       hipEvent_t pe = energy ? nullptr : prof_begin();
       launch_fene(stream, mixed(), energy, accum, (int)n, pos.p, posq.p, id.p, slot_of_id.p, nid, f.bstart.p,
-                  f.bpartner.p, g, a.fp, fx.p, fy.p, fz.p, partial_out);
+                  f.bpartner.p, g, a.fp, fx.p, fy.p, fz.p, partial_out, stop);
       prof_end(5, pe);
     }
   }
@@ -945,15 +971,20 @@ struct espp_sim {
   // posq2, which then become pos / posq; the step's max |dx|^2 arrives in hmax.
   // Returns false, having launched nothing, when the last interaction is not
   // such a kernel.
-  bool calc_forces_step(StepArgs sa) {
+  // With in_window, the step is one of a window launched without a host wait:
+  // every kernel of it returns at once once the device-side skin check has
+  // raised spec->stop, and the displacement goes to spec instead of hmax.
+  bool calc_forces_step(StepArgs sa, bool in_window = false) {
     if (active.empty() || n == 0) return false;
     InteractionState& last = *ias[(size_t)active.back()];
-    if (last.kind == 1) return fene_step(last, sa);
+    if (in_window) sa.stop = &spec.p->stop;
+    if (last.kind == 1) return fene_step(last, sa, in_window);
     if (!mixed() || last.kind != 0) return false;
     ensure_lists();
     VerletListState& v = *vls[(size_t)last.list];
     if (!v.tiled) return false;
-    for (size_t q = 0; q + 1 < active.size(); ++q) run_interaction(*ias[(size_t)active[q]], false, q > 0, nullptr);
+    for (size_t q = 0; q + 1 < active.size(); ++q)
+      run_interaction(*ias[(size_t)active[q]], false, q > 0, nullptr, sa.stop);
     upload_lj(last);
     const bool onetype = last.dtab_nt == 1;
     blockmax.resize((size_t)std::max(lj_tiled_blocks((int)n, T), 1));
@@ -967,12 +998,13 @@ struct espp_sim {
     // full list and prune it at these positions
     PruneOut po{};
     const bool inner = prune_plan(last, v, po);
-    __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
+    if (!in_window) __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
     hipEvent_t pe = prof_begin();
     launch_lj_tiled_step(stream, onetype, active.size() > 1, (int)n, T, max_region * (onetype ? 12 : 16), posq.p,
                          pos.p, (const uint4*)(inner ? last.in_nbr.p : v.nbr.p), inner ? last.in_cnt.p : v.cnt.p,
                          v.cap / 8, g, runs.p, nruns.p, inner ? last.in_lane.p : v.lane_slot.p,
-                         last.dtab.p, last.dtab_nt, fx.p, fy.p, fz.p, sa, ctl.p, dmax, po.nbr ? &po : nullptr);
+                         last.dtab.p, last.dtab_nt, fx.p, fy.p, fz.p, sa, ctl.p, in_window ? nullptr : dmax,
+                         po.nbr ? &po : nullptr, in_window ? spec.p : nullptr);
     prof_end(inner ? 7 : 6, pe);
     pos.swap(pos2);
     posq.swap(posq2);
@@ -980,9 +1012,10 @@ struct espp_sim {
   }
 
   // The same with FENE bonds as the last interaction (either precision).
-  bool fene_step(InteractionState& last, StepArgs sa) {
+  bool fene_step(InteractionState& last, StepArgs sa, bool in_window = false) {
     ensure_lists();
-    for (size_t q = 0; q + 1 < active.size(); ++q) run_interaction(*ias[(size_t)active[q]], false, q > 0, nullptr);
+    for (size_t q = 0; q + 1 < active.size(); ++q)
+      run_interaction(*ias[(size_t)active[q]], false, q > 0, nullptr, sa.stop);
     FixedPairListState& f = *fpls[(size_t)last.list];
     if (f.dirty || f.built_nid != nid) upload_bonds(f);
     blockmax.resize((size_t)std::max(kinetic_blocks((int)n), 1));
@@ -992,10 +1025,11 @@ struct espp_sim {
     sa.pos_out = pos2.p;
     sa.posq_out = mixed() ? posq2.p : nullptr;
     sa.blockmax = blockmax.p;
-    __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
+    if (!in_window) __atomic_store_n(hmax, kNoValue, __ATOMIC_RELEASE);
     hipEvent_t pe = prof_begin();
     launch_fene_step(stream, mixed(), active.size() > 1, (int)n, pos.p, posq.p, id.p, slot_of_id.p, nid, f.bstart.p,
-                     f.bpartner.p, g, last.fp, fx.p, fy.p, fz.p, sa, ctl.p, dmax);
+                     f.bpartner.p, g, last.fp, fx.p, fy.p, fz.p, sa, ctl.p, in_window ? nullptr : dmax,
+                     in_window ? spec.p : nullptr);
     prof_end(5, pe);
     pos.swap(pos2);
     if (mixed()) posq.swap(posq2);
@@ -1113,6 +1147,44 @@ struct espp_sim {
       }
       auto tc = clk::now();
       t_resort += std::chrono::duration<double>(tc - tb).count();
+      // Steps i, i+1, ... whose force kernel finishes them are launched as one
+      // window without waiting on the host: the device adds each step's max
+      // displacement to maxDist and, once it passes skin/2, stops the window's
+      // remaining kernels; the host then reads how many steps ran and goes on
+      // with the resort check as after any step.  The window is sized to end
+      // about when the list is due, from the displacement per step so far.
+      const int avail = nsteps - 1 - i;
+      if (avail >= 2 && window_ok()) {
+        int K = spec_max;
+        if (spec_rate > 0.0) K = (int)std::min<double>(spec_max, std::floor((skinHalf - maxDist) / spec_rate) + 1.0);
+        K = std::max(2, std::min(K, avail));
+        launch_spec_begin(stream, spec.p, maxDist, skinHalf);
+        for (int k = 0; k < K; ++k) {
+          StepArgs sa = step_args(lang, 1);
+          sa.call = lang ? therm_calls + (uint64_t)k : 0ull;
+          calc_forces_step(sa, true);
+        }
+        hip_check(hipMemcpyAsync(hspec, spec.p, sizeof(SpecState), hipMemcpyDeviceToHost, stream), "window readback");
+        sync();
+        const int d = hspec->done;
+        if (d < 1 || d > K) fail(ESPP_ERR_DEVICE, "a window of steps reported " + std::to_string(d) + " steps");
+        // each launch swapped the position buffers; only d steps ran
+        if ((K - d) & 1) { pos.swap(pos2); posq.swap(posq2); }
+        const double grow = hspec->dmax - maxDist;
+        maxDist = hspec->dmax;
+        disp_since_bin += grow;
+        coord_bound += grow;
+        if (std::isfinite(grow) && grow > 0.0) spec_rate = grow / d;
+        if (lang) therm_calls += (uint64_t)d;
+        ++spec_windows;
+        spec_launched += K;
+        spec_ran += d;
+        fused_prev = true;
+        t_force += std::chrono::duration<double>(clk::now() - tc).count();
+        step += d;
+        i += d - 1;
+        continue;
+      }
       fused_prev = i + 1 < nsteps && calc_forces_step(step_args(lang, 1));
       if (fused_prev) {
         if (lang) therm_calls++;
@@ -1260,6 +1332,8 @@ int espp_create(const double box[3], const int32_t cell_grid[3], int32_t half_ce
       hip_check(hipHostMalloc((void**)&s->hmax, 64, hipHostMallocCoherent | hipHostMallocMapped), "hipHostMalloc");
       hip_check(hipHostGetDevicePointer((void**)&s->dmax, s->hmax, 0), "hipHostGetDevicePointer");
       hip_check(hipHostMalloc((void**)&s->hred, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+      hip_check(hipHostMalloc((void**)&s->hspec, sizeof(SpecState), hipHostMallocDefault), "hipHostMalloc");
+      s->spec.resize(1);
       s->red.resize(16);
       s->scratch_dev = scratch_for_device(s->device);
       int64_t nc = s->ncell();
@@ -2057,6 +2131,23 @@ int espp_prof_stats(espp_sim* s, int32_t kernel, int64_t* launches, double* tota
     s->prof_collect();
     *launches = s->prof_count[kernel];
     *total_ms = s->prof_ms[kernel];
+  });
+}
+
+int espp_set_step_window(espp_sim* s, int32_t max_steps) {
+  return guarded([&] {
+    need(s != nullptr, "NULL handle");
+    need(max_steps >= 0 && max_steps <= 4096, "window length must be between 0 and 4096");
+    s->spec_max = max_steps;
+  });
+}
+
+int espp_step_window_stats(espp_sim* s, int64_t* windows, int64_t* launched, int64_t* ran) {
+  return guarded([&] {
+    need(s && windows && launched && ran, "NULL argument");
+    *windows = s->spec_windows;
+    *launched = s->spec_launched;
+    *ran = s->spec_ran;
   });
 }
 

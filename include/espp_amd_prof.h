@@ -54,6 +54,19 @@ int espp_vl_stats(espp_sim *sim, int32_t vl, int64_t *entries, int32_t *cap, int
  * lane, which is what a wavefront runs. */
 int espp_vl_batch_stats(espp_sim *sim, int32_t vl, int32_t batch, int64_t *lane_batches, int64_t *wave_batches);
 
+/* Longest window of steps that espp_run launches without waiting on the host
+ * (default 32; 0 or 1: a host wait after every step).  Within a window each
+ * step's kernels first read a device flag, raised by the previous step's
+ * displacement reduction once the summed displacement bound (the skin check's
+ * maxDist) passes skin/2, and return at once when it is set; the host reads how
+ * many steps ran, rebuilds and goes on.  Results are those of the step-by-step
+ * loop.  Used when every force of a step is a tiled mixed-precision Lennard-Jones
+ * or a FENE interaction, pruned lists are off and launch timing is off. */
+int espp_set_step_window(espp_sim *sim, int32_t max_steps);
+
+/* Windows run, steps launched in them and steps that ran, since the handle was made. */
+int espp_step_window_stats(espp_sim *sim, int64_t *windows, int64_t *launched, int64_t *ran);
+
 /* Width w of the shell that the pruned Lennard-Jones lists keep beyond the
  * largest cutoff (0 or at least the skin: no pruning, the default; below 0:
  * skin / 3).  A force walk of the mixed path over the full list also writes

@@ -104,6 +104,7 @@ void lj_fluid(int precision) {
   std::vector<double> out((size_t)3 * n), rdf(64);
   for (int f : {ESPP_FIELD_POS, ESPP_FIELD_VEL, ESPP_FIELD_FORCE, ESPP_FIELD_IMAGE}) CHECK(espp_get_particles(s, f, out.data()));
   CHECK(espp_get_particles_by_id(s, ESPP_FIELD_POS, 3, &id[10], out.data()));
+  CHECK(espp_get_particles_by_id(s, ESPP_FIELD_VEL, 2, &id[20], out.data()));
   CHECK(espp_set_particles(s, ESPP_FIELD_VEL, 2, &id[20], out.data()));
   CHECK(espp_rdf(s, 64, rdf.data()));
   double mv2 = 0;
@@ -125,11 +126,18 @@ void lj_fluid(int precision) {
   CHECK(espp_remove_interaction(s, 3));
   CHECK(espp_remove_interaction(s, 2));
   CHECK(espp_fene_set(s, fene, 25.0, 0.0, 1.6, INFINITY, 0.0));
-  // pruned lists on the mixed path, checkpoint state
+  // pruned lists, then windows of steps, on the mixed path (the hot start from the
+  // doubled LJ of the tables needs the smaller time step); checkpoint state
   CHECK(espp_set_prune_shell(s, -1.0));
-  CHECK(espp_run(s, 30, 0.005));
+  CHECK(espp_run(s, 30, 0.002));
   int64_t prunes = 0, walks = 0, entries = 0;
   CHECK(espp_prune_stats(s, lj, &prunes, &walks, &entries));
+  CHECK(espp_set_prune_shell(s, 0.0));
+  CHECK(espp_set_step_window(s, 8));
+  CHECK(espp_run(s, 30, 0.002));
+  int64_t windows = 0, launched = 0, ran = 0;
+  CHECK(espp_step_window_stats(s, &windows, &launched, &ran));
+  EXPECT_ERROR(espp_set_step_window(s, -1));
   uint64_t seed = 0, calls = 0;
   double maxdist = 0;
   int32_t resort = 0;
