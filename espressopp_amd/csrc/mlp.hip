@@ -42,10 +42,10 @@ constexpr int kS = 72;                 // LDS row stride (bf16): 64 + 8, rows st
 constexpr int kMat = 64 * kS;          // one 64 x 64 operand in LDS
 constexpr int kThreads = 256;
 
+// FP32 -> bfloat16 bits, rounded to nearest even (one v_cvt_pk_bf16_f32 on gfx950;
+// FP32 denormals are kept, so it rounds every finite value as the oracle does)
 __device__ __forceinline__ unsigned short bf16_bits(float f) {
-  unsigned int u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
+  return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 __device__ __forceinline__ float bf16_val(unsigned short b) { return __uint_as_float((unsigned int)b << 16); }
 __device__ __forceinline__ float bf16_round(float f) { return bf16_val(bf16_bits(f)); }
