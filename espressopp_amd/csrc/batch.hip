@@ -59,7 +59,7 @@ struct RepDesc {
   double L, invL, ics, qinv, qsd;
   double skin_half, cutsq;              // skin / 2 and (rc + skin)^2
   double ff1, ff2, rc2, shift, sig2, eps4, dlo, dhi;
-  double cqlo, cqhi;                    // (rc + skin)^2 window in fixed-point units^2 (list build)
+  double cflo, cfhi;                    // (rc + skin)^2 window of the list build's FP32 prefilter
   double ulo, uhi, uff1, uff2, usig2;   // dlo, dhi, ff1, ff2, sig2 in fixed-point units (force launch)
   double pref1, pref2;                  // -gamma, sqrt(24 T gamma / dt)
   unsigned long long seed;
@@ -405,7 +405,8 @@ __global__ __launch_bounds__(kPartThreads) void kb_copy_back(const BlockMap* __r
                                                              const int* __restrict__ id2,
                                                              const int4* __restrict__ img2, double4* __restrict__ pos,
                                                              double4* __restrict__ vel, int* __restrict__ id,
-                                                             int4* __restrict__ img, uint4* __restrict__ posq) {
+                                                             int4* __restrict__ img, uint4* __restrict__ posq,
+                                                             float4* __restrict__ posf) {
   const BlockMap m = bm[blockIdx.x];
   const int r = m.rep;
   if (!st[r].rebuild || threadIdx.x >= (unsigned)m.count) return;
@@ -413,6 +414,7 @@ __global__ __launch_bounds__(kPartThreads) void kb_copy_back(const BlockMap* __r
   const double4 p = pos2[s];
   const double qinv = rd[r].qinv;
   pos[s] = p;
+  posf[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
   vel[s] = vel2[s];
   id[s] = id2[s];
   img[s] = img2[s];
@@ -424,10 +426,10 @@ __global__ __launch_bounds__(kPartThreads) void kb_copy_back(const BlockMap* __r
 // FP64 without contraction, d in the orientation of the half-shell visit (cell
 // offset of j lexicographically positive, z most significant: d = xi - (xj + sL),
 // else d = xj - (xi - sL)).  Entry k of slot s: nbr[((s/64)*cap + k)*64 + s%64].
-__global__ __launch_bounds__(kPartThreads) void kb_build(const BlockMap* __restrict__ bm,
+__global__ __launch_bounds__(kPartThreads) __attribute__((amdgpu_waves_per_eu(5))) void kb_build(const BlockMap* __restrict__ bm,
                                                          const RepDesc* __restrict__ rd,
                                                          RepState* __restrict__ st, const double4* __restrict__ pos,
-                                                         const uint4* __restrict__ posq,
+                                                         const float4* __restrict__ posf,
                                                          const int* __restrict__ cell_of_slot,
                                                          const int* __restrict__ cstart, int* __restrict__ nbr,
                                                          int* __restrict__ cnt, int cap) {
@@ -441,7 +443,7 @@ __global__ __launch_bounds__(kPartThreads) void kb_build(const BlockMap* __restr
   if (t < m.count) {
     const int i = m.s0 + t;
     const double4 pi = pos[i];
-    const uint4 qi = posq[i];
+    const float cflo = (float)d.cflo, cfhi = (float)d.cfhi;
     const int nc = d.nc, s0 = d.slot0;
     const int c = cell_of_slot[i];
     const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
@@ -458,41 +460,41 @@ __global__ __launch_bounds__(kPartThreads) void kb_build(const BlockMap* __restr
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nc; sz = -1; } else if (qz >= nc) { qz -= nc; sz = 1; }
       const double gz = oz == 0 ? 0.0 : gap(pi.z, qz, (double)sz * L);
+      // particle i moved by minus the candidate cells' periodic shift, in FP32
+      const float zs = (float)(pi.z - (double)sz * L);
       for (int oy = -1; oy <= 1; ++oy) {
         int qy = cy + oy, sy = 0;
         if (qy < 0) { qy += nc; sy = -1; } else if (qy >= nc) { qy -= nc; sy = 1; }
         const double gy = oy == 0 ? 0.0 : gap(pi.y, qy, (double)sy * L);
         if (gy * gy + gz * gz > cutsq) continue;
+        const float ys = (float)(pi.y - (double)sy * L);
         for (int ox = -1; ox <= 1; ++ox) {
           int qx = cx + ox, sx = 0;
           if (qx < 0) { qx += nc; sx = -1; } else if (qx >= nc) { qx -= nc; sx = 1; }
           const double gx = ox == 0 ? 0.0 : gap(pi.x, qx, (double)sx * L);
           if (gx * gx + gy * gy + gz * gz > cutsq) continue;
+          const float xs = (float)(pi.x - (double)sx * L);
           const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
           const bool unshifted = (sx | sy | sz) == 0;
           const double shx = (double)sx * L, shy = (double)sy * L, shz = (double)sz * L;
           const int c2 = d.cell0 + qx + nc * (qy + nc * qz);
           const int jb = s0 + cstart[c2], je = s0 + cstart[c2 + 1];
-          // Candidates eight at a time, their mirrors loaded together.  The
-          // mirrors' differences are the minimum image (period L, at least three
-          // cells of cutoff + skin), which for a pair within cutoff + skin is this
-          // cell's shift; r^2 in fixed-point units settles all but a thin window
-          // around the cut, where the reference's FP64 test decides.
+          // Candidates eight at a time, their FP32 copies loaded together; FP32
+          // r^2 settles all but a thin window around the cut, where the
+          // reference's FP64 test decides.
           for (int j0 = jb; j0 < je; j0 += 8) {
-            uint4 qc[8];
+            float4 qc[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) qc[u] = posq[min(j0 + u, je - 1)];
+            for (int u = 0; u < 8; ++u) qc[u] = posf[min(j0 + u, je - 1)];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
               const int j = j0 + u;
               if (j >= je) break;
               if (j == i && unshifted) continue;
-              const double ux = (double)(int)(qi.x - qc[u].x);
-              const double uy = (double)(int)(qi.y - qc[u].y);
-              const double uz = (double)(int)(qi.z - qc[u].z);
-              const double u2 = ux * ux + uy * uy + uz * uz;
-              bool in = u2 < d.cqlo;
-              if (!in && u2 <= d.cqhi) {
+              const float fx = xs - qc[u].x, fy = ys - qc[u].y, fz = zs - qc[u].z;
+              const float f2 = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
+              bool in = f2 < cflo;
+              if (!in && f2 <= cfhi) {
                 const double4 pj = pos[j];
                 double dx, dy, dz;
                 if (fwd) { dx = pi.x - (pj.x + shx); dy = pi.y - (pj.y + shy); dz = pi.z - (pj.z + shz); }
@@ -760,6 +762,7 @@ struct espp_batch {
   DBuf<int> id, id2, id_snap;
   DBuf<int4> img, img2, img_snap;
   DBuf<uint4> posq;
+  DBuf<float4> posf;                      // FP32 copy of the positions at the last binning (list build)
   DBuf<double> fx, fy, fz;
   DBuf<int> cell_of_slot, cstart, ccount, tcell, trank, order;
   DBuf<BlockMap> bm_cell;                 // cell blocks of kb_sort_cells
@@ -849,11 +852,11 @@ struct espp_batch {
     hipLaunchKernelGGL(kb_permute, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, order.p,
                        pos.p, vel.p, id.p, img.p, tcell.p, pos2.p, vel2.p, id2.p, img2.p, cell_of_slot.p);
     hipLaunchKernelGGL(kb_copy_back, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos2.p,
-                       vel2.p, id2.p, img2.p, pos.p, vel.p, id.p, img.p, posq.p);
+                       vel2.p, id2.p, img2.p, pos.p, vel.p, id.p, img.p, posq.p, posf.p);
   }
   void launch_build() {
     hipLaunchKernelGGL(kb_build, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos.p,
-                       posq.p, cell_of_slot.p, cstart.p, nbr.p, cnt.p, cap);
+                       posf.p, cell_of_slot.p, cstart.p, nbr.p, cnt.p, cap);
   }
   void launch_force() {
     hipLaunchKernelGGL(kb_force<false>, dim3(nb_force), dim3(kPartThreads), 0, stream, bm_force.p, drd.p, dst.p,
@@ -1091,11 +1094,14 @@ int espp_batch_create(int32_t nrep, const int32_t* n, const double* box, double 
       const double win = 4.0 * std::sqrt(3.0) * d.qsd * rc + 1e-300;
       d.dlo = d.rc2 - win;
       d.dhi = d.rc2 + win;
-      // the list build's prefilter: r^2 of the mirrors' differences, in fixed-point
-      // units, below cqlo is surely within cutoff + skin, above cqhi surely not
-      const double cwin = 4.0 * std::sqrt(3.0) * d.qsd * cutV + 1e-12 * d.cutsq;
-      d.cqlo = (d.cutsq - cwin) * d.qinv * d.qinv;
-      d.cqhi = (d.cutsq + cwin) * d.qinv * d.qinv;
+      // the list build's FP32 prefilter: its coordinates (a folded position, or one
+      // shifted by a box length) are below M = 2L in magnitude, each rounded once,
+      // so a difference is off by at most one FP32 ulp of M and r^2 by at most
+      // 2 sqrt(3) (rc+skin) ulp(M); the window takes twice that plus 1e-6 relative
+      const double ulpM = std::ldexp(1.0, std::ilogb(2.0 * box[r]) - 23);
+      const double cwin = 1e-6 + 4.0 * std::sqrt(3.0) * ulpM / cutV;
+      d.cflo = d.cutsq * (1.0 - cwin);
+      d.cfhi = d.cutsq * (1.0 + cwin);
       // the force launch works in fixed-point units: r^2 = u^2 q^2 (q = L / 2^32),
       // so ff(r^2) = u^-8 (ff1 q^-14 u^-6 - ff2 q^-8) and F = u q ff
       const double q2 = d.qsd * d.qsd;
@@ -1129,6 +1135,7 @@ int espp_batch_create(int32_t nrep, const int32_t* n, const double* box, double 
     b->id.resize(S + 4); b->id2.resize(S); b->id_snap.resize(S + 4);
     b->img.resize(S); b->img2.resize(S); b->img_snap.resize(S);
     b->posq.resize(S);
+    b->posf.resize(S);
     b->fx.resize(S); b->fy.resize(S); b->fz.resize(S);
     b->cell_of_slot.resize(S); b->tcell.resize(S); b->trank.resize(S); b->order.resize(S);
     b->cstart.resize((size_t)cell);
