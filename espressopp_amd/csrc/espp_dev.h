@@ -24,6 +24,7 @@ struct Geom {
   double qsd[3];                   // L / 2^32 in FP64
   double qs2;                      // (L / 2^32)^2 for cubic boxes (tiled mixed kernel)
   float qsf;                       // L / 2^32 as FP32 (cubic boxes)
+  double q16inv;                   // 1 / delta of the list build's 16-bit copy at binning (0: none)
   // The caller's cell grid with halfCellInt h (src/storage/DomainDecomposition.cpp:325-374):
   // nh = h * n cells of size >= (rc+skin)/h, stencil +-h.  The binning above uses the
   // grid n = nh / h; the caller's grid only fixes which of a pair's two particles the

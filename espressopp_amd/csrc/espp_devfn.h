@@ -19,6 +19,18 @@ __device__ __forceinline__ uint32_t fixq(double x, double qinv) {
   return (uint32_t)(unsigned long long)(long long)__builtin_rint(x * qinv);
 }
 
+// The list build's 16-bit copy of a position: each coordinate to round(x / delta)
+// modulo 2^16 (q16inv = 1 / delta, delta = L / (2^16 m) for a whole m), x and y
+// in the low and high half of the first word, z in the low half of the second
+// (its high half zero).  16-bit differences of two copies are the coordinate
+// differences of the periodic image that lies within 2^15 delta.
+__device__ __forceinline__ uint2 q16_pack(const double4& p, double q16inv) {
+  const uint32_t ux = (uint32_t)(unsigned long long)(long long)__builtin_rint(p.x * q16inv) & 0xFFFFu;
+  const uint32_t uy = (uint32_t)(unsigned long long)(long long)__builtin_rint(p.y * q16inv) & 0xFFFFu;
+  const uint32_t uz = (uint32_t)(unsigned long long)(long long)__builtin_rint(p.z * q16inv) & 0xFFFFu;
+  return make_uint2(ux | (uy << 16), uz);
+}
+
 __device__ __forceinline__ float rcp_nr(float x) {
   // v_rcp_f32 followed by one Newton step: ~0.5 ulp, three VALU ops
   float r = __builtin_amdgcn_rcpf(x);

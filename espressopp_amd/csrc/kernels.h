@@ -18,8 +18,8 @@ void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, 
                     const double4* vel, const double* fx, const double* fy, const double* fz,
                     const int* id, const int4* img, double4* pos2, double4* vel2, double* fx2,
                     double* fy2, double* fz2, int* id2, int4* img2, int* cell_of_slot,
-                    int* slot_of_id, uint4* posq, float4* posf, const Geom& g);
-void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf);
+                    int* slot_of_id, uint4* posq, float4* posf, uint2* posh, const Geom& g);
+void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf, uint2* posh, double q16inv);
 // x sub-bins of every cell for the tiled list build (ncell * (kXBins + 1) ints),
 // from the positions the binning has just sorted
 constexpr int kXBins = 16;
@@ -42,7 +42,8 @@ void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, con
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
                             const int* ex_ids, uint4* nbr, int* cnt, int capg, int* lane_slot,
-                            float clip_margin, const int* xbin, Ctl* ctl);
+                            float clip_margin, float row_margin, const int* xbin, const uint2* posh,
+                            double q16inv, int q16_tlo, int q16_w, Ctl* ctl);
 void launch_lj_tiled(hipStream_t s, bool mixed, bool onetype, bool energy, bool accum, int n, int T, int lds_bytes,
                      const uint4* posq, const double4* pos, const uint4* nbr, const int* cnt, int capg,
                      const Geom& g, const TileRun* runs, const int* nruns, const int* lane_slot, const LJParam* prm,

@@ -344,7 +344,8 @@ __global__ void k_permute(int n, const int* __restrict__ order, const int* __res
                           double* __restrict__ fy2, double* __restrict__ fz2,
                           int* __restrict__ id2, int4* __restrict__ img2,
                           int* __restrict__ cell_of_slot, int* __restrict__ slot_of_id,
-                          uint4* __restrict__ posq, float4* __restrict__ posf, Geom g) {
+                          uint4* __restrict__ posq, float4* __restrict__ posf, uint2* __restrict__ posh,
+                          Geom g) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int o = order[s];
@@ -365,6 +366,7 @@ __global__ void k_permute(int n, const int* __restrict__ order, const int* __res
     posq[s] = make_uint4(fixq(p.x, g.qinv[0]), fixq(p.y, g.qinv[1]), fixq(p.z, g.qinv[2]),
                          (uint32_t)(int)p.w);
   posf[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+  if (posh) posh[s] = q16_pack(p, g.q16inv);
   if (g.hci > 1) g.hcell[s] = caller_cell(p, g);
 }
 
@@ -404,11 +406,13 @@ __global__ void k_xbins(int n, int ncell, const int* __restrict__ cstart, const 
   }
 }
 
-__global__ void k_posf(int n, const double4* __restrict__ pos, float4* __restrict__ posf) {
+__global__ void k_posf(int n, const double4* __restrict__ pos, float4* __restrict__ posf, uint2* __restrict__ posh,
+                       double q16inv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double4 p = pos[i];
   posf[i] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+  if (posh) posh[i] = q16_pack(p, q16inv);
 }
 
 __global__ void k_fill_int(int n, int* __restrict__ a, int v) {
@@ -1051,6 +1055,27 @@ __device__ __forceinline__ size_t tiled_group(int i, int capg, int g) {
 // short block wastes less of a row's clipped range at its end
 constexpr int kCandChunk = 16;
 constexpr uint32_t kCandAll = (1u << kCandChunk) - 1u;
+typedef short v2i16 __attribute__((ext_vector_type(2)));
+typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
+
+// a.lo^2 + a.hi^2 + c (signed 16-bit halves) with the accumulator c in a scalar
+// register: the three-operand v_dot2_i32_i16, which the compiler would replace
+// by a copy of c and the accumulating two-operand form
+__device__ __forceinline__ int sdot2_sacc(v2i16 a, int c) {
+  int r;
+  asm("v_dot2_i32_i16 %0, %1, %1, %2" : "=v"(r) : "v"(a), "s"(c));
+  return r;
+}
+// the same with the accumulator in a vector register
+__device__ __forceinline__ int sdot2_acc(v2i16 a, int c) {
+  int r;
+  asm("v_dot2_i32_i16 %0, %1, %1, %2" : "=v"(r) : "v"(a), "v"(c));
+  return r;
+}
+
+// entries of a lane's LDS ring in the list build (a power of two, >= 8 + kCandChunk)
+constexpr int kRing = 32;
+static_assert(kRing >= 8 + kCandChunk && (kRing & (kRing - 1)) == 0, "ring of the list build");
 
 // Tiled Verlet-list build (cell grids with at least 3 cells per dimension).
 // For each of its 9 neighbour x-rows a particle scans the candidates of the
@@ -1065,20 +1090,21 @@ constexpr uint32_t kCandAll = (1u << kCandChunk) - 1u;
 // of (rc+skin)^2, are settled by the reference's FP64 test in the reference's
 // orientation (see k_nbr_build).  Accepted candidates are appended as their
 // offset in the tile's staged region, eight entries per 16-byte store.
-template <bool HAS_EXCL>
+template <bool HAS_EXCL, bool Q16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_nbr_build_tiled(
     int n, int ntiles, const float4* __restrict__ posf, const double4* __restrict__ pos, const int* __restrict__ id,
     const int* __restrict__ cell_of_slot, const int* __restrict__ cstart, Geom g, double cutsq, float cut_lo,
     float cut_hi, const TileRun* __restrict__ runs, const int* __restrict__ nruns,
     const int* __restrict__ ex_start, const int* __restrict__ ex_ids, uint4* __restrict__ nbr,
-    int* __restrict__ cnt, int capg, int* __restrict__ lane_slot, float clip_margin,
-    const int* __restrict__ xbin, Ctl* ctl) {
+    int* __restrict__ cnt, int capg, int* __restrict__ lane_slot, float clip_margin, float row_margin,
+    const int* __restrict__ xbin, const uint2* __restrict__ posh, double q16inv, int q16_tlo, int q16_w,
+    Ctl* ctl) {
   // a lost particle: the binning put it in cell 0 and the run ends in an error;
   // do not scan what may be one huge cell.  A failed tile plan: the host plans
   // smaller tiles and builds again.
   if (ctl->nanpos || ctl->plan_fail) return;
   __shared__ TileRun sr[kMaxRuns];
-  __shared__ __align__(16) unsigned short build_ring[256 * 16];
+  __shared__ __align__(16) unsigned short build_ring[256 * kRing];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = nruns[tile];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)tile * kMaxRuns + threadIdx.x];
@@ -1094,29 +1120,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
     const int myid = HAS_EXCL ? id[i] : 0;
+    // Q16: particle i's 16-bit copy (the same for every row: the 16-bit
+    // differences pick the periodic image themselves)
+    v2u16 lxy = {0, 0}, lz = {0, 0};
+    if (Q16) {
+      const uint2 h = q16_pack(pi, q16inv);
+      lxy = __builtin_bit_cast(v2u16, h.x);
+      lz = __builtin_bit_cast(v2u16, h.y);
+    }
     uint4* out = nbr + tiled_group(i, capg, 0);
     const int cap = 8 * capg;
-    // entries not yet stored: a ring of two 8-entry groups in LDS, entry q at
-    // ring[q & 15]; a completed group is stored from it as one 16-byte write
-    unsigned short* ring = build_ring + (size_t)threadIdx.x * 16;
+    // entries not yet stored: a ring of four 8-entry groups in LDS, entry q at
+    // ring[q % kRing]; after each block of candidates the groups it completed
+    // are stored from it, one 16-byte write each (a block adds at most
+    // kCandChunk entries to fewer than 8 pending, so it never laps the ring)
+    unsigned short* ring = build_ring + (size_t)threadIdx.x * kRing;
     const uint4* ring4 = reinterpret_cast<const uint4*>(ring);
-    // reach in x: the prefilter's top with a relative margin for FP32 rounding, plus
-    // clip_margin when the x order inside the cells may be stale: a particle moved
-    // by at most d since its cell was sorted leaves the order wrong by at most 2d,
-    // and a stretch with that much more reach keeps every candidate within the
-    // prefilter's top
-    const float reach = sqrtf(cut_hi) * 1.0001f + clip_margin;
+    // the prefilter's top as a length, with a relative margin for FP32 rounding
+    const float top = sqrtf(cut_hi) * 1.0001f;
+    const float top2 = top * top;
     const float cs_x = (float)(g.L[0] / nx), inv_cs = 1.0f / cs_x;
     const float bmargin = 1e-4f * cs_x;
+    const float cs_y = (float)(g.L[1] / ny), cs_z = (float)(g.L[2] / nz);
+    // With folded positions (row_margin >= 0) a particle binned into row (qy, qz)
+    // lay inside the row's y and z range and has since moved by less than
+    // row_margin, so every candidate of the row is at least the particle's
+    // distance to that range (less the margin) away in y and z: the row's reach
+    // in x shrinks to the rest of the prefilter's top, and a row beyond the top
+    // is skipped.  1e-4 of a cell covers the FP32 rounding of the bounds.
+    const bool rowclip = row_margin >= 0.0f;
+    const float ymar = row_margin + 1e-4f * cs_y, zmar = row_margin + 1e-4f * cs_z;
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nz; sz = -1; } else if (qz >= nz) { qz -= nz; sz = 1; }
       // particle i moved by minus the candidate cells' periodic shift, in FP32
       const float zs = (float)(pi.z - (double)sz * g.L[2]);
+      const float zlo = (float)qz * cs_z;
+      const float dz_lo = fmaxf(fmaxf(zlo - zs, zs - (zlo + cs_z)) - zmar, 0.0f);
       for (int oy = -1; oy <= 1; ++oy) {
         int qy = cy + oy, sy = 0;
         if (qy < 0) { qy += ny; sy = -1; } else if (qy >= ny) { qy -= ny; sy = 1; }
         const float ys = (float)(pi.y - (double)sy * g.L[1]);
+        const float ylo = (float)qy * cs_y;
+        const float dy_lo = fmaxf(fmaxf(ylo - ys, ys - (ylo + cs_y)) - ymar, 0.0f);
+        const float rx2 = rowclip ? top2 - __builtin_fmaf(dy_lo, dy_lo, dz_lo * dz_lo) : top2;
+        if (!(rx2 > 0.0f)) continue;
+        // reach in x: the row's share of the prefilter's top, plus clip_margin when
+        // the x order inside the cells may be stale: a particle moved by at most d
+        // since its cell was sorted leaves the order wrong by at most 2d, and a
+        // stretch with that much more reach keeps every candidate within the top
+        const float reach = sqrtf(rx2) * 1.0001f + clip_margin;
         const int row = (qy + ny * qz) * nx;
         // x pieces of the row segment (cx-1 .. cx+1), split where the row wraps
         int pa0, pb0, pa1 = 0, pb1 = -1;
@@ -1166,22 +1219,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
           }
           for (int j0 = ja; j0 < je; j0 += kCandChunk) {
             const int jn = je - j0;
-            const float4* cand = posf + j0;
             uint32_t in_bits = 0, maybe_bits = 0;
-            // candidates kCandChunk-1 .. 0, so that candidate t ends at bit t of the masks
+            // candidates kCandChunk-1 .. 0, so that candidate t ends at bit t of the masks;
+            // each test shifts in the sign bit of r^2 - cut_lo (surely in) and of
+            // r^2 - cut_hi (in or undecided: the window's top lies above every r^2
+            // that may be in)
+            if (Q16) {
+              // r^2 - cut_lo in 16-bit units: two packed differences and two dot
+              // products; the sums stay below 2^31 for every candidate of a row
+              const uint2* cand = posh + j0;
+              const int neg_tlo = -q16_tlo;
 #pragma unroll
-            for (int u = kCandChunk - 4; u >= 0; u -= 4) {
-              float4 q[4];
+              for (int u = kCandChunk - 4; u >= 0; u -= 4) {
+                uint2 q[4];
 #pragma unroll
-              for (int v = 0; v < 4; ++v) q[v] = cand[u + v];
+                for (int v = 0; v < 4; ++v) q[v] = cand[u + v];
 #pragma unroll
-              for (int v = 3; v >= 0; --v) {
-                const float dx = xs - q[v].x, dy = ys - q[v].y, dz = zs - q[v].z;
-                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                // the sign bit of r2 - cut shifted in at the bottom: r2 < cut_lo, and
-                // r2 < cut_hi (the window's top lies above every r2 that may be in)
-                in_bits = __builtin_amdgcn_alignbit(in_bits, __float_as_uint(r2 - cut_lo), 31);
-                maybe_bits = __builtin_amdgcn_alignbit(maybe_bits, __float_as_uint(r2 - cut_hi), 31);
+                for (int v = 3; v >= 0; --v) {
+                  const v2i16 dxy = __builtin_bit_cast(v2i16, lxy - __builtin_bit_cast(v2u16, q[v].x));
+                  const v2i16 dz = __builtin_bit_cast(v2i16, lz - __builtin_bit_cast(v2u16, q[v].y));
+                  const int t = sdot2_acc(dz, sdot2_sacc(dxy, neg_tlo));
+                  in_bits = __builtin_amdgcn_alignbit(in_bits, (uint32_t)t, 31);
+                  maybe_bits = __builtin_amdgcn_alignbit(maybe_bits, (uint32_t)(t - q16_w), 31);
+                }
+              }
+            } else {
+              const float4* cand = posf + j0;
+              const float win = cut_hi - cut_lo;
+#pragma unroll
+              for (int u = kCandChunk - 4; u >= 0; u -= 4) {
+                float4 q[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) q[v] = cand[u + v];
+#pragma unroll
+                for (int v = 3; v >= 0; --v) {
+                  const float dx = xs - q[v].x, dy = ys - q[v].y, dz = zs - q[v].z;
+                  // r^2 - cut_lo with the cut folded into the chain of FMAs
+                  const float t = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, -cut_lo)));
+                  in_bits = __builtin_amdgcn_alignbit(in_bits, __float_as_uint(t), 31);
+                  maybe_bits = __builtin_amdgcn_alignbit(maybe_bits, __float_as_uint(t - win), 31);
+                }
               }
             }
             uint32_t valid = jn >= kCandChunk ? kCandAll : ((1u << jn) - 1u);
@@ -1207,6 +1284,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
               if (exact_sqr_le(ex, ey, ez, cutsq)) in_bits |= 1u << t;
             }
             const uint32_t ebase = (uint32_t)(base + (j0 - jb));   // entry of candidate 0
+            const int k0 = k;
             // up to four entries per pass, so the wavefront's passes follow the
             // largest count of its lanes divided by four
             while (in_bits) {
@@ -1216,14 +1294,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                   const int t = __ffs(in_bits) - 1;
                   in_bits &= in_bits - 1u;
                   if (!(HAS_EXCL && is_excluded(myid, id[j0 + t], ex_start, ex_ids))) {
-                    ring[k & 15] = (unsigned short)(ebase + (uint32_t)t);
+                    ring[k & (kRing - 1)] = (unsigned short)(ebase + (uint32_t)t);
                     ++k;
-                    if ((k & 7) == 0 && k <= cap)
-                      out[(size_t)((k >> 3) - 1) * 64] = ring4[((k >> 3) - 1) & 1];
                   }
                 }
               }
             }
+            // the groups this block completed (at most two), inside the capacity
+            for (int gq = k0 >> 3; gq < min(k >> 3, capg); ++gq)
+              out[(size_t)gq * 64] = ring4[gq & (kRing / 8 - 1)];
           }
         }
       }
@@ -1231,8 +1310,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     // last partial group: the entries past the count are zero (offset 0 stages a
     // valid slot; the pair kernels read but never use them)
     if ((k & 7) != 0 && k < cap) {
-      for (int r = k & 7; r < 8; ++r) ring[(k & 8) + r] = 0;
-      out[(size_t)(k >> 3) * 64] = ring4[(k >> 3) & 1];
+      for (int r = k & 7; r < 8; ++r) ring[(k & (kRing - 8)) + r] = 0;
+      out[(size_t)(k >> 3) * 64] = ring4[(k >> 3) & (kRing / 8 - 1)];
     }
     cnt[i] = k;
     if (k > cap) ctl->overflow = 1;
@@ -2364,10 +2443,11 @@ void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, 
                     const double4* vel, const double* fx, const double* fy, const double* fz,
                     const int* id, const int4* img, double4* pos2, double4* vel2, double* fx2,
                     double* fy2, double* fz2, int* id2, int4* img2, int* cell_of_slot,
-                    int* slot_of_id, uint4* posq, float4* posf, const Geom& g) {
+                    int* slot_of_id, uint4* posq, float4* posf, uint2* posh, const Geom& g) {
   if (n > 0)
     hipLaunchKernelGGL(k_permute, dim3(nblocks(n, 256)), dim3(256), 0, s, n, order, cell_of, pos, vel, fx, fy, fz, id, img,
-                       pos2, vel2, fx2, fy2, fz2, id2, img2, cell_of_slot, slot_of_id, posq, posf, g);
+                       pos2, vel2, fx2, fy2, fz2, id2, img2, cell_of_slot, slot_of_id, posq, posf,
+                       g.q16inv > 0.0 ? posh : nullptr, g);
 }
 
 void launch_spec_begin(hipStream_t s, SpecState* sp, double dmax, double skin_half) {
@@ -2381,8 +2461,10 @@ void launch_xbins(hipStream_t s, int n, int ncell, const int* cstart, const int*
   hipLaunchKernelGGL(k_xbins, dim3(nblocks(m, 256)), dim3(256), 0, s, n, ncell, cstart, cell_of_slot, posf, g, xbin);
 }
 
-void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf) {
-  if (n > 0) hipLaunchKernelGGL(k_posf, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, posf);
+void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf, uint2* posh, double q16inv) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_posf, dim3(nblocks(n, 256)), dim3(256), 0, s, n, pos, posf, q16inv > 0.0 ? posh : nullptr,
+                       q16inv);
 }
 
 void launch_fill_int(hipStream_t s, int n, int* a, int v) {
@@ -2443,13 +2525,15 @@ void launch_nbr_build_tiled(hipStream_t s, int n, int T, const float4* posf, con
                             const int* id, const int* cell_of_slot, const int* cstart, const Geom& g, double cutsq,
                             float cut_lo, float cut_hi, const TileRun* runs, const int* nruns, const int* ex_start,
                             const int* ex_ids, uint4* nbr, int* cnt, int capg, int* lane_slot,
-                            float clip_margin, const int* xbin, Ctl* ctl) {
+                            float clip_margin, float row_margin, const int* xbin, const uint2* posh,
+                            double q16inv, int q16_tlo, int q16_w, Ctl* ctl) {
   if (n <= 0) return;
   const int ntiles = nblocks(n, T);
   dim3 grid(ntiles), block(T);
-#define NB(EX) hipLaunchKernelGGL((k_nbr_build_tiled<EX>), grid, block, 0, s, n, ntiles, posf, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, lane_slot, clip_margin, xbin, ctl)
-  if (ex_start) NB(true);
-  else NB(false);
+#define NB(EX, Q) hipLaunchKernelGGL((k_nbr_build_tiled<EX, Q>), grid, block, 0, s, n, ntiles, posf, pos, id, cell_of_slot, cstart, g, cutsq, cut_lo, cut_hi, runs, nruns, ex_start, ex_ids, nbr, cnt, capg, lane_slot, clip_margin, row_margin, xbin, posh, q16inv, q16_tlo, q16_w, ctl)
+  const bool q16 = q16inv > 0.0 && posh;
+  if (ex_start) { if (q16) NB(true, true); else NB(true, false); }
+  else { if (q16) NB(false, true); else NB(false, false); }
 #undef NB
 }
 

@@ -34,6 +34,24 @@ namespace {
 
 constexpr size_t kPosqPad = 64;   // spare fixed-point mirror entries after the last particle
 
+// Scale 1 / delta of the list build's 16-bit copies (q16_pack), or 0 where they
+// do not apply (a grid that is not cubic, or a box too small for one period).
+// delta = L / (2^16 m) keeps a box length a whole number of 2^16-unit periods,
+// so a 16-bit difference is that of the periodic image within 2^15 units.  m is
+// the largest whole number for which every candidate of a particle's rows stays
+// below 32000 units per coordinate (in x the clipped stretch: one cell, an
+// eighth of slack and the drift terms; in y and z the two cells across the row
+// plus twice the drift) and the three squares sum below 46000^2 < 2^31.
+double q16_scale(const Geom& g, double disp) {
+  if (!(g.L[0] == g.L[1] && g.L[1] == g.L[2] && g.n[0] == g.n[1] && g.n[1] == g.n[2])) return 0.0;
+  const double cs = g.L[0] / g.n[0];
+  const double bx = cs * 1.0012 + cs / 8.0 + 4.0 * disp;
+  const double byz = 2.0 * cs * 1.0001 + 2.0 * disp;
+  const double dmin = std::max(std::max(bx, byz) / 32000.0, std::sqrt(bx * bx + 2.0 * byz * byz) / 46000.0);
+  const double m = std::floor(g.L[0] / (65536.0 * dmin));
+  return m >= 1.0 ? 65536.0 * m / g.L[0] : 0.0;
+}
+
 // Device scratch of espp_ia_energy_noalloc: per-workgroup partial sums of an
 // interaction's energy and virial, and their reduction in the last 8 entries.
 // One copy per GPU, held until the process ends; every call on that GPU shares it.
@@ -289,6 +307,7 @@ struct espp_sim {
   DBuf<uint4> posq;
   DBuf<uint4> posq2;              // the next step's mirror while a fused force kernel reads posq
   DBuf<float4> posf;              // FP32 positions, candidates of the tiled list build
+  DBuf<uint2> posh;               // 16-bit modular positions (q16_pack), the same for cubic grids
   DBuf<int> cell_of, rank, order, cell_of_slot, cell_count, cstart;
   DBuf<int> xbin;                 // per cell, the x sub-bin starts of the tiled list build
   DBuf<int> scan_sum, scan_off;   // per-chunk totals and offsets of the cell scan
@@ -342,6 +361,7 @@ struct espp_sim {
   int64_t prunes = 0, pruned_walks = 0;   // step kernels that pruned / walked a pruned list
   double disp_since_bin = 0.0;    // bound on any particle's displacement since the last binning
   double coord_bound = 0.0;       // bound on |coordinate| of the current positions
+  bool bin_folded = false;        // the last binning folded every position into the box
   bool resortFlag = true;
   int64_t nResorts = 0;
 
@@ -461,6 +481,7 @@ struct espp_sim {
     posq.grow_keep(m + kPosqPad, keep, stream);
     posq2.resize(m + kPosqPad);
     posf.resize(m + kPosqPad);
+    if (g.q16inv > 0.0) posh.resize(m + kPosqPad);
   }
 
   int64_t ncell() const { return (int64_t)g.n[0] * g.n[1] * g.n[2]; }
@@ -493,7 +514,7 @@ struct espp_sim {
       launch_sort_cells(stream, (int)ncell(), cstart.p, order.p, pos.p, ctl.p);
       launch_permute(stream, ni, order.p, cell_of.p, pos.p, vel.p, fx.p, fy.p, fz.p, id.p, img.p, pos2.p,
                      vel2.p, carry_forces ? fx2.p : nullptr, fy2.p, fz2.p, id2.p, img2.p, cell_of_slot.p,
-                     slot_of_id.p, posq.p, posf.p, g);
+                     slot_of_id.p, posq.p, posf.p, posh.p, g);
       pos.swap(pos2); vel.swap(vel2);
       if (carry_forces) { fx.swap(fx2); fy.swap(fy2); fz.swap(fz2); }
       id.swap(id2); img.swap(img2);
@@ -504,6 +525,7 @@ struct espp_sim {
     plan_valid = false;
     for (auto* v : vls) v->stale = true;
     if (do_fold) coord_bound = std::max(g.L[0], std::max(g.L[1], g.L[2]));
+    bin_folded = do_fold;
     sorted = true;
     if (defer_check) {
       nan_pending = true;
@@ -631,8 +653,6 @@ struct espp_sim {
     }
     v.cnt.resize((size_t)std::max<int64_t>(n, 1));
     v.tiled = T > 0 && !v.untiled_only;
-    // the FP32 candidate positions are the binning's until a particle moves
-    if (v.tiled && disp_since_bin > 0.0) launch_posf(stream, (int)n, pos.p, posf.p);
     // FP32 prefilter window of the tiled build.  Its coordinates (a position, or
     // one shifted by a box length) are below M in magnitude; each is rounded to
     // FP32 once (half an ulp of M), so a difference is off by at most one ulp and
@@ -645,6 +665,29 @@ struct espp_sim {
     const float cut_lo_q = (float)(cutsq * (1.0 - win)), cut_hi_q = (float)(cutsq * (1.0 + win));
     // particles moved since the binning sorted the cells by x: widen the x clip
     const float clip_margin = disp_since_bin > 0.0 ? (float)(2.0 * disp_since_bin * (1.0 + 1e-6) + 4.0 * ulp) : 0.0f;
+    // rows clipped in y and z when the binning folded every position into the box
+    // (each cell then holds only particles of its own range); the margin covers the
+    // particles' moves since, negative turns the clip off
+    const float row_margin = bin_folded ? (float)(disp_since_bin * (1.0 + 1e-6) + 4.0 * ulp) : -1.0f;
+    // 16-bit candidate copies (cubic grids, positions folded at the binning) with
+    // a scale for the drift since the binning, or FP32 ones
+    const bool q16_fits = std::sqrt((double)cut_hi_q) * 1.0001 <= 1.001 * g.L[0] / g.n[0];   // the x bound of q16_scale
+    const double q16 = (v.tiled && bin_folded && q16_fits) ? q16_scale(g, disp_since_bin) : 0.0;
+    // the candidate copies are the binning's until a particle moves
+    if (v.tiled && (disp_since_bin > 0.0 || (q16 > 0.0 && q16 != g.q16inv)))
+      launch_posf(stream, (int)n, pos.p, posf.p, posh.p, q16);
+    // Thresholds of r^2 in 16-bit units (U = (rc+skin) / delta): each coordinate
+    // of a copy is within half a unit of the position, so a difference is off by
+    // at most one unit and r^2 by at most 2 sqrt(3) |d| + 3 units^2.  Below q16_tlo
+    // the pair is surely within rc+skin, at or above q16_tlo + q16_w surely not.
+    int q16_tlo = 0, q16_w = 0;
+    if (q16 > 0.0) {
+      const double U = cutVerlet * q16, U2 = U * U;
+      const double tlo = std::floor(U2 - 2.0 * std::sqrt(3.0) * U * 1.0001 - 8.0);
+      const double thi = std::ceil(U2 + 2.0 * std::sqrt(3.0) * U * 1.01 + 16.0);
+      q16_tlo = (int)std::max(tlo, 0.0);
+      q16_w = (int)(thi - q16_tlo);
+    }
     for (int attempt = 0; attempt < 4; ++attempt) {
       launch_reset_ctl(stream, ctl.p, kCtlBuild);
       hipEvent_t pe = prof_begin();
@@ -655,7 +698,8 @@ struct espp_sim {
         launch_nbr_build_tiled(stream, (int)n, T, posf.p, pos.p, id.p, cell_of_slot.p, cstart.p, g,
                                cutsq, cut_lo_q, cut_hi_q, runs.p, nruns.p, v.has_excl ? v.ex_start.p : nullptr,
                                v.has_excl ? v.ex_ids.p : nullptr, (uint4*)v.nbr.p, v.cnt.p, v.cap / 8,
-                               v.lane_slot.p, clip_margin, xbin.p, ctl.p);
+                               v.lane_slot.p, clip_margin, row_margin, xbin.p, posh.p, q16, q16_tlo, q16_w,
+                               ctl.p);
       } else {
         v.nbr.resize((size_t)std::max(ngroups, 1) * (size_t)v.cap * 64);
         launch_nbr_build(stream, (int)n, pos.p, id.p, cell_of_slot.p, cstart.p, g, cutsq,
@@ -1259,6 +1303,7 @@ void set_geom(Geom& g, const double box[3], const int32_t cells[3], int hci) {
   }
   g.qs2 = g.qsd[0] * g.qsd[0];
   g.qsf = (float)g.qsd[0];
+  g.q16inv = q16_scale(g, 0.0);
 }
 
 template <class T>
