@@ -1073,9 +1073,19 @@ __device__ __forceinline__ int sdot2_acc(v2i16 a, int c) {
   return r;
 }
 
-// entries of a lane's LDS ring in the list build (a power of two, >= 8 + kCandChunk)
-constexpr int kRing = 32;
-static_assert(kRing >= 8 + kCandChunk && (kRing & (kRing - 1)) == 0, "ring of the list build");
+// entries of a lane's LDS staging buffer in the list build (pending and new entries
+// of one block)
+constexpr int kRing = 8 + kCandChunk;
+
+// the 8 list entries at positions p0 .. p0+7 of a thread's staging buffer (position
+// p at r[p * 256]) as one 16-byte group
+__device__ __forceinline__ uint4 ring_group(const unsigned short* r, int p0) {
+  uint32_t w[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    w[q] = (uint32_t)r[(p0 + 2 * q) * 256] | ((uint32_t)r[(p0 + 2 * q + 1) * 256] << 16);
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
 
 // Tiled Verlet-list build (cell grids with at least 3 cells per dimension).
 // For each of its 9 neighbour x-rows a particle scans the candidates of the
@@ -1104,7 +1114,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   // smaller tiles and builds again.
   if (ctl->nanpos || ctl->plan_fail) return;
   __shared__ TileRun sr[kMaxRuns];
-  __shared__ __align__(16) unsigned short build_ring[256 * kRing];
+  __shared__ unsigned short build_ring[kRing * 256];
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int nr = nruns[tile];
   if ((int)threadIdx.x < nr) sr[threadIdx.x] = runs[(size_t)tile * kMaxRuns + threadIdx.x];
@@ -1130,12 +1140,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     }
     uint4* out = nbr + tiled_group(i, capg, 0);
     const int cap = 8 * capg;
-    // entries not yet stored: a ring of four 8-entry groups in LDS, entry q at
-    // ring[q % kRing]; after each block of candidates the groups it completed
-    // are stored from it, one 16-byte write each (a block adds at most
-    // kCandChunk entries to fewer than 8 pending, so it never laps the ring)
-    unsigned short* ring = build_ring + (size_t)threadIdx.x * kRing;
-    const uint4* ring4 = reinterpret_cast<const uint4*>(ring);
+    // entries not yet stored, in LDS: the pending part of the current 8-entry
+    // group at positions [0, k & 7), a block's new entries after it; after each
+    // block the groups it completed are stored, one 16-byte write each, and the
+    // partial group moves back to positions 0..7 (a block adds at most kCandChunk
+    // entries to fewer than 8 pending, so they fit).  Position p of this thread is
+    // ring[p * 256]: the threads of a wavefront sit side by side at every
+    // position, so their 16-bit accesses spread over all banks whatever
+    // positions they address.
+    unsigned short* ring = build_ring + threadIdx.x;
     // the prefilter's top as a length, with a relative margin for FP32 rounding
     const float top = sqrtf(cut_hi) * 1.0001f;
     const float top2 = top * top;
@@ -1283,26 +1296,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
               else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
               if (exact_sqr_le(ex, ey, ez, cutsq)) in_bits |= 1u << t;
             }
-            const uint32_t ebase = (uint32_t)(base + (j0 - jb));   // entry of candidate 0
-            const int k0 = k;
-            // up to four entries per pass, so the wavefront's passes follow the
-            // largest count of its lanes divided by four
-            while (in_bits) {
-#pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                if (in_bits) {
-                  const int t = __ffs(in_bits) - 1;
-                  in_bits &= in_bits - 1u;
-                  if (!(HAS_EXCL && is_excluded(myid, id[j0 + t], ex_start, ex_ids))) {
-                    ring[k & (kRing - 1)] = (unsigned short)(ebase + (uint32_t)t);
-                    ++k;
-                  }
-                }
+            if (HAS_EXCL) {
+              // drop the excluded pairs (one lookup per accepted candidate)
+              for (uint32_t b = in_bits; b; b &= b - 1u) {
+                const int t = __ffs(b) - 1;
+                if (is_excluded(myid, id[j0 + t], ex_start, ex_ids)) in_bits &= ~(1u << t);
               }
             }
-            // the groups this block completed (at most two), inside the capacity
-            for (int gq = k0 >> 3; gq < min(k >> 3, capg); ++gq)
-              out[(size_t)gq * 64] = ring4[gq & (kRing / 8 - 1)];
+            const uint32_t ebase = (uint32_t)(base + (j0 - jb));   // entry of candidate 0
+            // Every candidate t writes its entry at the ring position of its rank
+            // among the accepted ones, the k & 7 pending entries first; a rejected
+            // one writes to the next accepted one's position (overwritten by it,
+            // as later writes of a lane land after earlier ones) or past the last.
+            // No branch, so no lane waits on another's count.
+            const int kp = k & 7;
+#pragma unroll
+            for (int t = 0; t < kCandChunk; ++t)
+              ring[(kp + __popc(in_bits & ((1u << t) - 1u))) * 256] = (unsigned short)(ebase + (uint32_t)t);
+            const int kn = kp + __popc(in_bits);
+            // the groups this block completed (at most two), inside the capacity,
+            // then the partial group back to positions 0..7
+            const int g0 = k >> 3;
+            for (int gq = 0; gq < min(kn >> 3, capg - g0); ++gq) out[(size_t)(g0 + gq) * 64] = ring_group(ring, 8 * gq);
+            if (kn >= 8) {
+              const int src = 8 * (kn >> 3);
+#pragma unroll
+              for (int e = 0; e < 8; ++e) ring[e * 256] = ring[(src + e) * 256];
+            }
+            k += kn - kp;
           }
         }
       }
@@ -1310,8 +1331,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     // last partial group: the entries past the count are zero (offset 0 stages a
     // valid slot; the pair kernels read but never use them)
     if ((k & 7) != 0 && k < cap) {
-      for (int r = k & 7; r < 8; ++r) ring[(k & (kRing - 8)) + r] = 0;
-      out[(size_t)(k >> 3) * 64] = ring4[(k >> 3) & (kRing / 8 - 1)];
+      for (int r = k & 7; r < 8; ++r) ring[r * 256] = 0;
+      out[(size_t)(k >> 3) * 64] = ring_group(ring, 0);
     }
     cnt[i] = k;
     if (k > cap) ctl->overflow = 1;

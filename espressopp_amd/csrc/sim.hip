@@ -545,6 +545,9 @@ struct espp_sim {
     if (!sorted) bin(false);
   }
 
+  // entries of a tile's staged region the plan allows (the force kernels' LDS)
+  int plan_budget() const { return (lds_cap - 12288) / (mixed() ? 16 : 32); }
+
   // Tile plan for the current slot order: the largest tile size whose staged
   // regions fit the LDS budget; T = 0 selects the untiled kernels.
   // With tentative, one tile size is launched without waiting for its result:
@@ -568,8 +571,7 @@ struct espp_sim {
     if (n == 0) return;
     // the tiled list build assumes at least 3 cells per dimension
     if (g.n[0] < 3 || g.n[1] < 3 || g.n[2] < 3) return;
-    const int elem = mixed() ? 16 : 32;
-    const int budget = (lds_cap - 12288) / elem;
+    const int budget = plan_budget();
     const int sizes_mixed[3] = {256, 128, 64}, sizes_fp64[2] = {128, 64};
     const int* sizes = mixed() ? sizes_mixed : sizes_fp64;
     const int nsz = mixed() ? 3 : 2;
