@@ -106,6 +106,38 @@ def test_pairs_borderline_distances(gpu):
     assert pair_set(vl.pairs_array()) == pair_set(ref.vl_pairs(rvl))
 
 
+@pytest.mark.parametrize("L", [20.0, 40.0])
+def test_pairs_borderline_distances_16bit(gpu, L):
+    # Cubic grids wide enough for the list build's 16-bit position copies (delta = L / (2^16 m),
+    # m >= 1 below; a box of 10 is too small and takes the FP32 prefilter): pairs at (rc+skin)
+    # times 1 +- a few ulp (the reference's FP64 test decides), 1 +- 1e-4 (inside the 16-bit
+    # window) and 1 +- 1e-3 (outside it), along axes and diagonals, inside the box and across
+    # its periodic boundary, among random particles
+    rc, skin = 2.5, 0.3
+    cut = rc + skin
+    c = int(L // cut)
+    cs = L / c
+    bx, byz = cs * 1.0012 + cs / 8.0, 2.0 * cs * 1.0001
+    dmin = max(max(bx, byz) / 32000.0, math.sqrt(bx * bx + 2.0 * byz * byz) / 46000.0)
+    assert math.floor(L / (65536.0 * dmin)) >= 1
+    rng = np.random.default_rng(int(L))
+    dirs = [np.asarray(v, dtype=float) / np.linalg.norm(v)
+            for v in ((1, 0, 0), (0, 1, 0), (0, 0, -1), (1, 1, 0), (1, -1, 1), (0.3, 0.7, -0.2))]
+    pos = [rng.uniform(0.0, L, 3) for _ in range(1500)]
+    for d in dirs:
+        for rel in (-1e-3, -1e-4, -4e-16, -2e-16, 0.0, 2e-16, 4e-16, 1e-4, 1e-3):
+            for edge in (False, True):
+                a = rng.uniform(0.0, L, 3)
+                if edge:                       # the partner lies across the periodic boundary
+                    a = np.where(d > 0, L - 0.2 * rng.uniform(0.0, 1.0, 3), np.where(d < 0, 0.2 * rng.uniform(0.0, 1.0, 3), a))
+                b = a + d * (cut * (1.0 + rel))
+                pos.append(a)
+                pos.append(b - np.floor(b / L) * L)
+    pos = np.asarray(pos)
+    system, vl, _, ref, rvl, _ = build_lj(pos, (L, L, L), (c, c, c), rc, skin, "mixed")
+    assert pair_set(vl.pairs_array()) == pair_set(ref.vl_pairs(rvl))
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("cells", [(2, 2, 2), (3, 3, 3)])
 def test_forces_borderline_cutoff(gpu, precision, cells):
