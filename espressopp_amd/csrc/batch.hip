@@ -314,15 +314,19 @@ __global__ __launch_bounds__(kPartThreads) void kb_scatter(const BlockMap* __res
   order[d.slot0 + cstart[d.cell0 + tcell[s]] + trank[s]] = s - d.slot0;
 }
 
-// Each cell ordered by particle id, so that the slot order does not depend on
-// the atomics' order: one wavefront per cell, up to four entries per lane,
-// ranks by comparison.  A cell map block covers kSortCells cells of one replica.
+// Each cell ordered by x, ties by particle id, so that the slot order does not
+// depend on the atomics' order and a row of cells is x-monotone for the list
+// build's clipping: one wavefront per cell, up to four entries per lane, ranks
+// by comparison.  A cell map block covers kSortCells cells of one replica.  An
+// empty cell's x sub-bin table (kb_xbins fills the others) is its start.
 constexpr int kSortCells = 16;
 __global__ __launch_bounds__(kPartThreads) void kb_sort_cells(const BlockMap* __restrict__ cm,
                                                               const RepDesc* __restrict__ rd,
                                                               RepState* __restrict__ st,
                                                               const int* __restrict__ cstart,
-                                                              const int* __restrict__ id, int* __restrict__ order) {
+                                                              const double4* __restrict__ pos,
+                                                              const int* __restrict__ id, int* __restrict__ order,
+                                                              int* __restrict__ xbin) {
   const BlockMap m = cm[blockIdx.x];
   const int r = m.rep;
   if (!st[r].rebuild) return;
@@ -332,24 +336,28 @@ __global__ __launch_bounds__(kPartThreads) void kb_sort_cells(const BlockMap* __
   for (int cc = wv; cc < m.count; cc += kPartThreads / 64) {
     const int c = m.s0 + cc;                      // cell index within the replica
     const int b = cstart[d.cell0 + c], mlen = cstart[d.cell0 + c + 1] - b;
+    if (mlen == 0 && lane <= kXBins) xbin[(size_t)(d.cell0 + c) * (kXBins + 1) + lane] = s0 + b;
     if (mlen <= 1) continue;
     if (mlen > kMaxCellFill) {
       if (lane == 0) st[r].cellfull = 1;
       continue;
     }
     int ent[4], key[4];
+    double kx[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int k = lane + 64 * u;
       ent[u] = k < mlen ? order[s0 + b + k] : 0;
       key[u] = k < mlen ? id[s0 + ent[u]] : INT_MAX;
+      kx[u] = k < mlen ? pos[s0 + ent[u]].x : __builtin_inf();
     }
     int rk[4] = {0, 0, 0, 0};
     if (mlen <= 64) {
       // one entry per lane (the common case)
       for (int q = 0; q < mlen; ++q) {
         const int other = __shfl(key[0], q, 64);
-        rk[0] += other < key[0] || (other == key[0] && q < lane);
+        const double ox = __shfl(kx[0], q, 64);
+        rk[0] += ox < kx[0] || (ox == kx[0] && (other < key[0] || (other == key[0] && q < lane)));
       }
     } else {
 #pragma unroll
@@ -357,12 +365,14 @@ __global__ __launch_bounds__(kPartThreads) void kb_sort_cells(const BlockMap* __
         if (64 * u2 >= mlen) break;     // wavefront-uniform
         for (int q = 0; q < 64; ++q) {
           const int other = __shfl(key[u2], q, 64);
+          const double ox = __shfl(kx[u2], q, 64);
           const int ok = q + 64 * u2;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int k = lane + 64 * u;
             // ids are unique within a replica; ties (padding) keep the entry index order
-            rk[u] += (ok < mlen) && (other < key[u] || (other == key[u] && ok < k));
+            rk[u] += (ok < mlen) &&
+                     (ox < kx[u] || (ox == kx[u] && (other < key[u] || (other == key[u] && ok < k))));
           }
         }
       }
@@ -421,19 +431,54 @@ __global__ __launch_bounds__(kPartThreads) void kb_copy_back(const BlockMap* __r
   posq[s] = make_uint4(fixq(p.x, qinv), fixq(p.y, qinv), fixq(p.z, qinv), (uint32_t)(int)p.w);
 }
 
+// Per cell of a rebuilding replica, xbin[(cell0 + c) (kXBins+1) + k] is the
+// first slot of the cell whose x sub-bin is at least k (k < kXBins), and the
+// cell's end for k = kXBins (absolute slots).  Cells are sorted by x, so the
+// sub-bins rise along a cell: slot s writes the entries from its predecessor's
+// sub-bin (exclusive) to its own, the last slot of a cell the entries above
+// its own (kb_sort_cells fills the empty cells).
+__global__ __launch_bounds__(kPartThreads) void kb_xbins(const BlockMap* __restrict__ bm,
+                                                         const RepDesc* __restrict__ rd,
+                                                         const RepState* __restrict__ st,
+                                                         const int* __restrict__ cell_of_slot,
+                                                         const float4* __restrict__ posf, int* __restrict__ xbin) {
+  const BlockMap m = bm[blockIdx.x];
+  const int r = m.rep;
+  if (!st[r].rebuild || threadIdx.x >= (unsigned)m.count) return;
+  const RepDesc& d = rd[r];
+  const int s = m.s0 + threadIdx.x;
+  const int nc = d.nc;
+  const float inv_cs = 1.0f / (float)(d.L / nc);
+  const int c = cell_of_slot[s];
+  const int cx = c % nc;
+  int* out = xbin + (size_t)(d.cell0 + c) * (kXBins + 1);
+  const int k = xsub_bin(posf[s].x, inv_cs, cx);
+  const int kp = (s > d.slot0 && cell_of_slot[s - 1] == c) ? xsub_bin(posf[s - 1].x, inv_cs, cx) : -1;
+  for (int q = kp + 1; q <= k; ++q) out[q] = s;
+  if (s + 1 == d.slot0 + d.n || cell_of_slot[s + 1] != c)
+    for (int q = k + 1; q <= kXBins; ++q) out[q] = s + 1;
+}
+
 // Full Verlet list of every replica that rebuilt this step.  Membership is the
 // reference's half-list test (src/VerletList.cpp:247-276): d^2 <= (rc+skin)^2 in
 // FP64 without contraction, d in the orientation of the half-shell visit (cell
 // offset of j lexicographically positive, z most significant: d = xi - (xj + sL),
 // else d = xj - (xi - sL)).  Entry k of slot s: nbr[((s/64)*cap + k)*64 + s%64].
-__global__ __launch_bounds__(kPartThreads) __attribute__((amdgpu_waves_per_eu(5))) void kb_build(const BlockMap* __restrict__ bm,
-                                                         const RepDesc* __restrict__ rd,
-                                                         RepState* __restrict__ st, const double4* __restrict__ pos,
-                                                         const float4* __restrict__ posf,
-                                                         const int* __restrict__ cell_of_slot,
-                                                         const int* __restrict__ cstart, int* __restrict__ nbr,
-                                                         int* __restrict__ cnt, int cap) {
-#pragma clang fp contract(off)
+//
+// A particle scans 9 neighbour x-rows of three cells, each one contiguous slot
+// range (two where the row wraps).  The binning has just folded every position
+// into its cell, so a row whose (y, z) range lies beyond the prefilter's top is
+// skipped and the others keep the x reach the y/z distance leaves; cells are
+// sorted by x, and the x sub-bin table (kb_xbins) cuts each range to a stretch
+// around x_i +- reach.  Sixteen candidates at a time, the FP32 copies form two
+// bit masks without a branch (r^2 below the window: in; below its top: in or
+// undecided); the undecided take the reference's FP64 test.
+__global__ __launch_bounds__(kPartThreads) __attribute__((amdgpu_waves_per_eu(6))) void kb_build(
+    const BlockMap* __restrict__ bm, const RepDesc* __restrict__ rd, RepState* __restrict__ st,
+    const double4* __restrict__ pos, const float4* __restrict__ posf, const int* __restrict__ cell_of_slot,
+    const int* __restrict__ cstart, const int* __restrict__ xbin, int* __restrict__ nbr, int* __restrict__ cnt,
+    int cap) {
+  constexpr int kChunk = 16;
   const BlockMap m = bm[blockIdx.x];
   const int r = m.rep;
   if (!st[r].rebuild || st[r].nanpos || st[r].cellfull) return;
@@ -443,67 +488,118 @@ __global__ __launch_bounds__(kPartThreads) __attribute__((amdgpu_waves_per_eu(5)
   if (t < m.count) {
     const int i = m.s0 + t;
     const double4 pi = pos[i];
-    const float cflo = (float)d.cflo, cfhi = (float)d.cfhi;
     const int nc = d.nc, s0 = d.slot0;
     const int c = cell_of_slot[i];
     const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
     int* out = nbr + (size_t)(i >> 6) * (size_t)cap * 64 + (i & 63);
     const double L = d.L, cutsq = d.cutsq;
-    // A neighbour cell whose box lies farther than the cutoff from particle i
-    // holds no partner: the binning just folded every position into its cell
-    // (to FP64 rounding, which gmargin covers), so that cell is skipped.
-    const double cs = L / (double)nc, gmargin = 1e-9 * L;
-    auto gap = [&](double xi, int q, double sh) {
-      return fmax(0.0, fmax((double)q * cs + sh - xi, xi - ((double)(q + 1) * cs + sh)) - gmargin);
-    };
+    const float cflo = (float)d.cflo, cfhi = (float)d.cfhi, win = cfhi - cflo;
+    const float cs = (float)(L / nc), inv_cs = 1.0f / cs;
+    // 1e-4 of a cell covers the FP32 rounding of the bounds and of the sub-bins
+    const float bmargin = 1e-4f * cs;
+    const float top = sqrtf(cfhi) * 1.0001f, top2 = top * top;
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nc; sz = -1; } else if (qz >= nc) { qz -= nc; sz = 1; }
-      const double gz = oz == 0 ? 0.0 : gap(pi.z, qz, (double)sz * L);
       // particle i moved by minus the candidate cells' periodic shift, in FP32
       const float zs = (float)(pi.z - (double)sz * L);
+      const float zlo = (float)qz * cs;
+      const float dz_lo = fmaxf(fmaxf(zlo - zs, zs - (zlo + cs)) - bmargin, 0.0f);
       for (int oy = -1; oy <= 1; ++oy) {
         int qy = cy + oy, sy = 0;
         if (qy < 0) { qy += nc; sy = -1; } else if (qy >= nc) { qy -= nc; sy = 1; }
-        const double gy = oy == 0 ? 0.0 : gap(pi.y, qy, (double)sy * L);
-        if (gy * gy + gz * gz > cutsq) continue;
         const float ys = (float)(pi.y - (double)sy * L);
-        for (int ox = -1; ox <= 1; ++ox) {
-          int qx = cx + ox, sx = 0;
-          if (qx < 0) { qx += nc; sx = -1; } else if (qx >= nc) { qx -= nc; sx = 1; }
-          const double gx = ox == 0 ? 0.0 : gap(pi.x, qx, (double)sx * L);
-          if (gx * gx + gy * gy + gz * gz > cutsq) continue;
-          const float xs = (float)(pi.x - (double)sx * L);
-          const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
-          const bool unshifted = (sx | sy | sz) == 0;
-          const double shx = (double)sx * L, shy = (double)sy * L, shz = (double)sz * L;
-          const int c2 = d.cell0 + qx + nc * (qy + nc * qz);
-          const int jb = s0 + cstart[c2], je = s0 + cstart[c2 + 1];
-          // Candidates eight at a time, their FP32 copies loaded together; FP32
-          // r^2 settles all but a thin window around the cut, where the
-          // reference's FP64 test decides.
-          for (int j0 = jb; j0 < je; j0 += 8) {
-            float4 qc[8];
+        const float ylo = (float)qy * cs;
+        const float dy_lo = fmaxf(fmaxf(ylo - ys, ys - (ylo + cs)) - bmargin, 0.0f);
+        const float rx2 = top2 - __builtin_fmaf(dy_lo, dy_lo, dz_lo * dz_lo);
+        if (!(rx2 > 0.0f)) continue;
+        const float reach = sqrtf(rx2) * 1.0001f;
+        const int row = d.cell0 + (qy + nc * qz) * nc;
+        // x pieces of the row segment (cx-1 .. cx+1), split where the row wraps
+        int pa0, pb0, pa1 = 0, pb1 = -1;
+        if (cx >= 1 && cx + 1 < nc) { pa0 = cx - 1; pb0 = cx + 1; }
+        else if (cx == 0) { pa0 = nc - 1; pb0 = nc - 1; pa1 = 0; pb1 = 1; }
+        else { pa0 = cx - 1; pb0 = nc - 1; pa1 = 0; pb1 = 0; }
+        for (int p = 0; p < 2; ++p) {
+          const int xa = p ? pa1 : pa0, xb = p ? pb1 : pb0;
+          if (xb < xa) break;
+          const int sxp = (cx == 0 && p == 0) ? -1 : ((cx == nc - 1 && p == 1) ? 1 : 0);
+          const float xs = (float)(pi.x - (double)sxp * L);
+          const int jb = s0 + cstart[row + xa], je0 = s0 + cstart[row + xb + 1];
+          int ja, je;
+          {
+            const float ulo = (xs - reach - bmargin) * inv_cs;
+            const float uhi = (xs + reach + bmargin) * inv_cs;
+            if (!(ulo > (float)xa)) {
+              ja = jb;
+            } else if (ulo >= (float)(xb + 1)) {
+              ja = je0;
+            } else {
+              const int cc = min((int)floorf(ulo), xb);
+              const int kb = min(max((int)floorf((ulo - (float)cc) * (float)kXBins), 0), kXBins - 1);
+              ja = xbin[(size_t)(row + cc) * (kXBins + 1) + kb];
+            }
+            if (!(uhi < (float)(xb + 1))) {
+              je = je0;
+            } else if (uhi < (float)xa) {
+              je = jb;
+            } else {
+              const int cc = max((int)floorf(uhi), xa);
+              const int kb = min(max((int)floorf((uhi - (float)cc) * (float)kXBins), 0) + 1, kXBins);
+              je = xbin[(size_t)(row + cc) * (kXBins + 1) + kb];
+            }
+            je = max(je, ja);
+          }
+          for (int j0 = ja; j0 < je; j0 += kChunk) {
+            const int jn = je - j0;
+            uint32_t in_bits = 0, maybe_bits = 0;
+            // candidates kChunk-1 .. 0, so that candidate u ends at bit u; each test
+            // shifts in the sign bit of r^2 - cflo and of r^2 - cfhi
 #pragma unroll
-            for (int u = 0; u < 8; ++u) qc[u] = posf[min(j0 + u, je - 1)];
+            for (int u = kChunk - 4; u >= 0; u -= 4) {
+              float4 q[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const int j = j0 + u;
-              if (j >= je) break;
-              if (j == i && unshifted) continue;
-              const float fx = xs - qc[u].x, fy = ys - qc[u].y, fz = zs - qc[u].z;
-              const float f2 = __builtin_fmaf(fx, fx, __builtin_fmaf(fy, fy, fz * fz));
-              bool in = f2 < cflo;
-              if (!in && f2 <= cfhi) {
-                const double4 pj = pos[j];
-                double dx, dy, dz;
-                if (fwd) { dx = pi.x - (pj.x + shx); dy = pi.y - (pj.y + shy); dz = pi.z - (pj.z + shz); }
-                else { dx = pj.x - (pi.x - shx); dy = pj.y - (pi.y - shy); dz = pj.z - (pi.z - shz); }
-                in = dx * dx + dy * dy + dz * dz <= cutsq;
+              for (int v = 0; v < 4; ++v) q[v] = posf[j0 + u + v];
+#pragma unroll
+              for (int v = 3; v >= 0; --v) {
+                const float dx = xs - q[v].x, dy = ys - q[v].y, dz = zs - q[v].z;
+                const float f = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, -cflo)));
+                in_bits = __builtin_amdgcn_alignbit(in_bits, __float_as_uint(f), 31);
+                maybe_bits = __builtin_amdgcn_alignbit(maybe_bits, __float_as_uint(f - win), 31);
               }
-              if (in) {
-                if (k < cap) out[(size_t)k * 64] = j;
-                ++k;
+            }
+            uint32_t valid = jn >= kChunk ? 0xFFFFu : ((1u << jn) - 1u);
+            const int self = i - j0;
+            if (self >= 0 && self < kChunk) valid &= ~(1u << self);
+            in_bits &= valid;
+            uint32_t border = maybe_bits & valid & ~in_bits;
+            while (border) {
+              // the reference's FP64 test in the reference's orientation (rare)
+              const int tb = __ffs(border) - 1;
+              border &= border - 1u;
+              const int j = j0 + tb;
+              const int cjx = cell_of_slot[j] % nc;
+              int ox = cjx - cx, sx = 0;
+              if (ox > 1) { ox = -1; sx = -1; } else if (ox < -1) { ox = 1; sx = 1; }
+              const bool fwd = (oz > 0) || (oz == 0 && (oy > 0 || (oy == 0 && ox >= 0)));
+              const double4 pj = pos[j];
+              const double shx = (double)sx * L, shy = (double)sy * L, shz = (double)sz * L;
+              double ex, ey, ez;
+              if (fwd) { ex = pi.x - (pj.x + shx); ey = pi.y - (pj.y + shy); ez = pi.z - (pj.z + shz); }
+              else { ex = pj.x - (pi.x - shx); ey = pj.y - (pi.y - shy); ez = pj.z - (pi.z - shz); }
+              if (sqr_nc(ex, ey, ez) <= cutsq) in_bits |= 1u << tb;
+            }
+            // up to four entries per pass, so the wavefront's passes follow the
+            // largest count of its lanes divided by four
+            while (in_bits) {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                if (in_bits) {
+                  const int tb = __ffs(in_bits) - 1;
+                  in_bits &= in_bits - 1u;
+                  if (k < cap) out[(size_t)k * 64] = j0 + tb;
+                  ++k;
+                }
               }
             }
           }
@@ -765,6 +861,7 @@ struct espp_batch {
   DBuf<float4> posf;                      // FP32 copy of the positions at the last binning (list build)
   DBuf<double> fx, fy, fz;
   DBuf<int> cell_of_slot, cstart, ccount, tcell, trank, order;
+  DBuf<int> xbin;                         // x sub-bin table of every cell ((kXBins + 1) slots each)
   DBuf<BlockMap> bm_cell;                 // cell blocks of kb_sort_cells
   int nb_cell = 0;
   DBuf<int> nbr, cnt;
@@ -848,15 +945,17 @@ struct espp_batch {
     hipLaunchKernelGGL(kb_scatter, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, cstart.p,
                        tcell.p, trank.p, order.p);
     hipLaunchKernelGGL(kb_sort_cells, dim3(nb_cell), dim3(kPartThreads), 0, stream, bm_cell.p, drd.p, dst.p,
-                       cstart.p, id.p, order.p);
+                       cstart.p, pos.p, id.p, order.p, xbin.p);
     hipLaunchKernelGGL(kb_permute, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, order.p,
                        pos.p, vel.p, id.p, img.p, tcell.p, pos2.p, vel2.p, id2.p, img2.p, cell_of_slot.p);
     hipLaunchKernelGGL(kb_copy_back, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos2.p,
                        vel2.p, id2.p, img2.p, pos.p, vel.p, id.p, img.p, posq.p, posf.p);
+    hipLaunchKernelGGL(kb_xbins, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p,
+                       cell_of_slot.p, posf.p, xbin.p);
   }
   void launch_build() {
     hipLaunchKernelGGL(kb_build, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos.p,
-                       posf.p, cell_of_slot.p, cstart.p, nbr.p, cnt.p, cap);
+                       posf.p, cell_of_slot.p, cstart.p, xbin.p, nbr.p, cnt.p, cap);
   }
   void launch_force() {
     hipLaunchKernelGGL(kb_force<false>, dim3(nb_force), dim3(kPartThreads), 0, stream, bm_force.p, drd.p, dst.p,
@@ -1135,11 +1234,12 @@ int espp_batch_create(int32_t nrep, const int32_t* n, const double* box, double 
     b->id.resize(S + 4); b->id2.resize(S); b->id_snap.resize(S + 4);
     b->img.resize(S); b->img2.resize(S); b->img_snap.resize(S);
     b->posq.resize(S);
-    b->posf.resize(S);
+    b->posf.resize(S + 16);                // the list build reads up to 15 entries past a range
     b->fx.resize(S); b->fy.resize(S); b->fz.resize(S);
     b->cell_of_slot.resize(S); b->tcell.resize(S); b->trank.resize(S); b->order.resize(S);
     b->cstart.resize((size_t)cell);
     b->ccount.resize((size_t)cell);
+    b->xbin.resize((size_t)cell * (kXBins + 1));
     b->cnt.resize(S);
     b->nbr.resize((S / 64) * (size_t)b->cap * 64);
     b->red.resize(2 * (size_t)nrep);

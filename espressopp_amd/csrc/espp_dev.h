@@ -14,6 +14,10 @@ namespace espp {
 constexpr int kWave = 64;          // CDNA wavefront
 constexpr int kMaxTypes = 16;      // particle types with a device parameter table
 
+// x sub-bins per cell of the list builds' row clipping (a cell's table holds
+// kXBins + 1 slots)
+constexpr int kXBins = 16;
+
 // Box and cell-grid geometry, passed by value to kernels.
 struct Geom {
   double L[3], invL[3], halfL[3];

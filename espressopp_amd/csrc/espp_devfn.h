@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "espp_dev.h"
+
 namespace espp {
 
 // |d|^2 summed left to right without contraction (Real3D::sqr, src/Real3D.hpp)
@@ -29,6 +31,14 @@ __device__ __forceinline__ uint2 q16_pack(const double4& p, double q16inv) {
   const uint32_t uy = (uint32_t)(unsigned long long)(long long)__builtin_rint(p.y * q16inv) & 0xFFFFu;
   const uint32_t uz = (uint32_t)(unsigned long long)(long long)__builtin_rint(p.z * q16inv) & 0xFFFFu;
   return make_uint2(ux | (uy << 16), uz);
+}
+
+// x sub-bin of a candidate position for the list builds' row clipping: cell
+// units u = x / cs from the FP32 copy, k = floor((u - cx) kXBins) clamped to the
+// cell (the same arithmetic as the builds' lookups, so both order alike)
+__device__ __forceinline__ int xsub_bin(float x, float inv_cs, int cx) {
+  const float u = x * inv_cs;
+  return min(max((int)floorf((u - (float)cx) * (float)kXBins), 0), kXBins - 1);
 }
 
 __device__ __forceinline__ float rcp_nr(float x) {

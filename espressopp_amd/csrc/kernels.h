@@ -22,7 +22,6 @@ void launch_permute(hipStream_t s, int n, const int* order, const int* cell_of, 
 void launch_posf(hipStream_t s, int n, const double4* pos, float4* posf, uint2* posh, double q16inv);
 // x sub-bins of every cell for the tiled list build (ncell * (kXBins + 1) ints),
 // from the positions the binning has just sorted
-constexpr int kXBins = 16;
 void launch_xbins(hipStream_t s, int n, int ncell, const int* cstart, const int* cell_of_slot, const float4* posf,
                   const Geom& g, int* xbin);
 void launch_fill_int(hipStream_t s, int n, int* a, int v);

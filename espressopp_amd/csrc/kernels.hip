@@ -370,14 +370,6 @@ __global__ void k_permute(int n, const int* __restrict__ order, const int* __res
   if (g.hci > 1) g.hcell[s] = caller_cell(p, g);
 }
 
-// FP32 copy of the positions, the candidates of the tiled list build.
-// x sub-bin of a candidate position for the list build's row clipping: cell
-// units u = x / cs from the FP32 copy, k = floor((u - cx) kXBins) clamped to the
-// cell (the same arithmetic as the build's lookups, so both order alike).
-__device__ __forceinline__ int xsub_bin(float x, float inv_cs, int cx) {
-  const float u = x * inv_cs;
-  return min(max((int)floorf((u - (float)cx) * (float)kXBins), 0), kXBins - 1);
-}
 
 // Per cell c, xbin[c (kXBins+1) + k] is the first slot of the cell whose x
 // sub-bin is at least k (k < kXBins), and the cell's end for k = kXBins.  The
