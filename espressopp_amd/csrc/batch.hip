@@ -63,8 +63,9 @@ struct RepDesc {
   double ulo, uhi, uff1, uff2, usig2;   // dlo, dhi, ff1, ff2, sig2 in fixed-point units (force launch)
   double pref1, pref2;                  // -gamma, sqrt(24 T gamma / dt)
   unsigned long long seed;
+  double q16inv;                        // 1 / delta of the list build's 16-bit copies (0: FP32 prefilter)
   int slot0, n, cell0, nc;
-  int lang, pad0, pad1, pad2;
+  int lang, q16_tlo, q16_w, pad0;       // r^2 thresholds of the 16-bit prefilter (q16_thresholds)
 };
 
 struct RepState {
@@ -80,6 +81,7 @@ struct BlockMap {
 };
 
 // ------------------------------------------------------------------ device
+
 
 __device__ __forceinline__ void thermostat(int pid, const double4& v, double& f0, double& f1, double& f2,
                                            double pref1, double pref2, unsigned long long seed,
@@ -406,7 +408,8 @@ __global__ __launch_bounds__(kPartThreads) void kb_permute(const BlockMap* __res
   cell_of_slot[s] = tcell[o];
 }
 
-// ... and back into the primary buffers, with the fixed-point mirror.
+// ... and back into the primary buffers, with the fixed-point mirror, the FP32
+// copy and, where the replica's box allows them, the 16-bit copies.
 __global__ __launch_bounds__(kPartThreads) void kb_copy_back(const BlockMap* __restrict__ bm,
                                                              const RepDesc* __restrict__ rd,
                                                              const RepState* __restrict__ st,
@@ -416,7 +419,7 @@ __global__ __launch_bounds__(kPartThreads) void kb_copy_back(const BlockMap* __r
                                                              const int4* __restrict__ img2, double4* __restrict__ pos,
                                                              double4* __restrict__ vel, int* __restrict__ id,
                                                              int4* __restrict__ img, uint4* __restrict__ posq,
-                                                             float4* __restrict__ posf) {
+                                                             float4* __restrict__ posf, uint2* __restrict__ posh) {
   const BlockMap m = bm[blockIdx.x];
   const int r = m.rep;
   if (!st[r].rebuild || threadIdx.x >= (unsigned)m.count) return;
@@ -425,6 +428,7 @@ __global__ __launch_bounds__(kPartThreads) void kb_copy_back(const BlockMap* __r
   const double qinv = rd[r].qinv;
   pos[s] = p;
   posf[s] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+  if (rd[r].q16inv > 0.0) posh[s] = q16_pack(p, rd[r].q16inv);
   vel[s] = vel2[s];
   id[s] = id2[s];
   img[s] = img2[s];
@@ -476,8 +480,8 @@ __global__ __launch_bounds__(kPartThreads) void kb_xbins(const BlockMap* __restr
 __global__ __launch_bounds__(kPartThreads) __attribute__((amdgpu_waves_per_eu(6))) void kb_build(
     const BlockMap* __restrict__ bm, const RepDesc* __restrict__ rd, RepState* __restrict__ st,
     const double4* __restrict__ pos, const float4* __restrict__ posf, const int* __restrict__ cell_of_slot,
-    const int* __restrict__ cstart, const int* __restrict__ xbin, int* __restrict__ nbr, int* __restrict__ cnt,
-    int cap) {
+    const int* __restrict__ cstart, const int* __restrict__ xbin, const uint2* __restrict__ posh,
+    int* __restrict__ nbr, int* __restrict__ cnt, int cap) {
   constexpr int kChunk = 16;
   const BlockMap m = bm[blockIdx.x];
   const int r = m.rep;
@@ -498,6 +502,16 @@ __global__ __launch_bounds__(kPartThreads) __attribute__((amdgpu_waves_per_eu(6)
     // 1e-4 of a cell covers the FP32 rounding of the bounds and of the sub-bins
     const float bmargin = 1e-4f * cs;
     const float top = sqrtf(cfhi) * 1.0001f, top2 = top * top;
+    // the 16-bit prefilter (replicas whose box allows it): particle i's copy, the
+    // same for every row, and the thresholds
+    const bool q16 = d.q16inv > 0.0;
+    v2u16 lxy = {0, 0}, lz = {0, 0};
+    if (q16) {
+      const uint2 h = q16_pack(pi, d.q16inv);
+      lxy = __builtin_bit_cast(v2u16, h.x);
+      lz = __builtin_bit_cast(v2u16, h.y);
+    }
+    const int neg_tlo = -d.q16_tlo, q16_w = d.q16_w;
     for (int oz = -1; oz <= 1; ++oz) {
       int qz = cz + oz, sz = 0;
       if (qz < 0) { qz += nc; sz = -1; } else if (qz >= nc) { qz -= nc; sz = 1; }
@@ -554,18 +568,36 @@ __global__ __launch_bounds__(kPartThreads) __attribute__((amdgpu_waves_per_eu(6)
             const int jn = je - j0;
             uint32_t in_bits = 0, maybe_bits = 0;
             // candidates kChunk-1 .. 0, so that candidate u ends at bit u; each test
-            // shifts in the sign bit of r^2 - cflo and of r^2 - cfhi
+            // shifts in the sign bit of r^2 below the window (in) and below its top
+            if (q16) {
+              // 16-bit units: two packed differences and two dot products
 #pragma unroll
-            for (int u = kChunk - 4; u >= 0; u -= 4) {
-              float4 q[4];
+              for (int u = kChunk - 4; u >= 0; u -= 4) {
+                uint2 q[4];
 #pragma unroll
-              for (int v = 0; v < 4; ++v) q[v] = posf[j0 + u + v];
+                for (int v = 0; v < 4; ++v) q[v] = posh[j0 + u + v];
 #pragma unroll
-              for (int v = 3; v >= 0; --v) {
-                const float dx = xs - q[v].x, dy = ys - q[v].y, dz = zs - q[v].z;
-                const float f = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, -cflo)));
-                in_bits = __builtin_amdgcn_alignbit(in_bits, __float_as_uint(f), 31);
-                maybe_bits = __builtin_amdgcn_alignbit(maybe_bits, __float_as_uint(f - win), 31);
+                for (int v = 3; v >= 0; --v) {
+                  const v2i16 dxy = __builtin_bit_cast(v2i16, lxy - __builtin_bit_cast(v2u16, q[v].x));
+                  const v2i16 dz = __builtin_bit_cast(v2i16, lz - __builtin_bit_cast(v2u16, q[v].y));
+                  const int f = sdot2_acc(dz, sdot2_sacc(dxy, neg_tlo));
+                  in_bits = __builtin_amdgcn_alignbit(in_bits, (uint32_t)f, 31);
+                  maybe_bits = __builtin_amdgcn_alignbit(maybe_bits, (uint32_t)(f - q16_w), 31);
+                }
+              }
+            } else {
+#pragma unroll
+              for (int u = kChunk - 4; u >= 0; u -= 4) {
+                float4 q[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) q[v] = posf[j0 + u + v];
+#pragma unroll
+                for (int v = 3; v >= 0; --v) {
+                  const float dx = xs - q[v].x, dy = ys - q[v].y, dz = zs - q[v].z;
+                  const float f = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, -cflo)));
+                  in_bits = __builtin_amdgcn_alignbit(in_bits, __float_as_uint(f), 31);
+                  maybe_bits = __builtin_amdgcn_alignbit(maybe_bits, __float_as_uint(f - win), 31);
+                }
               }
             }
             uint32_t valid = jn >= kChunk ? 0xFFFFu : ((1u << jn) - 1u);
@@ -859,6 +891,7 @@ struct espp_batch {
   DBuf<int4> img, img2, img_snap;
   DBuf<uint4> posq;
   DBuf<float4> posf;                      // FP32 copy of the positions at the last binning (list build)
+  DBuf<uint2> posh;                       // 16-bit copies of the same (q16_pack; replicas with q16inv > 0)
   DBuf<double> fx, fy, fz;
   DBuf<int> cell_of_slot, cstart, ccount, tcell, trank, order;
   DBuf<int> xbin;                         // x sub-bin table of every cell ((kXBins + 1) slots each)
@@ -949,13 +982,13 @@ struct espp_batch {
     hipLaunchKernelGGL(kb_permute, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, order.p,
                        pos.p, vel.p, id.p, img.p, tcell.p, pos2.p, vel2.p, id2.p, img2.p, cell_of_slot.p);
     hipLaunchKernelGGL(kb_copy_back, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos2.p,
-                       vel2.p, id2.p, img2.p, pos.p, vel.p, id.p, img.p, posq.p, posf.p);
+                       vel2.p, id2.p, img2.p, pos.p, vel.p, id.p, img.p, posq.p, posf.p, posh.p);
     hipLaunchKernelGGL(kb_xbins, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p,
                        cell_of_slot.p, posf.p, xbin.p);
   }
   void launch_build() {
     hipLaunchKernelGGL(kb_build, dim3(nb_part), dim3(kPartThreads), 0, stream, bm_part.p, drd.p, dst.p, pos.p,
-                       posf.p, cell_of_slot.p, cstart.p, xbin.p, nbr.p, cnt.p, cap);
+                       posf.p, cell_of_slot.p, cstart.p, xbin.p, posh.p, nbr.p, cnt.p, cap);
   }
   void launch_force() {
     hipLaunchKernelGGL(kb_force<false>, dim3(nb_force), dim3(kPartThreads), 0, stream, bm_force.p, drd.p, dst.p,
@@ -1201,6 +1234,11 @@ int espp_batch_create(int32_t nrep, const int32_t* n, const double* box, double 
       const double cwin = 1e-6 + 4.0 * std::sqrt(3.0) * ulpM / cutV;
       d.cflo = d.cutsq * (1.0 - cwin);
       d.cfhi = d.cutsq * (1.0 + cwin);
+      // the 16-bit prefilter where the box allows it (the list is built right after
+      // the binning, so with no drift)
+      d.q16inv = q16_scale_cubic(box[r], d.nc, 0.0, std::sqrt((double)(float)d.cfhi) * 1.0001);
+      d.q16_tlo = d.q16_w = 0;
+      if (d.q16inv > 0.0) q16_thresholds(cutV, d.q16inv, d.q16_tlo, d.q16_w);
       // the force launch works in fixed-point units: r^2 = u^2 q^2 (q = L / 2^32),
       // so ff(r^2) = u^-8 (ff1 q^-14 u^-6 - ff2 q^-8) and F = u q ff
       const double q2 = d.qsd * d.qsd;
@@ -1235,6 +1273,7 @@ int espp_batch_create(int32_t nrep, const int32_t* n, const double* box, double 
     b->img.resize(S); b->img2.resize(S); b->img_snap.resize(S);
     b->posq.resize(S);
     b->posf.resize(S + 16);                // the list build reads up to 15 entries past a range
+    b->posh.resize(S + 16);
     b->fx.resize(S); b->fy.resize(S); b->fz.resize(S);
     b->cell_of_slot.resize(S); b->tcell.resize(S); b->trank.resize(S); b->order.resize(S);
     b->cstart.resize((size_t)cell);

@@ -33,6 +33,23 @@ __device__ __forceinline__ uint2 q16_pack(const double4& p, double q16inv) {
   return make_uint2(ux | (uy << 16), uz);
 }
 
+typedef short v2i16 __attribute__((ext_vector_type(2)));
+typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
+
+// a.lo^2 + a.hi^2 + c (signed 16-bit halves), the accumulator in a scalar or a
+// vector register: the three-operand v_dot2_i32_i16, which the compiler would
+// replace by a copy of c and the accumulating two-operand form
+__device__ __forceinline__ int sdot2_sacc(v2i16 a, int c) {
+  int r;
+  asm("v_dot2_i32_i16 %0, %1, %1, %2" : "=v"(r) : "v"(a), "s"(c));
+  return r;
+}
+__device__ __forceinline__ int sdot2_acc(v2i16 a, int c) {
+  int r;
+  asm("v_dot2_i32_i16 %0, %1, %1, %2" : "=v"(r) : "v"(a), "v"(c));
+  return r;
+}
+
 // x sub-bin of a candidate position for the list builds' row clipping: cell
 // units u = x / cs from the FP32 copy, k = floor((u - cx) kXBins) clamped to the
 // cell (the same arithmetic as the builds' lookups, so both order alike)

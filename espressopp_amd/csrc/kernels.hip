@@ -1047,23 +1047,6 @@ __device__ __forceinline__ size_t tiled_group(int i, int capg, int g) {
 // short block wastes less of a row's clipped range at its end
 constexpr int kCandChunk = 16;
 constexpr uint32_t kCandAll = (1u << kCandChunk) - 1u;
-typedef short v2i16 __attribute__((ext_vector_type(2)));
-typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
-
-// a.lo^2 + a.hi^2 + c (signed 16-bit halves) with the accumulator c in a scalar
-// register: the three-operand v_dot2_i32_i16, which the compiler would replace
-// by a copy of c and the accumulating two-operand form
-__device__ __forceinline__ int sdot2_sacc(v2i16 a, int c) {
-  int r;
-  asm("v_dot2_i32_i16 %0, %1, %1, %2" : "=v"(r) : "v"(a), "s"(c));
-  return r;
-}
-// the same with the accumulator in a vector register
-__device__ __forceinline__ int sdot2_acc(v2i16 a, int c) {
-  int r;
-  asm("v_dot2_i32_i16 %0, %1, %1, %2" : "=v"(r) : "v"(a), "v"(c));
-  return r;
-}
 
 // entries of a lane's LDS staging buffer in the list build (pending and new entries
 // of one block)

@@ -34,22 +34,11 @@ namespace {
 
 constexpr size_t kPosqPad = 64;   // spare fixed-point mirror entries after the last particle
 
-// Scale 1 / delta of the list build's 16-bit copies (q16_pack), or 0 where they
-// do not apply (a grid that is not cubic, or a box too small for one period).
-// delta = L / (2^16 m) keeps a box length a whole number of 2^16-unit periods,
-// so a 16-bit difference is that of the periodic image within 2^15 units.  m is
-// the largest whole number for which every candidate of a particle's rows stays
-// below 32000 units per coordinate (in x the clipped stretch: one cell, an
-// eighth of slack and the drift terms; in y and z the two cells across the row
-// plus twice the drift) and the three squares sum below 46000^2 < 2^31.
-double q16_scale(const Geom& g, double disp) {
+// Scale of the list build's 16-bit copies for the grid (q16_scale_cubic; 0 for
+// a grid that is not cubic).
+double q16_scale(const Geom& g, double disp, double top) {
   if (!(g.L[0] == g.L[1] && g.L[1] == g.L[2] && g.n[0] == g.n[1] && g.n[1] == g.n[2])) return 0.0;
-  const double cs = g.L[0] / g.n[0];
-  const double bx = cs * 1.0012 + cs / 8.0 + 4.0 * disp;
-  const double byz = 2.0 * cs * 1.0001 + 2.0 * disp;
-  const double dmin = std::max(std::max(bx, byz) / 32000.0, std::sqrt(bx * bx + 2.0 * byz * byz) / 46000.0);
-  const double m = std::floor(g.L[0] / (65536.0 * dmin));
-  return m >= 1.0 ? 65536.0 * m / g.L[0] : 0.0;
+  return q16_scale_cubic(g.L[0], g.n[0], disp, top);
 }
 
 // Device scratch of espp_ia_energy_noalloc: per-workgroup partial sums of an
@@ -673,23 +662,13 @@ struct espp_sim {
     const float row_margin = bin_folded ? (float)(disp_since_bin * (1.0 + 1e-6) + 4.0 * ulp) : -1.0f;
     // 16-bit candidate copies (cubic grids, positions folded at the binning) with
     // a scale for the drift since the binning, or FP32 ones
-    const bool q16_fits = std::sqrt((double)cut_hi_q) * 1.0001 <= 1.001 * g.L[0] / g.n[0];   // the x bound of q16_scale
-    const double q16 = (v.tiled && bin_folded && q16_fits) ? q16_scale(g, disp_since_bin) : 0.0;
+    const double q16 = (v.tiled && bin_folded) ? q16_scale(g, disp_since_bin, std::sqrt((double)cut_hi_q) * 1.0001)
+                                               : 0.0;
     // the candidate copies are the binning's until a particle moves
     if (v.tiled && (disp_since_bin > 0.0 || (q16 > 0.0 && q16 != g.q16inv)))
       launch_posf(stream, (int)n, pos.p, posf.p, posh.p, q16);
-    // Thresholds of r^2 in 16-bit units (U = (rc+skin) / delta): each coordinate
-    // of a copy is within half a unit of the position, so a difference is off by
-    // at most one unit and r^2 by at most 2 sqrt(3) |d| + 3 units^2.  Below q16_tlo
-    // the pair is surely within rc+skin, at or above q16_tlo + q16_w surely not.
     int q16_tlo = 0, q16_w = 0;
-    if (q16 > 0.0) {
-      const double U = cutVerlet * q16, U2 = U * U;
-      const double tlo = std::floor(U2 - 2.0 * std::sqrt(3.0) * U * 1.0001 - 8.0);
-      const double thi = std::ceil(U2 + 2.0 * std::sqrt(3.0) * U * 1.01 + 16.0);
-      q16_tlo = (int)std::max(tlo, 0.0);
-      q16_w = (int)(thi - q16_tlo);
-    }
+    if (q16 > 0.0) q16_thresholds(cutVerlet, q16, q16_tlo, q16_w);
     for (int attempt = 0; attempt < 4; ++attempt) {
       launch_reset_ctl(stream, ctl.p, kCtlBuild);
       hipEvent_t pe = prof_begin();
@@ -1305,7 +1284,8 @@ void set_geom(Geom& g, const double box[3], const int32_t cells[3], int hci) {
   }
   g.qs2 = g.qsd[0] * g.qsd[0];
   g.qsf = (float)g.qsd[0];
-  g.q16inv = q16_scale(g, 0.0);
+  // at the binning the list's cut is not known: the largest a cell allows
+  g.q16inv = q16_scale(g, 0.0, g.L[0] / g.n[0]);
 }
 
 template <class T>

@@ -64,8 +64,11 @@ def by_id(ref, field):
 
 @pytest.mark.parametrize("lpp", [1, 4])
 def test_batch_pairs_forces_energy_match_oracle(gpu, lpp):
-    states = [replica_state(n, 10 + k) for k, n in enumerate((2048, 3375, 5000))]
-    b = make_batch(states, [1.0] * 3, gamma=0.0, lpp=[lpp] * 3)
+    # the list build's prefilter: FP32 at N = 2048 (box too small for the 16-bit copies),
+    # 16-bit copies with one period per box (3375, 5000) and with two (24000)
+    sizes = (2048, 3375, 5000, 24000)
+    states = [replica_state(n, 10 + k) for k, n in enumerate(sizes)]
+    b = make_batch(states, [1.0] * len(sizes), gamma=0.0, lpp=[lpp] * len(sizes))
     b.run(0, DT)
     e, w = b.energy()
     for r, st in enumerate(states):
