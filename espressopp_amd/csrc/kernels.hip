@@ -1405,7 +1405,8 @@ __device__ __forceinline__ void step_block_max(double sq, const StepArgs& a) {
 // appending to a 128-bit shift register as the list build does; then the
 // pruned lengths rank the tile's lanes for the walks of the pruned list.
 template <bool ONETYPE, bool ENERGY, bool ACCUM, bool CUBIC, bool STEP, bool PRUNE = false>
-__global__ __launch_bounds__(256) void k_lj_tiled_mixed(
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(ONETYPE && STEP && !PRUNE && !ENERGY ? 4 : 1))) void k_lj_tiled_mixed(
     int n, int ntiles, const uint4* __restrict__ posq, const double4* __restrict__ pos,
     const uint4* __restrict__ nbr, const int* __restrict__ cnt, int capg, Geom g,
     const TileRun* __restrict__ runs, const int* __restrict__ nruns, const int* __restrict__ lane_slot,
