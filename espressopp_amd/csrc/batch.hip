@@ -345,20 +345,20 @@ __global__ __launch_bounds__(kPartThreads) void kb_sort_cells(const BlockMap* __
       continue;
     }
     int ent[4], key[4];
-    double kx[4];
+    long long kx[4];                              // x_key of the folded x
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int k = lane + 64 * u;
       ent[u] = k < mlen ? order[s0 + b + k] : 0;
       key[u] = k < mlen ? id[s0 + ent[u]] : INT_MAX;
-      kx[u] = k < mlen ? pos[s0 + ent[u]].x : __builtin_inf();
+      kx[u] = k < mlen ? x_key(pos[s0 + ent[u]].x) : LLONG_MAX;
     }
     int rk[4] = {0, 0, 0, 0};
     if (mlen <= 64) {
       // one entry per lane (the common case)
       for (int q = 0; q < mlen; ++q) {
         const int other = __shfl(key[0], q, 64);
-        const double ox = __shfl(kx[0], q, 64);
+        const long long ox = __shfl(kx[0], q, 64);
         rk[0] += ox < kx[0] || (ox == kx[0] && (other < key[0] || (other == key[0] && q < lane)));
       }
     } else {
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(kPartThreads) void kb_sort_cells(const BlockMap* __
         if (64 * u2 >= mlen) break;     // wavefront-uniform
         for (int q = 0; q < 64; ++q) {
           const int other = __shfl(key[u2], q, 64);
-          const double ox = __shfl(kx[u2], q, 64);
+          const long long ox = __shfl(kx[u2], q, 64);
           const int ok = q + 64 * u2;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {

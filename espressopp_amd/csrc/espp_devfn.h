@@ -50,6 +50,13 @@ __device__ __forceinline__ int sdot2_acc(v2i16 a, int c) {
   return r;
 }
 
+// Order-preserving integer image of a double (negative values reflected), a
+// total order with NaN included, so ranks by comparison always form a permutation
+__device__ __forceinline__ long long x_key(double x) {
+  const long long b = __double_as_longlong(x);
+  return b ^ ((b >> 63) & 0x7fffffffffffffffll);
+}
+
 // x sub-bin of a candidate position for the list builds' row clipping: cell
 // units u = x / cs from the FP32 copy, k = floor((u - cx) kXBins) clamped to the
 // cell (the same arithmetic as the builds' lookups, so both order alike)

@@ -213,13 +213,8 @@ __global__ void k_scatter(int n, const int* __restrict__ cell_of, const int* __r
 // which lets the list build clip a row to the candidates within reach in x.
 constexpr int kSortLanes = 32, kSortCellsPerBlock = 8, kSortMax = 96;
 
-// Sort key of a coordinate: the bits of the double mapped to a signed integer
-// whose order is the numeric order.  It is a total order (NaNs included), so
+// The sort key is x_key of the coordinate (a total order, NaNs included), so
 // the ranks below always form a permutation.
-__device__ __forceinline__ long long x_key(double x) {
-  const long long b = __double_as_longlong(x);
-  return b ^ ((b >> 63) & 0x7fffffffffffffffll);
-}
 
 __global__ __launch_bounds__(kSortLanes * kSortCellsPerBlock) void k_sort_cells(
     int ncell, const int* __restrict__ start, int* __restrict__ order, const double4* __restrict__ pos) {
