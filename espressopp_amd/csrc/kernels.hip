@@ -1060,16 +1060,19 @@ __device__ __forceinline__ uint4 ring_group(const unsigned short* r, int p0) {
 // Tiled Verlet-list build (cell grids with at least 3 cells per dimension).
 // For each of its 9 neighbour x-rows a particle scans the candidates of the
 // three cells (cx-1, cx, cx+1), one contiguous slot range or two when the row
-// wraps, clipped in x by binary search (cells are sorted by x) and read from an
-// FP32 copy of the positions (lanes of one cell read the same candidates, so
-// the loads broadcast; the copy is padded so reads past a range stay inside
-// it).  Particle i's coordinates are shifted by each piece's periodic image
-// once, so a candidate costs three FP32 differences.  For kCandChunk candidates
-// at a time two bit masks are formed without branches: FP32 r^2 below the
-// window (surely in) and below its top (in or undecided).  Undecided pairs, within a small window
-// of (rc+skin)^2, are settled by the reference's FP64 test in the reference's
-// orientation (see k_nbr_build).  Accepted candidates are appended as their
-// offset in the tile's staged region, eight entries per 16-byte store.
+// wraps.  A row whose y/z range lies beyond the prefilter's top is skipped, the
+// others keep the x reach the y/z distance leaves (when the binning folded the
+// positions), and the range is cut to that reach with the cells' x sub-bins
+// (cells are sorted by x).  Candidates come from a copy of the positions padded
+// so that reads past a range stay inside it: with Q16 the 16-bit modular copy
+// (q16_pack; the packed differences pick the periodic image themselves), else
+// FP32 with particle i shifted by each piece's periodic image once.  For
+// kCandChunk candidates at a time two bit masks are formed without branches:
+// r^2 below the window (surely in) and below its top (in or undecided).
+// Undecided pairs, within a small window of (rc+skin)^2, are settled by the
+// reference's FP64 test in the reference's orientation (see k_nbr_build).
+// Accepted candidates are appended, without a branch, as their offset in the
+// tile's staged region, eight entries per 16-byte store.
 template <bool HAS_EXCL, bool Q16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_nbr_build_tiled(
     int n, int ntiles, const float4* __restrict__ posf, const double4* __restrict__ pos, const int* __restrict__ id,
