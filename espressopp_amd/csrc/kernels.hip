@@ -1030,6 +1030,25 @@ __device__ __forceinline__ int xcd_tile(int b, int ntiles) {
   return x * q + min(x, r) + k;
 }
 
+// lds[e] for the list entry e held in the low (HI false) or high half of the
+// word w: one v_mad_u32_u16 reads the half and forms the LDS byte address
+// e * sizeof(S) + lds, where a mask or shift and a multiply-add would take two.
+template <bool HI, typename S>
+__device__ __forceinline__ S lds_entry(const S* lds, uint32_t w) {
+  typedef __attribute__((address_space(3))) const uint32_t* lds_word;
+  static_assert(sizeof(S) % 4 == 0, "slots of whole words");
+  const uint32_t base = (uint32_t)(uintptr_t)(lds_word)(const void*)lds;
+  uint32_t a;
+  if (HI) asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,0,0,0]" : "=v"(a) : "v"(w), "n"((int)sizeof(S)), "v"(base));
+  else asm("v_mad_u32_u16 %0, %1, %2, %3" : "=v"(a) : "v"(w), "n"((int)sizeof(S)), "v"(base));
+  const lds_word p = (lds_word)(uintptr_t)a;
+  S r;
+  uint32_t* rw = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(S) / 4); ++k) rw[k] = p[k];
+  return r;
+}
+
 // Tiled list layout: entry q of particle i is 16-bit half (q & 1) of word
 // ((q >> 1) & 3) of the uint4 at ((i >> 6) * capg + (q >> 3)) * 64 + (i & 63);
 // a lane reads or writes eight entries with one 16-byte access and the 64 lanes
@@ -1484,10 +1503,7 @@ __attribute__((amdgpu_waves_per_eu(ONETYPE && STEP && !PRUNE && !ENERGY ? 4 : 1)
       }
       Slot qa[kBatch];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int e = (u & 1) ? (int)(w[u >> 1] >> 16) : (int)(w[u >> 1] & 0xFFFFu);
-        qa[u] = lds[e];
-      }
+      for (int u = 0; u < kBatch; ++u) qa[u] = (u & 1) ? lds_entry<true>(lds, w[u >> 1]) : lds_entry<false>(lds, w[u >> 1]);
       bool anynear = false;
       uint32_t keep = 0;                           // PRUNE: entries of the batch that stay
 #pragma unroll
