@@ -202,6 +202,7 @@ def c4_batched_pt(steps=500, every=100, seed=2026):
     lib.espp_device_synchronize()
     graph_ms = (time.perf_counter() - t1) / every * 1e3
     force_ms = b.force_time(20)
+    build_ms = b.build_time(10)
     accepted = sum(1 for e in pt.log if e[4])
     resorts = sum(b.stats(r)["resorts"] for r in range(64))
     return {"workload": "64 LJ replicas, N uniform in [2048, 65536] (seed %d), %d particles, rho 0.8442, "
@@ -209,7 +210,7 @@ def c4_batched_pt(steps=500, every=100, seed=2026):
                         % (seed, total, every),
             "value": total * steps / dt, "unit": "particle-steps/s (all replicas)", "steps": steps,
             "ms_per_step": dt / steps * 1e3, "ms_per_step_graph": graph_ms, "ms_per_step_direct_launch": direct_ms,
-            "force_launch_ms": force_ms, "autotune": {"lpp_by_replica": tune["lpp"].tolist(),
+            "force_launch_ms": force_ms, "build_launch_ms_all_replicas": build_ms, "autotune": {"lpp_by_replica": tune["lpp"].tolist(),
                                                       "table_ms_class_by_lpp_1_2_4_8": tune["table_ms"].tolist(),
                                                       "before_ms": tune["before_ms"], "after_ms": tune["after_ms"]},
             "exchanges_attempted": len(pt.log), "exchanges_accepted": accepted, "resorts": resorts,

@@ -93,6 +93,7 @@ SIGNATURES = {
     "espp_batch_set_lpp": [_vp, _vp],
     "espp_batch_get_lpp": [_vp, _vp],
     "espp_batch_force_time": [_vp, _c_int, _P(_c_dbl)],
+    "espp_batch_build_time": [_vp, _c_int, _P(_c_dbl)],
     "espp_batch_autotune": [_vp, _c_int, _vp, _vp, _P(_c_dbl), _P(_c_dbl)],
     "espp_batch_set_graph": [_vp, _c_int],
     "espp_batch_stats": [_vp, _c_int, _P(_c_i64), _P(_c_i64), _P(_c_int)],

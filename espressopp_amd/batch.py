@@ -131,6 +131,13 @@ class ReplicaBatch:
         self._call("espp_batch_force_time", int(reps), ctypes.byref(ms))
         return ms.value
 
+    def build_time(self, reps=10):
+        """Average time (ms) of one list build of all replicas from one binning
+        (every replica is rebuilt and binned once first)."""
+        ms = ctypes.c_double(0.0)
+        self._call("espp_batch_build_time", int(reps), ctypes.byref(ms))
+        return ms.value
+
     def autotune(self, reps=20):
         """Choose lanes per particle per size class; returns a report dict."""
         lpp = np.zeros(self.R, dtype=np.int32)

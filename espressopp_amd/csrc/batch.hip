@@ -1153,6 +1153,27 @@ double time_force(espp_batch* b, int reps) {
   return (double)ms / reps;
 }
 
+// Time of one list build over every replica: every replica is rebuilt (a
+// requested rebuild) and binned once, one build is untimed, then reps builds
+// from that binning are timed with HIP events.
+double time_build(espp_batch* b, int reps) {
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  b->launch_rep_op(2);
+  b->launch_check_bin(0);
+  b->launch_build();
+  hip_check(hipEventRecord(e0, b->stream), "hipEventRecord");
+  for (int k = 0; k < reps; ++k) b->launch_build();
+  hip_check(hipEventRecord(e1, b->stream), "hipEventRecord");
+  hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)ms / reps;
+}
+
 espp_batch* checked(espp_batch* b) {
   if (!b) fail(ESPP_ERR_ARG, "NULL batch handle");
   return b;
@@ -1527,6 +1548,17 @@ int espp_batch_force_time(espp_batch* b, int32_t reps, double* ms) {
     b->check_all_loaded();
     if (b->need_bin) b->start_forces();
     *ms = time_force(b, reps);
+  });
+}
+
+// Average time of one list build over all replicas (ms), from one binning.
+int espp_batch_build_time(espp_batch* b, int32_t reps, double* ms) {
+  return guarded([&] {
+    checked(b);
+    need(ms != nullptr && reps >= 1, "bad argument");
+    b->check_all_loaded();
+    if (b->need_bin) b->start_forces();
+    *ms = time_build(b, reps);
   });
 }
 

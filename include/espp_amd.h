@@ -357,6 +357,12 @@ int espp_batch_get_lpp(espp_batch *b, int32_t *lpp);
 /* Average time (ms) of one force launch over all replicas, over reps launches. */
 int espp_batch_force_time(espp_batch *b, int32_t reps, double *ms);
 
+/* Average time (ms) of one list build of all replicas, over reps builds from one
+ * binning: every replica is rebuilt (as a requested rebuild does, counted in its
+ * builds) and binned once first.  No reference counterpart: a measurement of the
+ * batched engine's list kernel. */
+int espp_batch_build_time(espp_batch *b, int32_t reps, double *ms);
+
 /* Autotuner of the force launch: replicas in four size classes (quartiles of
  * N), each class trying 1, 2, 4 and 8 lanes per particle with HIP-event timing;
  * the fastest shape is kept.  lpp_out (nrep), table (16 times, class-major),

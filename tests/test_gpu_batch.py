@@ -131,6 +131,22 @@ def test_batch_graph_equals_direct_launches(gpu):
             assert np.array_equal(x, y)
 
 
+def test_batch_build_time_keeps_pair_sets(gpu):
+    # the build timing rebins every replica and builds its lists several times from
+    # that binning: the lists keep their pair sets, and a run continues afterwards
+    states = [replica_state(n, 40 + k) for k, n in enumerate((2048, 5000, 24000))]
+    b = make_batch(states, [1.0, 1.1, 1.2], gamma=0.0)
+    b.run(0, DT)
+    before = [b.pairs(r) for r in range(3)]
+    ms = b.build_time(3)
+    print("build of 3 replicas: %.3f ms" % ms, flush=True)
+    assert ms > 0.0
+    for r in range(3):
+        assert np.array_equal(b.pairs(r), before[r])
+    b.run(10, DT)
+    assert all(np.isfinite(b.get(r, "pos")).all() for r in range(3))
+
+
 def c4_replicas(count=64, seed=2026):
     rng = np.random.default_rng(seed)
     ns = rng.integers(2048, 65536, size=count, endpoint=True)
